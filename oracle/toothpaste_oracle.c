@@ -1,0 +1,782 @@
+/*
+ * CPU oracle for the CGGI/BMMP bootstrapping path over the Goldilocks prime.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker the HIP library is
+ * compared against: only the tests, the smoke check and the CPU-baseline leg of
+ * the benchmark load it.  Nothing in the product links it.
+ *
+ * It restates the reference algorithm (LSchild/FHEToothpaste) in plain C with
+ * exact arithmetic modulo Q = 2^64 - 2^32 + 1, step by step in the reference's
+ * own operation order (NTT-domain monomial products, digit order, Rep2 rows),
+ * so that a GPU implementation that reorganises the computation can be checked
+ * for bit equality.  Each function cites the reference file:line it follows.
+ *
+ * Pinning (what makes this restatement trustworthy, see tests/test_oracle_*):
+ *   - field ops against unsigned __int128 '%' (textbook) on edge values;
+ *   - NTT against the O(N^2) closed form out[k] = sum_j a_j psi^((2 brv(k)+1) j);
+ *   - NTT against schoolbook negacyclic convolution (convolution theorem);
+ *   - decomposers against the recomposition identity sum_j g_j d_j;
+ *   - blind rotation with sigma = 0 keys against the exact phase relation
+ *     phase(out) = X^(b - <a,s>) * phase(acc) (reference test
+ *     tests/cggi_blind_rotation_test.cpp:118-123);
+ *   - key switching with sigma = 0 against the exact phase relation.
+ * Functions at the HEXL call sites with the 64-bit modulus use exact mod-Q
+ * semantics; parity with HEXL there is unpinned (HEXL is absent and unpinned).
+ */
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <math.h>
+#include <pthread.h>
+#include <time.h>
+
+typedef unsigned __int128 u128;
+#define OR_Q 0xFFFFFFFF00000001ULL
+#define OR_EPS 0xFFFFFFFFULL
+
+#define OR_EXPORT __attribute__((visibility("default")))
+
+/* ------------------------------------------------------------------------ */
+/* Field arithmetic: lib/include/backend/solinas_internal.h:39-111          */
+/* ------------------------------------------------------------------------ */
+
+OR_EXPORT uint64_t or_add(uint64_t a, uint64_t b) {
+    uint64_t s = a + b;
+    int carry = s < a;
+    if (carry || s >= OR_Q) s -= OR_Q; /* on carry: s + 2^64 - Q = s + eps */
+    return s;
+}
+
+OR_EXPORT uint64_t or_sub(uint64_t a, uint64_t b) {
+    uint64_t d = a - b;
+    if (a < b) d += OR_Q;
+    return d;
+}
+
+/* x = lo + 2^64 (h0 + 2^32 h1) = lo + eps*h0 - h1  (solinas_internal.h:61-73) */
+OR_EXPORT uint64_t or_reduce128(uint64_t hi, uint64_t lo) {
+    uint64_t h1 = hi >> 32, h0 = hi & OR_EPS;
+    uint64_t t0 = lo - h1;
+    if (lo < h1) t0 -= OR_EPS;
+    uint64_t t1 = (h0 << 32) - h0;
+    uint64_t r = t0 + t1;
+    if (r < t0) r += OR_EPS;
+    if (r >= OR_Q) r -= OR_Q;
+    return r;
+}
+
+OR_EXPORT uint64_t or_mul(uint64_t a, uint64_t b) {
+    u128 p = (u128)a * b;
+    return or_reduce128((uint64_t)(p >> 64), (uint64_t)p);
+}
+
+/* Textbook reference for the tests: 128-bit remainder. */
+OR_EXPORT uint64_t or_mul_textbook(uint64_t a, uint64_t b) {
+    return (uint64_t)(((u128)a * b) % OR_Q);
+}
+
+OR_EXPORT uint64_t or_pow(uint64_t base, uint64_t e) {
+    uint64_t r = 1;
+    base %= OR_Q;
+    while (e) {
+        if (e & 1) r = or_mul(r, base);
+        base = or_mul(base, base);
+        e >>= 1;
+    }
+    return r;
+}
+
+/* Fermat inverse; inv(0) = 0 as in solinas_internal.h:98-111. */
+OR_EXPORT uint64_t or_inv(uint64_t x) { return or_pow(x, OR_Q - 2); }
+
+static uint64_t neg(uint64_t x) { return x ? OR_Q - x : 0; }
+
+/* ------------------------------------------------------------------------ */
+/* NTT: lib/src/backend/solinas.cpp:168-313                                 */
+/* ------------------------------------------------------------------------ */
+
+static uint32_t brv(uint32_t x, unsigned bits) {
+    uint32_t r = 0;
+    for (unsigned i = 0; i < bits; ++i) { r = (r << 1) | (x & 1); x >>= 1; }
+    return r;
+}
+
+typedef struct {
+    uint32_t N, logN;
+    uint64_t psi, n_inv;
+    uint64_t *psi_rev, *psi_inv_rev; /* psi^brv(i), psi^-brv(i)  (solinas.cpp:180-192) */
+    uint64_t *mono;                  /* N*(N+2): rows NTT(X^i), then 2 zero rows (cggi_blind_rotation.cpp:474-485) */
+} or_ring;
+
+static void ntt_fwd(const or_ring* R, uint64_t* a);
+static void ntt_inv(const or_ring* R, uint64_t* a);
+
+OR_EXPORT or_ring* or_ring_create(uint32_t N) {
+    if (N < 2 || (N & (N - 1))) return NULL;
+    or_ring* R = (or_ring*)calloc(1, sizeof(or_ring));
+    R->N = N;
+    while ((1u << R->logN) < N) R->logN++;
+    /* psi = 7^((Q-1)/2N): 7 generates Z_Q^*  (solinas.cpp:175-176) */
+    R->psi = or_pow(7, (OR_Q - 1) >> (R->logN + 1));
+    uint64_t psi_inv = or_inv(R->psi);
+    R->n_inv = or_inv(N);
+    R->psi_rev = (uint64_t*)malloc(sizeof(uint64_t) * N);
+    R->psi_inv_rev = (uint64_t*)malloc(sizeof(uint64_t) * N);
+    for (uint32_t i = 0; i < N; ++i) {
+        uint32_t r = brv(i, R->logN);
+        R->psi_rev[i] = or_pow(R->psi, r);
+        R->psi_inv_rev[i] = or_pow(psi_inv, r);
+    }
+    R->mono = (uint64_t*)calloc((size_t)N * (N + 2), sizeof(uint64_t));
+    for (uint32_t i = 0; i < N; ++i) {
+        R->mono[(size_t)i * N + i] = 1;
+        ntt_fwd(R, R->mono + (size_t)i * N);
+    }
+    return R;
+}
+
+OR_EXPORT void or_ring_destroy(or_ring* R) {
+    if (!R) return;
+    free(R->psi_rev); free(R->psi_inv_rev); free(R->mono); free(R);
+}
+
+OR_EXPORT uint64_t or_ring_psi(const or_ring* R) { return R->psi; }
+
+/* Cooley-Tukey, natural in, bit-reversed out; twiddle of block i at level m
+ * is psi_rev[m + i] (solinas.cpp:228-261, same result as its fused levels). */
+static void ntt_fwd(const or_ring* R, uint64_t* a) {
+    const uint32_t N = R->N;
+    for (uint32_t m = 1, t = N / 2; m < N; m <<= 1, t >>= 1) {
+        for (uint32_t i = 0; i < m; ++i) {
+            const uint64_t w = R->psi_rev[m + i];
+            uint64_t* p = a + 2 * i * t;
+            for (uint32_t j = 0; j < t; ++j) {
+                uint64_t u = p[j], v = or_mul(p[j + t], w);
+                p[j] = or_add(u, v);
+                p[j + t] = or_sub(u, v);
+            }
+        }
+    }
+}
+
+/* Gentleman-Sande, bit-reversed in, natural out, times N^-1 (solinas.cpp:263-313). */
+static void ntt_inv(const or_ring* R, uint64_t* a) {
+    const uint32_t N = R->N;
+    for (uint32_t m = N / 2, t = 1; m >= 1; m >>= 1, t <<= 1) {
+        for (uint32_t i = 0; i < m; ++i) {
+            const uint64_t w = R->psi_inv_rev[m + i];
+            uint64_t* p = a + 2 * i * t;
+            for (uint32_t j = 0; j < t; ++j) {
+                uint64_t u = p[j], v = p[j + t];
+                p[j] = or_add(u, v);
+                p[j + t] = or_mul(or_sub(u, v), w);
+            }
+        }
+    }
+    for (uint32_t j = 0; j < N; ++j) a[j] = or_mul(a[j], R->n_inv);
+}
+
+OR_EXPORT void or_ntt_forward(const or_ring* R, uint64_t* out, const uint64_t* in, uint32_t count) {
+    for (uint32_t c = 0; c < count; ++c) {
+        if (out + (size_t)c * R->N != in + (size_t)c * R->N)
+            memcpy(out + (size_t)c * R->N, in + (size_t)c * R->N, sizeof(uint64_t) * R->N);
+        ntt_fwd(R, out + (size_t)c * R->N);
+    }
+}
+
+OR_EXPORT void or_ntt_inverse(const or_ring* R, uint64_t* out, const uint64_t* in, uint32_t count) {
+    for (uint32_t c = 0; c < count; ++c) {
+        if (out + (size_t)c * R->N != in + (size_t)c * R->N)
+            memcpy(out + (size_t)c * R->N, in + (size_t)c * R->N, sizeof(uint64_t) * R->N);
+        ntt_inv(R, out + (size_t)c * R->N);
+    }
+}
+
+/* Closed form, O(N^2): out[k] = sum_j a_j psi^((2 brv(k) + 1) j).  Used only to pin ntt_fwd. */
+OR_EXPORT void or_ntt_closed_form(const or_ring* R, uint64_t* out, const uint64_t* in) {
+    const uint32_t N = R->N;
+    for (uint32_t k = 0; k < N; ++k) {
+        uint64_t x = or_pow(R->psi, 2 * (uint64_t)brv(k, R->logN) + 1);
+        uint64_t acc = 0, xp = 1;
+        for (uint32_t j = 0; j < N; ++j) {
+            acc = or_add(acc, or_mul(in[j], xp));
+            xp = or_mul(xp, x);
+        }
+        out[k] = acc;
+    }
+}
+
+/* Schoolbook negacyclic product c = a*b mod (X^N + 1). */
+OR_EXPORT void or_negacyclic_mul(uint32_t N, uint64_t* c, const uint64_t* a, const uint64_t* b) {
+    memset(c, 0, sizeof(uint64_t) * N);
+    for (uint32_t i = 0; i < N; ++i)
+        for (uint32_t j = 0; j < N; ++j) {
+            uint64_t p = or_mul(a[i], b[j]);
+            uint32_t k = i + j;
+            if (k < N) c[k] = or_add(c[k], p);
+            else c[k - N] = or_sub(c[k - N], p);
+        }
+}
+
+static void vmul(uint64_t* o, const uint64_t* a, const uint64_t* b, uint32_t n) {
+    for (uint32_t i = 0; i < n; ++i) o[i] = or_mul(a[i], b[i]);
+}
+static void vadd(uint64_t* o, const uint64_t* a, const uint64_t* b, uint32_t n) {
+    for (uint32_t i = 0; i < n; ++i) o[i] = or_add(a[i], b[i]);
+}
+static void vsub(uint64_t* o, const uint64_t* a, const uint64_t* b, uint32_t n) {
+    for (uint32_t i = 0; i < n; ++i) o[i] = or_sub(a[i], b[i]);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Gadget decomposition                                                     */
+/* ------------------------------------------------------------------------ */
+
+/* ceil(64 / b): IntLog2(Q) = 64 (utils/math_utils.cpp:11-25). */
+static uint32_t max_digits(uint32_t b) { return 64 / b + (64 % b ? 1 : 0); }
+
+/* Corrector C and least-digit correction (static/gadget_decomp.h:16-28). */
+OR_EXPORT uint64_t or_signed_corrector(uint32_t b, uint64_t* least_out) {
+    uint64_t c = 0;
+    int64_t shift = 64 - (int64_t)b;
+    while (shift > 0) { c += 1ULL << (shift + b - 1); shift -= b; }
+    uint64_t least = 1ULL << (shift + b - 1);
+    c += least;
+    if (least_out) *least_out = least;
+    return c % OR_Q;
+}
+
+/* Unsigned digits of mux_operator.cpp:23-26,46-48,62-64:
+ * first_shift = b (ceil(64/b) - 1); digit j = (x >> (first_shift - j b)) & (2^b - 1).
+ * out: l rows of n. */
+OR_EXPORT void or_decompose_unsigned(uint64_t* out, const uint64_t* x, uint32_t n, uint32_t b, uint32_t l) {
+    const uint32_t first = b * (max_digits(b) - 1);
+    const uint64_t mask = (1ULL << b) - 1;
+    for (uint32_t j = 0; j < l; ++j)
+        for (uint32_t i = 0; i < n; ++i) out[(size_t)j * n + i] = (x[i] >> (first - j * b)) & mask;
+}
+
+/* Signed digits of gadget_decomp.h:156-205 (coefficient domain, before the NTT):
+ * x' = x + C mod Q; digit j = ((x' >> (64 - b(j+1))) & mask) - 2^(b-1) mod Q; when j is the
+ * (ceil(64/b)-1)-th digit it uses shift 0, the least-digit correction and a matching mask. */
+OR_EXPORT void or_decompose_signed(uint64_t* out, const uint64_t* x, uint32_t n, uint32_t b, uint32_t l) {
+    uint64_t least;
+    const uint64_t C = or_signed_corrector(b, &least);
+    const uint32_t D = max_digits(b);
+    for (uint32_t i = 0; i < n; ++i) {
+        const uint64_t xp = or_add(x[i], C);
+        int64_t shift = 64 - (int64_t)b;
+        for (uint32_t j = 0; j < l; ++j) {
+            uint64_t delta = 1ULL << (b - 1), mask = (1ULL << b) - 1;
+            if (j == D - 1) { delta = least; shift = 0; mask = (delta << 1) - 1; }
+            out[(size_t)j * n + i] = or_sub((xp >> shift) & mask, delta);
+            shift -= b;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* MuxOperator: lib/src/operators/mux_operator.cpp                          */
+/* ------------------------------------------------------------------------ */
+
+/* RLWEPrimeProduct (mux_operator.cpp:30-84), unsigned digits. */
+static void rlwe_prime_product(const or_ring* R, uint64_t* acc, const uint64_t* rp, const uint64_t* poly,
+                               int add, uint32_t b, uint32_t l, uint64_t* scratch) {
+    const uint32_t N = R->N;
+    const uint32_t first = b * (max_digits(b) - 1);
+    const uint64_t mask = (1ULL << b) - 1;
+    for (uint32_t j = 0; j < l; ++j) {
+        const uint32_t shift = first - j * b;
+        for (uint32_t i = 0; i < N; ++i) scratch[i] = (poly[i] >> shift) & mask;
+        ntt_fwd(R, scratch);
+        const uint64_t* ra = rp + (size_t)j * 2 * N;
+        if (!add && j == 0) {
+            vmul(acc, ra, scratch, N);
+            vmul(acc + N, ra + N, scratch, N);
+        } else {
+            for (uint32_t i = 0; i < N; ++i) {
+                acc[i] = or_add(acc[i], or_mul(ra[i], scratch[i]));
+                acc[N + i] = or_add(acc[N + i], or_mul(ra[N + i], scratch[i]));
+            }
+        }
+    }
+}
+
+/* ExternalProduct (mux_operator.cpp:114-121): rows [0,l) x digits(a), rows [l,2l) x digits(b). */
+OR_EXPORT void or_external_product(const or_ring* R, uint64_t* result, const uint64_t* rgsw, const uint64_t* rhs_coef,
+                                   int add, uint32_t b, uint32_t l) {
+    uint64_t* scratch = (uint64_t*)malloc(sizeof(uint64_t) * R->N);
+    rlwe_prime_product(R, result, rgsw, rhs_coef, add, b, l, scratch);
+    rlwe_prime_product(R, result, rgsw + (size_t)2 * R->N * l, rhs_coef + R->N, 1, b, l, scratch);
+    free(scratch);
+}
+
+/* BinaryCMux (mux_operator.cpp:136-152): monomial product in the NTT domain,
+ * INTT x2, external product, add.  acc is NTT-domain [a|b]. */
+static void binary_cmux(const or_ring* R, uint64_t* acc, const uint64_t* rgsw, const uint64_t* mono,
+                        uint32_t b, uint32_t l, uint64_t* ws /* 5N */) {
+    const uint32_t N = R->N;
+    uint64_t* s = ws;           /* 2N */
+    uint64_t* tmp = ws + 2 * N; /* 2N */
+    uint64_t* scratch = ws + 4 * N;
+    vmul(s, acc, mono, N);
+    vmul(s + N, acc + N, mono, N);
+    ntt_inv(R, s);
+    ntt_inv(R, s + N);
+    rlwe_prime_product(R, tmp, rgsw, s, 0, b, l, scratch);
+    rlwe_prime_product(R, tmp, rgsw + (size_t)2 * N * l, s + N, 1, b, l, scratch);
+    vadd(acc, acc, tmp, 2 * N);
+}
+
+OR_EXPORT void or_binary_cmux(const or_ring* R, uint64_t* acc, const uint64_t* rgsw, const uint64_t* mono,
+                              uint32_t b, uint32_t l) {
+    uint64_t* ws = (uint64_t*)malloc(sizeof(uint64_t) * 5 * R->N);
+    binary_cmux(R, acc, rgsw, mono, b, l, ws);
+    free(ws);
+}
+
+/* SignedGadgetDecomposeRepNTT<1> (gadget_decomp.h:156-205): row j = [NTT(d_j) | NTT(d_j)]. */
+static void signed_rep2_ntt(const or_ring* R, uint64_t* out, const uint64_t* x, uint32_t b, uint32_t l) {
+    const uint32_t N = R->N;
+    uint64_t* d = (uint64_t*)malloc(sizeof(uint64_t) * N * l);
+    or_decompose_signed(d, x, N, b, l);
+    for (uint32_t j = 0; j < l; ++j) {
+        uint64_t* row = out + (size_t)j * 2 * N;
+        memcpy(row, d + (size_t)j * N, sizeof(uint64_t) * N);
+        ntt_fwd(R, row);
+        memcpy(row + N, row, sizeof(uint64_t) * N);
+    }
+    free(d);
+}
+
+/* sum_k w_k (RGSW_k [x] acc) with one shared signed decomposition: the common core of
+ * TernaryCMux (mux_operator.cpp:156-202, k = 2) and MultiMux (:204-294, k = 3).
+ * Row r of RGSW_k multiplies the Rep2 digit row r; rows are summed, the sum is multiplied
+ * by the weight w_k (NTT domain) and added to acc. */
+static void multi_mux(const or_ring* R, uint64_t* acc, uint32_t k, const uint64_t* rgsws, const uint64_t* weights,
+                      uint32_t b, uint32_t l, uint64_t* ws) {
+    const uint32_t N = R->N;
+    const size_t rp = (size_t)2 * N * l, rg = 2 * rp;
+    uint64_t* in = ws;                 /* 2N */
+    uint64_t* dig = ws + 2 * N;        /* 4Nl */
+    uint64_t* prod = dig + rg;         /* 4Nl */
+    uint64_t* sum = prod + rg;         /* 2N */
+    ntt_inv(R, memcpy(in, acc, sizeof(uint64_t) * N));
+    ntt_inv(R, memcpy(in + N, acc + N, sizeof(uint64_t) * N));
+    signed_rep2_ntt(R, dig, in, b, l);
+    signed_rep2_ntt(R, dig + rp, in + N, b, l);
+    for (uint32_t c = 0; c < k; ++c) {
+        vmul(prod, rgsws + c * rg, dig, (uint32_t)rg);
+        memcpy(sum, prod, sizeof(uint64_t) * 2 * N);
+        for (uint32_t r = 1; r < 2 * l; ++r) vadd(sum, sum, prod + (size_t)r * 2 * N, 2 * N);
+        vmul(sum, sum, weights + (size_t)c * N, N);
+        vmul(sum + N, sum + N, weights + (size_t)c * N, N);
+        vadd(acc, acc, sum, 2 * N);
+    }
+}
+
+static size_t multi_mux_ws(uint32_t N, uint32_t l) { return (size_t)4 * N + (size_t)8 * N * l; }
+
+OR_EXPORT void or_multi_mux(const or_ring* R, uint64_t* acc, uint32_t k, const uint64_t* rgsws,
+                            const uint64_t* weights, uint32_t b, uint32_t l) {
+    uint64_t* ws = (uint64_t*)malloc(sizeof(uint64_t) * multi_mux_ws(R->N, l));
+    multi_mux(R, acc, k, rgsws, weights, b, l, ws);
+    free(ws);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Blind rotation                                                           */
+/* ------------------------------------------------------------------------ */
+
+enum { OR_BINARY = 0, OR_TERNARY = 1, OR_BMMP = 2 };
+
+/* acc <- acc * X^b in the NTT domain (cggi_blind_rotation.cpp:336-355). */
+static void mul_by_xb(const or_ring* R, uint64_t* acc, uint64_t bb, uint64_t* tmp) {
+    const uint32_t N = R->N;
+    if (bb >= N) {
+        bb -= N;
+        for (uint32_t i = 0; i < 2 * N; ++i) tmp[i] = neg(acc[i]);
+        vmul(acc, tmp, R->mono + bb * N, N);
+        vmul(acc + N, tmp + N, R->mono + bb * N, N);
+    } else {
+        vmul(acc, acc, R->mono + bb * N, N);
+        vmul(acc + N, acc + N, R->mono + bb * N, N);
+    }
+}
+
+/* NTT(+-X^e) for e in [0, 2N). */
+static void signed_mono(const or_ring* R, uint64_t* out, uint64_t e) {
+    const uint32_t N = R->N;
+    if (e >= N) for (uint32_t i = 0; i < N; ++i) out[i] = neg(R->mono[(e - N) * N + i]);
+    else memcpy(out, R->mono + e * N, sizeof(uint64_t) * N);
+}
+
+/* BMMP step polynomials (bmmp_blind_rotation.cpp:280-323):
+ * [X^a0 - 1 | X^a1 - 1 | X^(a0+a1) - X^a0 - X^a1 + 1], NTT domain. */
+static void bmmp_step_polys(const or_ring* R, uint64_t* p, uint64_t a0, uint64_t a1) {
+    const uint32_t N = R->N;
+    const uint64_t sum = (a0 + a1) & (2 * N - 1);
+    signed_mono(R, p, a0);
+    signed_mono(R, p + N, a1);
+    signed_mono(R, p + 2 * N, sum);
+    vsub(p + 2 * N, p + 2 * N, p, N);
+    vsub(p + 2 * N, p + 2 * N, p + N, N);
+    vsub(p, p, R->mono, N);
+    vsub(p + N, p + N, R->mono, N);
+    vadd(p + 2 * N, p + 2 * N, R->mono, N);
+}
+
+/* BlindRotate (cggi_blind_rotation.cpp:236-262, 313-472; bmmp_blind_rotation.cpp:325-419).
+ * lwe: n+1 values in Z_2N (b at index n).  acc_coef: 2N coefficient-domain [a|b].
+ * result: 2N, NTT domain unless as_coef.  result may alias acc_coef. */
+OR_EXPORT int or_blind_rotate(const or_ring* R, uint64_t* result, const uint64_t* lwe, const uint64_t* acc_coef,
+                              const uint64_t* brk, uint32_t n, uint32_t b, uint32_t l, int method, int as_coef) {
+    const uint32_t N = R->N;
+    if (method == OR_BMMP && (n & 1)) return -1;
+    uint64_t* ws = (uint64_t*)malloc(sizeof(uint64_t) * (multi_mux_ws(N, l) + 8 * (size_t)N));
+    uint64_t* mono = ws + multi_mux_ws(N, l); /* 3N */
+    uint64_t* tmp = mono + 3 * N;             /* 2N */
+    if (result != acc_coef) memcpy(result, acc_coef, sizeof(uint64_t) * 2 * N);
+    ntt_fwd(R, result);
+    ntt_fwd(R, result + N);
+    mul_by_xb(R, result, lwe[n], tmp);
+    if (method == OR_BINARY) {
+        const size_t rg = (size_t)4 * N * l;
+        for (uint32_t i = 0; i < n; ++i) {
+            const uint64_t ai = lwe[i];
+            if (ai == 0) continue;
+            signed_mono(R, mono, 2 * N - ai);            /* X^(2N - a_i) */
+            vsub(mono, mono, R->mono, N);                 /* - NTT(1) */
+            binary_cmux(R, result, brk + i * rg, mono, b, l, ws);
+        }
+    } else if (method == OR_TERNARY) {
+        const size_t rg = (size_t)8 * N * l;
+        for (uint32_t i = 0; i < n; ++i) {
+            const uint64_t ai = lwe[i];
+            if (ai == 0) continue;
+            signed_mono(R, mono, 2 * N - ai);            /* X^(-a_i) */
+            signed_mono(R, mono + N, ai);                 /* X^(a_i) */
+            vsub(mono, mono, R->mono, N);
+            vsub(mono + N, mono + N, R->mono, N);
+            multi_mux(R, result, 2, brk + i * rg, mono, b, l, ws);
+        }
+    } else if (method == OR_BMMP) {
+        const size_t rg = (size_t)12 * N * l;
+        const uint64_t m2 = 2 * N - 1;
+        for (uint32_t i = 0; i < n; i += 2) {
+            const uint64_t a0 = (2 * N - lwe[i]) & m2, a1 = (2 * N - lwe[i + 1]) & m2;
+            bmmp_step_polys(R, mono, a0, a1);
+            multi_mux(R, result, 3, brk + (i >> 1) * rg, mono, b, l, ws);
+        }
+    } else {
+        free(ws);
+        return -2;
+    }
+    if (as_coef) {
+        ntt_inv(R, result);
+        ntt_inv(R, result + N);
+    }
+    free(ws);
+    return 0;
+}
+
+/* SampleExtract at index 0 (static/sample_extraction.h:20-36); zero coefficients
+ * stay zero (the reference skips the write into a zeroed output). */
+OR_EXPORT void or_sample_extract(uint64_t* out, const uint64_t* rlwe_coef, uint32_t N) {
+    out[0] = rlwe_coef[0];
+    for (uint32_t i = 1; i < N; ++i) out[i] = neg(rlwe_coef[N - i]);
+    out[N] = rlwe_coef[N];
+}
+
+/* LWEtoLWEConverter::Convert (lwe_conversion.cpp:205-239) with overwrite semantics:
+ * out[k] = -sum d_ij KSK[i][j][k], out[n_out] = in[n_in] - sum d_ij KSK[i][j][n_out]. */
+OR_EXPORT int or_lwe_keyswitch(uint64_t* out, const uint64_t* in, const uint64_t* ksk, uint32_t n_in,
+                               uint32_t n_out, uint32_t b, uint32_t d) {
+    if (d >= max_digits(b) && 64 % b) return -1; /* reference shift would underflow */
+    uint64_t least;
+    const uint64_t C = or_signed_corrector(b, &least);
+    const uint32_t D = max_digits(b);
+    const uint64_t mask = (1ULL << b) - 1;
+    uint64_t* acc = (uint64_t*)calloc(n_out + 1, sizeof(uint64_t));
+    for (uint32_t i = 0; i < n_in; ++i) {
+        const uint64_t ai = or_add(in[i], C);
+        const uint64_t* chunk = ksk + (size_t)i * d * (n_out + 1);
+        uint32_t shift = 64 - b;
+        for (uint32_t j = 0; j < d; ++j, chunk += n_out + 1) {
+            const uint64_t delta = (j != D - 1) ? (1ULL << (b - 1)) : least;
+            const uint64_t dij = or_sub((ai >> shift) & mask, delta);
+            for (uint32_t k = 0; k <= n_out; ++k) acc[k] = or_add(acc[k], or_mul(chunk[k], dij));
+            shift -= b;
+        }
+    }
+    for (uint32_t k = 0; k < n_out; ++k) out[k] = neg(acc[k]);
+    out[n_out] = or_sub(in[n_in], acc[n_out]);
+    free(acc);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Seeded key generation (replaces the OpenFHE samplers, which have no seed  */
+/* API: base_crypto.cpp:34-38,139-142).  Byte layouts follow                */
+/* base_crypto.cpp:27-97 and cggi_blind_rotation.cpp:264-311.               */
+/* ------------------------------------------------------------------------ */
+
+typedef struct { uint64_t s; } or_rng;
+
+static uint64_t rng_next(or_rng* g) {
+    uint64_t z = (g->s += 0x9E3779B97F4A7C15ULL);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+static or_rng rng_make(uint64_t seed, uint64_t tag) {
+    or_rng g;
+    g.s = seed ^ (tag * 0xD1B54A32D192ED03ULL) ^ 0xA0761D6478BD642FULL;
+    return g;
+}
+static uint64_t rng_uniform_q(or_rng* g) {
+    uint64_t x;
+    do { x = rng_next(g); } while (x >= OR_Q);
+    return x;
+}
+
+/* Discrete Gaussian by inversion of a 63-bit cumulative table over |x| <= 12 sigma. */
+typedef struct { double sigma; uint32_t K; uint64_t T[256]; } or_gauss;
+static void gauss_init(or_gauss* G, double sigma) {
+    G->sigma = sigma;
+    if (sigma <= 0) { G->K = 0; G->T[0] = 1ULL << 63; return; }
+    uint32_t K = (uint32_t)ceil(sigma * 12.0);
+    if (K > 255) K = 255;
+    double w[256], tot = 0;
+    for (uint32_t k = 0; k <= K; ++k) {
+        w[k] = exp(-(double)(k * k) / (2.0 * sigma * sigma));
+        tot += (k ? 2.0 * w[k] : w[k]);
+    }
+    double c = 0;
+    for (uint32_t k = 0; k <= K; ++k) {
+        c += (k ? 2.0 * w[k] : w[k]);
+        double v = ldexp(c / tot, 63);
+        G->T[k] = (k == K || v >= 9223372036854775808.0) ? (1ULL << 63) : (uint64_t)v;
+    }
+    G->K = K;
+}
+static uint64_t gauss_sample(const or_gauss* G, or_rng* g) {
+    const uint64_t x = rng_next(g);
+    const uint64_t u = x >> 1;
+    uint32_t k = 0;
+    while (k < G->K && u >= G->T[k]) ++k;
+    return (x & 1) ? neg(k) : k;
+}
+
+/* Secret keys.  dist 0: binary {0,1}; 1: ternary P(0)=1/2, P(1)=P(-1)=1/4 with -1 stored as Q-1. */
+OR_EXPORT void or_keygen_secret(uint64_t* out, uint32_t len, int dist, uint64_t seed, uint64_t tag) {
+    or_rng g = rng_make(seed, tag);
+    for (uint32_t i = 0; i < len; ++i) {
+        if (dist == 0) out[i] = rng_next(&g) >> 63;
+        else {
+            const uint64_t r = rng_next(&g) >> 62;
+            out[i] = r < 2 ? 0 : (r == 2 ? 1 : OR_Q - 1);
+        }
+    }
+}
+
+/* MakeRLWE (base_crypto.cpp:27-58): a uniform (used as NTT domain), e Gaussian,
+ * b = a * s + NTT(e) + msg_ntt. */
+static void make_rlwe(const or_ring* R, uint64_t* out, const uint64_t* msg_ntt, const uint64_t* s_ntt,
+                      or_rng* g, const or_gauss* G, uint64_t* e) {
+    const uint32_t N = R->N;
+    for (uint32_t i = 0; i < N; ++i) out[i] = rng_uniform_q(g);
+    for (uint32_t i = 0; i < N; ++i) e[i] = gauss_sample(G, g);
+    ntt_fwd(R, e);
+    for (uint32_t i = 0; i < N; ++i)
+        out[N + i] = or_add(or_add(or_mul(out[i], s_ntt[i]), e[i]), msg_ntt[i]);
+}
+
+/* MakeRGSW (base_crypto.cpp:60-97) for a constant message m, with gadget weights g[0..l):
+ * row r < l encrypts -s m g_r, row l + r encrypts m g_r.  Rows are produced in the reference's
+ * order (r = l-1 down to 0, the -s m row first), which fixes the random-draw order. */
+static void make_rgsw_const(const or_ring* R, uint64_t* out, uint64_t m, const uint64_t* gad, uint32_t l,
+                            const uint64_t* s_ntt, or_rng* g, const or_gauss* G, uint64_t* ws /* 3N */) {
+    const uint32_t N = R->N;
+    uint64_t* msg = ws;      /* m g (NTT of a constant is the constant in every slot) */
+    uint64_t* msgs = ws + N; /* -s m g */
+    uint64_t* e = ws + 2 * N;
+    for (uint32_t t = 0; t < l; ++t) {
+        const uint32_t r = l - 1 - t;
+        const uint64_t mg = or_mul(m, gad[r]);
+        for (uint32_t i = 0; i < N; ++i) { msg[i] = mg; msgs[i] = neg(or_mul(mg, s_ntt[i])); }
+        make_rlwe(R, out + (size_t)r * 2 * N, msgs, s_ntt, g, G, e);
+        make_rlwe(R, out + (size_t)r * 2 * N + (size_t)2 * N * l, msg, s_ntt, g, G, e);
+    }
+}
+
+/* Gadget weights.  kind 0 (reference MakeRGSW, base_crypto.cpp:68,85-95): L^(D-1-j).
+ * kind 1 (signed decomposer, gadget_decomp.h:176-193): 2^(64 - b(j+1)), the j = D-1 digit weight 1.
+ * The two agree whenever b divides 64. */
+OR_EXPORT void or_gadget(uint64_t* g, uint32_t b, uint32_t l, int kind) {
+    const uint32_t D = max_digits(b);
+    for (uint32_t j = 0; j < l; ++j) {
+        if (kind == 0) g[j] = 1ULL << (b * (D - 1 - j));
+        else g[j] = (j == D - 1) ? 1 : (1ULL << (64 - b * (j + 1)));
+    }
+}
+
+/* Blind-rotation key.  method 0 binary: n x RGSW(s_i); 1 ternary: n x [RGSW(s_i==1) | RGSW(s_i==-1)];
+ * 2 BMMP: n/2 x [RGSW(s_2i) | RGSW(s_2i+1) | RGSW(s_2i s_2i+1)].  gadget_kind as or_gadget. */
+OR_EXPORT int or_keygen_brk(const or_ring* R, uint64_t* brk, const uint64_t* lwe_key, const uint64_t* rlwe_key,
+                            uint32_t n, uint32_t b, uint32_t l, int method, int gadget_kind, double sigma,
+                            uint64_t seed) {
+    const uint32_t N = R->N;
+    or_rng g = rng_make(seed, 3);
+    or_gauss G;
+    gauss_init(&G, sigma);
+    uint64_t* s_ntt = (uint64_t*)malloc(sizeof(uint64_t) * 4 * N);
+    uint64_t* ws = s_ntt + N;
+    uint64_t gad[64];
+    or_gadget(gad, b, l, gadget_kind);
+    memcpy(s_ntt, rlwe_key, sizeof(uint64_t) * N);
+    ntt_fwd(R, s_ntt);
+    const size_t rg = (size_t)4 * N * l;
+    for (uint32_t i = 0; i < n; ++i) {
+        if (method == OR_BINARY) {
+            make_rgsw_const(R, brk + i * rg, lwe_key[i], gad, l, s_ntt, &g, &G, ws);
+        } else if (method == OR_TERNARY) {
+            const uint64_t s = lwe_key[i];
+            const uint64_t p = (s == 1), m = (s != 0 && s != 1); /* cggi_blind_rotation.cpp:294-305 */
+            make_rgsw_const(R, brk + 2 * i * rg, p, gad, l, s_ntt, &g, &G, ws);
+            make_rgsw_const(R, brk + (2 * i + 1) * rg, m, gad, l, s_ntt, &g, &G, ws);
+        } else if (method == OR_BMMP) {
+            if (i & 1) continue;
+            uint64_t* chunk = brk + (size_t)3 * (i >> 1) * rg;
+            make_rgsw_const(R, chunk, lwe_key[i], gad, l, s_ntt, &g, &G, ws);
+            make_rgsw_const(R, chunk + rg, lwe_key[i + 1], gad, l, s_ntt, &g, &G, ws);
+            make_rgsw_const(R, chunk + 2 * rg, or_mul(lwe_key[i], lwe_key[i + 1]), gad, l, s_ntt, &g, &G, ws);
+        }
+    }
+    free(s_ntt);
+    return 0;
+}
+
+/* MakeLWE (base_crypto.cpp:137-155): a uniform, e Gaussian, b = e + <a, s> + m. */
+static void make_lwe(uint64_t* out, uint64_t m, const uint64_t* s, uint32_t n, or_rng* g, const or_gauss* G) {
+    for (uint32_t i = 0; i < n; ++i) out[i] = rng_uniform_q(g);
+    uint64_t bb = gauss_sample(G, g);
+    for (uint32_t i = 0; i < n; ++i) bb = or_add(bb, or_mul(out[i], s[i]));
+    out[n] = or_add(bb, m);
+}
+
+/* LWE->LWE key-switching key (lwe_conversion.cpp:174-203): [n_in][d][n_out+1],
+ * entry (i, j) = LWE_t(2^(64 - b(j+1)) * s_src[i]). */
+OR_EXPORT void or_keygen_lwe_ksk(uint64_t* ksk, const uint64_t* src, const uint64_t* dst, uint32_t n_in,
+                                 uint32_t n_out, uint32_t b, uint32_t d, double sigma, uint64_t seed) {
+    or_rng g = rng_make(seed, 4);
+    or_gauss G;
+    gauss_init(&G, sigma);
+    for (uint32_t i = 0; i < n_in; ++i) {
+        uint64_t gij = 1ULL << (64 - b);
+        for (uint32_t j = 0; j < d; ++j) {
+            make_lwe(ksk + ((size_t)i * d + j) * (n_out + 1), or_mul(gij, src[i]), dst, n_out, &g, &G);
+            gij >>= b;
+        }
+    }
+}
+
+/* RLWE encryption of a coefficient-domain message under s (coefficients); output NTT domain. */
+OR_EXPORT void or_encrypt_rlwe(const or_ring* R, uint64_t* out, const uint64_t* msg_coef, const uint64_t* s_coef,
+                               double sigma, uint64_t seed, uint64_t tag) {
+    const uint32_t N = R->N;
+    or_rng g = rng_make(seed, tag);
+    or_gauss G;
+    gauss_init(&G, sigma);
+    uint64_t* ws = (uint64_t*)malloc(sizeof(uint64_t) * 3 * N);
+    memcpy(ws, s_coef, sizeof(uint64_t) * N);
+    ntt_fwd(R, ws);
+    memcpy(ws + N, msg_coef, sizeof(uint64_t) * N);
+    ntt_fwd(R, ws + N);
+    make_rlwe(R, out, ws + N, ws, &g, &G, ws + 2 * N);
+    free(ws);
+}
+
+/* Uniform values in [0, bound), bound a power of two (LWE inputs in Z_2N). */
+OR_EXPORT void or_random_mod_pow2(uint64_t* out, size_t len, uint64_t bound, uint64_t seed, uint64_t tag) {
+    or_rng g = rng_make(seed, tag);
+    for (size_t i = 0; i < len; ++i) out[i] = rng_next(&g) & (bound - 1);
+}
+
+/* Gaussian samples (as values mod Q), for tests of the sampler. */
+OR_EXPORT void or_gauss_samples(uint64_t* out, size_t len, double sigma, uint64_t seed, uint64_t tag) {
+    or_rng g = rng_make(seed, tag);
+    or_gauss G;
+    gauss_init(&G, sigma);
+    for (size_t i = 0; i < len; ++i) out[i] = gauss_sample(&G, &g);
+}
+
+/* PhaseRLWE (base_crypto.cpp:99-108): coefficients of b - a s, for an NTT-domain RLWE. */
+OR_EXPORT void or_phase_rlwe(const or_ring* R, uint64_t* out, const uint64_t* rlwe_ntt, const uint64_t* s_coef) {
+    const uint32_t N = R->N;
+    uint64_t* s = (uint64_t*)malloc(sizeof(uint64_t) * N);
+    memcpy(s, s_coef, sizeof(uint64_t) * N);
+    ntt_fwd(R, s);
+    for (uint32_t i = 0; i < N; ++i) out[i] = or_sub(rlwe_ntt[N + i], or_mul(rlwe_ntt[i], s[i]));
+    ntt_inv(R, out);
+    free(s);
+}
+
+/* PhaseLWE (base_crypto.cpp:157-169): b - <a, s> mod Q. */
+OR_EXPORT uint64_t or_phase_lwe(const uint64_t* lwe, const uint64_t* s, uint32_t n) {
+    uint64_t acc = 0;
+    for (uint32_t i = 0; i < n; ++i) acc = or_add(acc, or_mul(lwe[i], s[i]));
+    return or_sub(lwe[n], acc);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Batched bootstrap for the CPU baseline: EvalFunc (negacyclic_function_    */
+/* evaluator.cpp:92-101) + LWE key switch, one rotator per thread.          */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    const or_ring* R;
+    const uint64_t *lwes, *tv, *brk, *ksk;
+    uint64_t* out;
+    uint32_t n, b, l, method, n_out, ks_b, ks_d;
+    size_t begin, end;
+} bs_job;
+
+static void* bs_worker(void* arg) {
+    bs_job* J = (bs_job*)arg;
+    const uint32_t N = J->R->N;
+    uint64_t* acc = (uint64_t*)malloc(sizeof(uint64_t) * (3 * N + 1));
+    uint64_t* ext = acc + 2 * N;
+    for (size_t c = J->begin; c < J->end; ++c) {
+        or_blind_rotate(J->R, acc, J->lwes + c * (J->n + 1), J->tv, J->brk, J->n, J->b, J->l, (int)J->method, 1);
+        or_sample_extract(ext, acc, N);
+        if (J->ksk)
+            or_lwe_keyswitch(J->out + c * (J->n_out + 1), ext, J->ksk, N, J->n_out, J->ks_b, J->ks_d);
+        else
+            memcpy(J->out + c * (N + 1), ext, sizeof(uint64_t) * (N + 1));
+    }
+    free(acc);
+    return NULL;
+}
+
+/* out: count x (n_out+1) if ksk != NULL, else count x (N+1).  Returns elapsed seconds. */
+OR_EXPORT double or_bootstrap_batch(const or_ring* R, uint64_t* out, const uint64_t* lwes, size_t count,
+                                    const uint64_t* tv, const uint64_t* brk, uint32_t n, uint32_t b, uint32_t l,
+                                    int method, const uint64_t* ksk, uint32_t n_out, uint32_t ks_b, uint32_t ks_d,
+                                    int threads) {
+    if (threads < 1) threads = 1;
+    pthread_t th[256];
+    bs_job jobs[256];
+    if (threads > 256) threads = 256;
+    struct timespec t0, t1;
+    clock_gettime(CLOCK_MONOTONIC, &t0);
+    for (int t = 0; t < threads; ++t) {
+        bs_job J = {R, lwes, tv, brk, ksk, out, n, b, l, (uint32_t)method, n_out, ks_b, ks_d,
+                    count * t / threads, count * (t + 1) / threads};
+        jobs[t] = J;
+        pthread_create(&th[t], NULL, bs_worker, &jobs[t]);
+    }
+    for (int t = 0; t < threads; ++t) pthread_join(th[t], NULL);
+    clock_gettime(CLOCK_MONOTONIC, &t1);
+    return (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
+}
