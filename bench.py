@@ -1,0 +1,198 @@
+"""Benchmark: CGGI bootstraps/s on one MI355X, batched persistent path vs plain path.
+
+Workload (one "step" = one batch): 4096 independent LWE inputs (n=630, binary key, entries
+uniform in [0, 2048)), blind rotation over N=1024 with a 61.9 MB NTT-domain bootstrapping
+key (gadget base 2^12, 3 digits, unsigned digits as in the reference binary path), sample
+extraction, then LWE->LWE key switching 1024 -> 630 (base 2^4, 14 digits).  Keys are
+seeded and synthetic; inputs are resident in HBM before the timed region.
+
+Prints one JSON line (rank 0).  --gpus above 1 is refused: this path runs on one GPU.
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+Q = 0xFFFFFFFF00000001
+N, n, BB, L = 1024, 630, 12, 3
+KS_OUT, KS_B, KS_D = 630, 4, 14
+BATCH = 4096
+SEED = 2026
+
+
+def algorithmic_mulmods():
+    """Mulmods per bootstrap (every NTT butterfly counts as one), binary CGGI + key switch."""
+    T = (N // 2) * (N.bit_length() - 1)
+    w_br = n * ((2 + 2 * L) * T + 2 * N + 4 * L * N) + 4 * T + 2 * N
+    w_ks = N * KS_D * (KS_OUT + 1)
+    return w_br, w_ks
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor()
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--batch", type=int, default=BATCH)
+    ap.add_argument("--plain-steps", type=int, default=3)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this path runs on one GPU; --gpus %d is not supported (sharding is described in "
+              "DESIGN.md)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import fhetoothpaste_amd as tp
+    from fhetoothpaste_amd import _lib
+    from fhetoothpaste_amd.operators import DeviceBuffer
+
+    if tp.device_count() < 1:
+        print("bench.py: no gfx950 device", file=sys.stderr)
+        sys.exit(1)
+
+    t0 = time.time()
+    lk = tp.keygen_secret(n, 0, SEED, 1)
+    rk = tp.keygen_secret(N, 0, SEED, 2)
+    brk = tp.keygen_brk(lk, rk, N, n, BB, L, 0, 0, 3.2, SEED)
+    ksk = tp.keygen_lwe_ksk(rk, lk, N, KS_OUT, KS_B, KS_D, 3.2, SEED)
+    keygen_s = time.time() - t0
+    ctx = tp.Context(N, n, BB, L, 0, KS_OUT, KS_B, KS_D)
+    ctx.import_brk(brk)
+    ctx.import_lwe_ksk(ksk)
+
+    B = args.batch
+    rng = np.random.default_rng(SEED)
+    lwes = rng.integers(0, 2 * N, size=(B, n + 1), dtype=np.uint64)
+    tv = np.zeros(2 * N, dtype=np.uint64)
+    tv[N:] = Q // 8
+    d_lwe = DeviceBuffer.from_host(ctx, lwes)
+    d_tv = DeviceBuffer.from_host(ctx, tv)
+    d_ext = DeviceBuffer(ctx, B * (N + 1))
+    d_out = DeviceBuffer(ctx, B * (KS_OUT + 1))
+    lib = _lib.lib()
+    chk = _lib.check
+
+    def step(count=B, lwe_p=None, out_p=None, ext_p=None, mark=False):
+        lwe_p = lwe_p or d_lwe.p
+        if mark:
+            chk(lib.tp_timer_mark(ctx.h, 0))
+        chk(lib.tp_eval_func_batch_device(ctx.h, count, lwe_p, d_tv.p, 0, ext_p or d_ext.p, None, 0))
+        if mark:
+            chk(lib.tp_timer_mark(ctx.h, 1))
+        chk(lib.tp_lwe_keyswitch_batch_device(ctx.h, count, ext_p or d_ext.p, N + 1, out_p or d_out.p))
+        if mark:
+            chk(lib.tp_timer_mark(ctx.h, 2))
+
+    for _ in range(args.warmup):
+        step()
+    ctx.sync()
+
+    # timed region: K batches, inputs already in HBM
+    import ctypes
+    br_ms, ks_ms = [], []
+    ctx.sync()
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        step(mark=True)
+        ms_a, ms_b = ctypes.c_float(), ctypes.c_float()
+        chk(lib.tp_timer_elapsed(ctx.h, 0, 1, ctypes.byref(ms_a)))
+        chk(lib.tp_timer_elapsed(ctx.h, 1, 2, ctypes.byref(ms_b)))
+        br_ms.append(ms_a.value)
+        ks_ms.append(ms_b.value)
+    ctx.sync()
+    elapsed = time.perf_counter() - t_start
+    ms_per_step = 1000.0 * elapsed / args.steps
+    value = B * args.steps / elapsed
+
+    # plain path: one bootstrap per launch (blind rotation grid of 1 + key switch of 1)
+    plain_lat = []
+    for i in range(args.plain_steps + 1):
+        ctx.sync()
+        t = time.perf_counter()
+        step(count=1)
+        ctx.sync()
+        if i:
+            plain_lat.append(time.perf_counter() - t)
+    plain_bs = 1.0 / (sum(plain_lat) / len(plain_lat))
+
+    # batch latency percentiles: every bootstrap of a batch completes with its batch
+    lat_ms = sorted(br + ks for br, ks in zip(br_ms, ks_ms))
+    p50 = float(np.percentile(lat_ms, 50))
+    p99 = float(np.percentile(lat_ms, 99))
+
+    w_br, w_ks = algorithmic_mulmods()
+    br_avg_s = 1e-3 * sum(br_ms) / len(br_ms)
+    achieved = (w_br * B) / br_avg_s / 1e12  # Tmulmod/s in the dominant kernel
+    peak = float(os.environ.get("TP_VALU_PEAK_TMULMOD", "2.4576"))
+
+    result = {
+        "metric": "CGGI bootstraps/sec at n=630,N=1024 (batched vs plain); p50/p99 latency",
+        "value": round(value, 2),
+        "unit": "bootstraps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64 (mod 2^64-2^32+1)",
+        "data": "synthetic (seeded keys and inputs)",
+        "config": {"workload": "batched CGGI bootstrap: blind rotation n=630 N=1024 b=2^12 l=3 + sample extract + "
+                               "LWE key switch 1024->630 b=2^4 d=14", "batch": B, "n": n, "N": N,
+                   "parallelism": "one GPU, one work-group per ciphertext"},
+        "plain_bootstraps_per_s": round(plain_bs, 2),
+        "batched_vs_plain": round(value / plain_bs, 2),
+        "latency_ms": {"batch_p50": round(p50, 3), "batch_p99": round(p99, 3),
+                       "plain_mean": round(1000 * sum(plain_lat) / len(plain_lat), 3)},
+        "kernel_ms": {"blind_rotate_avg": round(1000 * br_avg_s, 3),
+                      "keyswitch_avg": round(sum(ks_ms) / len(ks_ms), 3)},
+        "roofline": {"bound": "valu", "achieved": round(achieved, 4), "peak": peak, "unit": "Tmulmod/s",
+                     "frac": round(achieved / peak, 4), "traffic": None,
+                     "kernel": "br_binary_kernel", "work_per_launch_mulmod": w_br * B},
+        "keygen_s": round(keygen_s, 2),
+    }
+
+    if not args.no_cpu:
+        import oracle as O
+        # bounded CPU sample of the same workload: a few bootstraps per thread
+        threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+        threads = max(1, min(threads, 16))
+        per = 1
+        O_lwes = np.ascontiguousarray(lwes[: threads * per])
+        _, secs = O.bootstrap_batch(N, O_lwes.reshape(-1), tv, brk, n, BB, L, O.BINARY, ksk, KS_OUT, KS_B, KS_D,
+                                    threads)
+        cnt = threads * per
+        if secs < args.cpu_seconds / 3:
+            reps = max(1, int(args.cpu_seconds / max(secs, 1e-3)) - 1)
+            cnt2 = min(len(lwes), threads * per * reps)
+            _, secs = O.bootstrap_batch(N, np.ascontiguousarray(lwes[:cnt2]).reshape(-1), tv, brk, n, BB, L,
+                                        O.BINARY, ksk, KS_OUT, KS_B, KS_D, threads)
+            cnt = cnt2
+        result["cpu_baseline"] = {"value": round(cnt / secs, 3), "unit": "bootstraps/s", "cores": threads,
+                                  "kind": "port",
+                                  "sample": "%d bootstraps of the same workload on %d threads (%s), %.1f s" % (
+                                      cnt, threads, cpu_model(), secs)}
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,96 @@
+"""ctypes binding of libtoothpaste_amd.so (the C-ABI declared in include/toothpaste_amd.h).
+
+The library is built in-tree by fhetoothpaste_amd.build.  There is no CPU fallback: if the
+library or a gfx950 device is missing, calls raise ToothpasteError.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libtoothpaste_amd.so")
+
+Q = 0xFFFFFFFF00000001
+TP_OK, TP_E_INVALID, TP_E_DEVICE, TP_E_STATE, TP_E_NOMEM = 0, -1, -2, -3, -4
+METHOD_CGGI_BINARY, METHOD_CGGI_TERNARY, METHOD_BMMP = 0, 1, 2
+
+u64p = ctypes.POINTER(ctypes.c_uint64)
+vp = ctypes.c_void_p
+U32, U64, SZ, INT = ctypes.c_uint32, ctypes.c_uint64, ctypes.c_size_t, ctypes.c_int
+
+
+class ToothpasteError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("toothpaste_amd error %d: %s" % (code, msg))
+        self.code = code
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("ring_dim", U32), ("lwe_dim", U32), ("br_base_log2", U32), ("br_digits", U32),
+                ("method", U32), ("ks_out_dim", U32), ("ks_base_log2", U32), ("ks_digits", U32)]
+
+
+# name -> argtypes (all return int status)
+SIGNATURES = {
+    "tp_device_count": [ctypes.POINTER(INT)],
+    "tp_context_create": [ctypes.POINTER(vp), ctypes.POINTER(Params), INT],
+    "tp_context_destroy": [vp],
+    "tp_context_sync": [vp],
+    "tp_import_brk": [vp, u64p, SZ],
+    "tp_import_lwe_ksk": [vp, u64p, SZ],
+    "tp_ntt_forward_device": [vp, vp, vp, SZ],
+    "tp_ntt_inverse_device": [vp, vp, vp, SZ],
+    "tp_blind_rotate": [vp, u64p, u64p, u64p, INT],
+    "tp_eval_func_batch_device": [vp, SZ, vp, vp, SZ, vp, vp, INT],
+    "tp_lwe_keyswitch_batch_device": [vp, SZ, vp, SZ, vp],
+    "tp_bootstrap_batch_device": [vp, SZ, vp, vp, SZ, vp],
+    "tp_malloc": [vp, ctypes.POINTER(vp), SZ],
+    "tp_free": [vp, vp],
+    "tp_copy_to_device": [vp, vp, u64p, SZ],
+    "tp_copy_to_host": [vp, u64p, vp, SZ],
+    "tp_timer_mark": [vp, INT],
+    "tp_timer_elapsed": [vp, INT, INT, ctypes.POINTER(ctypes.c_float)],
+    "tp_keygen_secret": [u64p, U32, INT, U64, U64],
+    "tp_keygen_brk": [u64p, u64p, u64p, U32, U32, U32, U32, INT, INT, ctypes.c_double, U64],
+    "tp_keygen_lwe_ksk": [u64p, u64p, u64p, U32, U32, U32, U32, ctypes.c_double, U64],
+}
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ToothpasteError(TP_E_DEVICE, "libtoothpaste_amd.so is not built (python -m fhetoothpaste_amd.build)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, args in SIGNATURES.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = INT
+        L.tp_last_error.argtypes = []
+        L.tp_last_error.restype = ctypes.c_char_p
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != TP_OK:
+        raise ToothpasteError(rc, lib().tp_last_error().decode())
+    return rc
+
+
+def ptr(a):
+    """uint64* of a C-contiguous numpy uint64 array."""
+    if a is None:
+        return None
+    if a.dtype != np.uint64 or not a.flags["C_CONTIGUOUS"]:
+        raise ValueError("expected a C-contiguous uint64 array")
+    return a.ctypes.data_as(u64p)
+
+
+def device_count():
+    n = INT(0)
+    check(lib().tp_device_count(ctypes.byref(n)))
+    return n.value

@@ -1,0 +1,51 @@
+"""Builds libtoothpaste_amd.so in-tree for gfx950 with hipcc (no CMake, no JIT cache).
+
+    python -m fhetoothpaste_amd.build
+"""
+import os
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+OUT = os.path.join(HERE, "libtoothpaste_amd.so")
+BUILD_DIR = os.path.join(HERE, "_build")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+# Packed-FP32 instructions are switched off for all device code (none is wanted on this path).
+DEVICE_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
+                "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall"]
+
+SOURCES = [
+    ("kernels.hip", DEVICE_FLAGS),
+    ("capi.cpp", HOST_FLAGS),
+    ("keygen.cpp", HOST_FLAGS),
+    ("dropin.cpp", HOST_FLAGS),
+]
+
+
+def _compile(src, flags):
+    obj = os.path.join(BUILD_DIR, os.path.splitext(src)[0] + ".o")
+    cmd = [HIPCC] + flags + ["-c", os.path.join(CSRC, src), "-o", obj]
+    subprocess.check_call(cmd)
+    return obj
+
+
+def build(verbose=True):
+    os.makedirs(BUILD_DIR, exist_ok=True)
+    srcs = [(s, f) for s, f in SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    with ThreadPoolExecutor(max_workers=4) as ex:
+        objs = list(ex.map(lambda sf: _compile(*sf), srcs))
+    tmp = OUT + ".tmp"
+    subprocess.check_call([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs)
+    os.replace(tmp, OUT)
+    if verbose:
+        print("built", OUT)
+    return OUT
+
+
+if __name__ == "__main__":
+    build()
+    sys.exit(0)

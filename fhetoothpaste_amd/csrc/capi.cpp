@@ -1,0 +1,426 @@
+// C-ABI of the gfx950 bootstrapping path: contexts, key import (reference layouts ->
+// device layouts), launches.  Declared in include/toothpaste_amd.h.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/toothpaste_amd.h"
+#include "gl64.hpp"
+#include "host_ntt.hpp"
+#include "kernels.hpp"
+
+namespace {
+
+thread_local std::string g_error;
+
+int fail(int code, const std::string& msg) {
+    g_error = msg;
+    return code;
+}
+
+#define TP_HIP(call)                                                                            \
+    do {                                                                                        \
+        hipError_t e_ = (call);                                                                 \
+        if (e_ != hipSuccess) return fail(TP_E_DEVICE, std::string(#call ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+uint32_t max_digits(uint32_t b) { return 64 / b + (64 % b ? 1 : 0); }
+
+int ilog2(uint32_t x) {
+    int r = 0;
+    while ((1u << r) < x) ++r;
+    return r;
+}
+
+}  // namespace
+
+struct tp_context {
+    tp_params p{};
+    int device = 0;
+    uint32_t N = 0;
+    int logN = 0;
+    hipStream_t stream = nullptr;
+    uint64_t* d_tw = nullptr;
+    uint64_t* d_itw = nullptr;
+    uint64_t* d_brk = nullptr;
+    size_t brk_words = 0;
+    uint64_t* d_ksk = nullptr;
+    uint64_t* d_corr = nullptr;
+    size_t ksk_words = 0;
+    uint64_t ks_corrector = 0;
+    uint64_t* d_scratch = nullptr;  // extracted LWEs of tp_bootstrap_batch_device
+    size_t scratch_words = 0;
+    uint64_t* d_io = nullptr;  // staging of the host-pointer entry points
+    uint64_t* d_mono = nullptr;  // NTT(X^e) table (ternary, BMMP)
+    uint64_t br_corrector = 0, br_least = 0;
+    hipEvent_t ev[8] = {};
+};
+
+namespace {
+
+int set_device(tp_context* c) {
+    TP_HIP(hipSetDevice(c->device));
+    return TP_OK;
+}
+
+size_t brk_words_for(const tp_params& p) {
+    const size_t rg = (size_t)4 * p.ring_dim * p.br_digits;
+    switch (p.method) {
+        case TP_METHOD_CGGI_BINARY: return rg * p.lwe_dim;
+        case TP_METHOD_CGGI_TERNARY: return 2 * rg * p.lwe_dim;
+        case TP_METHOD_BMMP: return 3 * rg * (p.lwe_dim / 2);
+    }
+    return 0;
+}
+
+int ensure_scratch(tp_context* c, size_t words) {
+    if (c->scratch_words >= words) return TP_OK;
+    if (c->d_scratch) TP_HIP(hipFree(c->d_scratch));
+    c->d_scratch = nullptr;
+    c->scratch_words = 0;
+    if (hipMalloc(&c->d_scratch, words * 8) != hipSuccess) return fail(TP_E_NOMEM, "scratch allocation failed");
+    c->scratch_words = words;
+    return TP_OK;
+}
+
+int launch_eval(tp_context* c, size_t count, const uint64_t* lwes, const uint64_t* tv, size_t tv_stride,
+                uint64_t* out_lwe, uint64_t* out_rlwe, int rlwe_ntt, int grid) {
+    if (!c->d_brk) return fail(TP_E_STATE, "blind-rotation key not imported");
+    if (count == 0) return TP_OK;
+    if (count > 0x7fffffff) return fail(TP_E_INVALID, "count too large");
+    if (tv_stride != 0 && tv_stride != 2 * (size_t)c->N) return fail(TP_E_INVALID, "tv_stride must be 0 or 2N");
+    tp::BrArgs A{};
+    A.brk = c->d_brk;
+    A.lwe = lwes;
+    A.tv = tv;
+    A.tv_stride = tv_stride;
+    A.out_lwe = out_lwe;
+    A.out_rlwe = out_rlwe;
+    A.out_rlwe_ntt = rlwe_ntt;
+    A.tw = c->d_tw;
+    A.itw = c->d_itw;
+    A.n = (int)c->p.lwe_dim;
+    A.base_log2 = (int)c->p.br_base_log2;
+    A.first_shift = (int)(c->p.br_base_log2 * (max_digits(c->p.br_base_log2) - 1));
+    A.count = (int)count;
+    A.corrector = c->br_corrector;
+    A.least = c->br_least;
+    A.max_digits = (int)max_digits(c->p.br_base_log2);
+    A.mono = c->d_mono;
+    if (grid <= 0) grid = (int)count;
+    hipError_t e = tp::launch_blind_rotate(c->logN, (int)c->p.method, (int)c->p.br_digits, A, grid, c->stream);
+    if (e != hipSuccess) return fail(TP_E_DEVICE, std::string("blind rotation launch: ") + hipGetErrorString(e));
+    return TP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* tp_last_error(void) { return g_error.c_str(); }
+
+int tp_device_count(int* count) {
+    if (!count) return fail(TP_E_INVALID, "null count");
+    *count = 0;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return TP_OK;
+    for (int d = 0; d < n; ++d) {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, d) == hipSuccess && std::strncmp(prop.gcnArchName, "gfx950", 6) == 0)
+            ++*count;
+    }
+    return TP_OK;
+}
+
+int tp_context_create(tp_context** out, const tp_params* p, int device) {
+    if (!out || !p) return fail(TP_E_INVALID, "null argument");
+    *out = nullptr;
+    const uint32_t N = p->ring_dim;
+    if (N != 1024 && N != 2048 && N != 4096) return fail(TP_E_INVALID, "ring_dim must be 1024, 2048 or 4096");
+    if (p->method > TP_METHOD_BMMP) return fail(TP_E_INVALID, "unknown method");
+    if (p->br_base_log2 == 0 || p->br_base_log2 > 32 || p->br_digits == 0 ||
+        p->br_digits > max_digits(p->br_base_log2))
+        return fail(TP_E_INVALID, "bad gadget parameters");
+    if (p->method == TP_METHOD_BMMP && (p->lwe_dim & 1)) return fail(TP_E_INVALID, "BMMP needs an even lwe_dim");
+    if (p->ks_out_dim) {
+        const uint32_t b = p->ks_base_log2, d = p->ks_digits;
+        if (b == 0 || b > 16 || d == 0 || d > max_digits(b) || (d == max_digits(b) && 64 % b))
+            return fail(TP_E_INVALID, "bad key-switch gadget parameters");
+        if ((uint64_t)N * d << b >= (1ULL << 32)) return fail(TP_E_INVALID, "key-switch accumulator bound");
+    }
+    hipDeviceProp_t prop;
+    hipError_t e = hipGetDeviceProperties(&prop, device);
+    if (e != hipSuccess) return fail(TP_E_DEVICE, std::string("no device: ") + hipGetErrorString(e));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(TP_E_DEVICE, std::string("device is not gfx950: ") + prop.gcnArchName);
+    tp_context* c = new tp_context;
+    c->p = *p;
+    c->device = device;
+    c->N = N;
+    c->logN = ilog2(N);
+    int rc = set_device(c);
+    if (rc) { delete c; return rc; }
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return fail(TP_E_DEVICE, "stream creation failed");
+    }
+    tp::HostNtt T(N);
+    if (hipMalloc(&c->d_tw, N * 8) != hipSuccess || hipMalloc(&c->d_itw, N * 8) != hipSuccess ||
+        hipMalloc(&c->d_io, (size_t)8 * N * 8) != hipSuccess) {
+        tp_context_destroy(c);
+        return fail(TP_E_NOMEM, "table allocation failed");
+    }
+    hipMemcpy(c->d_tw, T.psi_rev.data(), N * 8, hipMemcpyHostToDevice);
+    hipMemcpy(c->d_itw, T.psi_inv_rev.data(), N * 8, hipMemcpyHostToDevice);
+    for (auto& ev : c->ev) hipEventCreate(&ev);
+    {
+        // signed-digit corrector (gadget_decomp.h:16-28)
+        const uint32_t b = p->br_base_log2;
+        uint64_t C = 0;
+        int64_t shift = 64 - (int64_t)b;
+        while (shift > 0) { C += 1ULL << (shift + b - 1); shift -= b; }
+        c->br_least = 1ULL << (shift + b - 1);
+        C += c->br_least;
+        c->br_corrector = C % gl::kQ;
+    }
+    if (p->method != TP_METHOD_CGGI_BINARY) {
+        // NTT(X^e)[pos] = psi^((2 brv(pos) + 1) e), stored in the lane-major key-row order
+        const uint32_t P = N / 16;
+        std::vector<uint64_t> mono((size_t)N * N);
+        const uint64_t psi = gl::pow(7, (gl::kQ - 1) >> (c->logN + 1));
+        for (uint32_t pos = 0; pos < N; ++pos) {
+            const uint64_t base = gl::mul(gl::mul(T.psi_rev[pos], T.psi_rev[pos]), psi);
+            const uint32_t L = pos / 16, k = pos % 16;
+            const size_t dpos = (size_t)(k >> 1) * 2 * P + 2 * L + (k & 1);
+            uint64_t v = 1;
+            for (uint32_t e = 0; e < N; ++e) {
+                mono[(size_t)e * N + dpos] = v;
+                v = gl::mul(v, base);
+            }
+        }
+        if (hipMalloc(&c->d_mono, mono.size() * 8) != hipSuccess) {
+            tp_context_destroy(c);
+            return fail(TP_E_NOMEM, "monomial table allocation failed");
+        }
+        hipMemcpy(c->d_mono, mono.data(), mono.size() * 8, hipMemcpyHostToDevice);
+    }
+    *out = c;
+    return TP_OK;
+}
+
+int tp_context_destroy(tp_context* c) {
+    if (!c) return TP_OK;
+    hipSetDevice(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    for (uint64_t* p : {c->d_tw, c->d_itw, c->d_brk, c->d_ksk, c->d_corr, c->d_scratch, c->d_io, c->d_mono})
+        if (p) hipFree(p);
+    for (auto& ev : c->ev)
+        if (ev) hipEventDestroy(ev);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return TP_OK;
+}
+
+int tp_context_sync(tp_context* c) {
+    if (!c) return fail(TP_E_INVALID, "null context");
+    TP_HIP(hipStreamSynchronize(c->stream));
+    return TP_OK;
+}
+
+int tp_import_brk(tp_context* c, const uint64_t* brk, size_t words) {
+    if (!c || !brk) return fail(TP_E_INVALID, "null argument");
+    const size_t want = brk_words_for(c->p);
+    if (words != want) return fail(TP_E_INVALID, "brk size mismatch: expected " + std::to_string(want));
+    int rc = set_device(c);
+    if (rc) return rc;
+    // Permute every N-word key row into the lane-major order of the kernels:
+    // NTT index 16 L + k goes to ((k >> 1) * 2P + 2L + (k & 1)).
+    const uint32_t N = c->N, P = N / 16;
+    std::vector<uint64_t> dev(words);
+    for (size_t r = 0; r < words / N; ++r) {
+        const uint64_t* src = brk + r * N;
+        uint64_t* dst = dev.data() + r * N;
+        for (uint32_t L = 0; L < P; ++L)
+            for (uint32_t k = 0; k < 16; ++k) dst[(k >> 1) * 2 * P + 2 * L + (k & 1)] = src[16 * L + k];
+    }
+    if (c->d_brk) { hipFree(c->d_brk); c->d_brk = nullptr; }
+    if (hipMalloc(&c->d_brk, words * 8) != hipSuccess) return fail(TP_E_NOMEM, "brk allocation failed");
+    TP_HIP(hipMemcpy(c->d_brk, dev.data(), words * 8, hipMemcpyHostToDevice));
+    c->brk_words = words;
+    return TP_OK;
+}
+
+int tp_import_lwe_ksk(tp_context* c, const uint64_t* ksk, size_t words) {
+    if (!c || !ksk) return fail(TP_E_INVALID, "null argument");
+    if (!c->p.ks_out_dim) return fail(TP_E_INVALID, "context has no key switch (ks_out_dim = 0)");
+    const uint32_t n_in = c->N, n_out = c->p.ks_out_dim, b = c->p.ks_base_log2, d = c->p.ks_digits;
+    const size_t W = n_out + 1;
+    if (words != (size_t)n_in * d * W) return fail(TP_E_INVALID, "ksk size mismatch");
+    int rc = set_device(c);
+    if (rc) return rc;
+    // corr[k] = sum_ij delta_j K[i][j][k]: the signed digits are d_ij = u_ij - delta_j.
+    const uint32_t D = max_digits(b);
+    const uint64_t least = 1ULL << ((64 - (int)b * (int)D) + (int)b - 1);
+    std::vector<uint64_t> colsum((size_t)d * W, 0), corr(W, 0);
+    for (uint32_t i = 0; i < n_in; ++i)
+        for (uint32_t j = 0; j < d; ++j) {
+            const uint64_t* row = ksk + ((size_t)i * d + j) * W;
+            uint64_t* cs = colsum.data() + (size_t)j * W;
+            for (size_t k = 0; k < W; ++k) cs[k] = gl::add(cs[k], row[k]);
+        }
+    for (uint32_t j = 0; j < d; ++j) {
+        const uint64_t delta = (j != D - 1) ? (1ULL << (b - 1)) : least;
+        for (size_t k = 0; k < W; ++k) corr[k] = gl::add(corr[k], gl::mul(colsum[(size_t)j * W + k], delta));
+    }
+    uint64_t C = 0;
+    int64_t shift = 64 - (int64_t)b;
+    while (shift > 0) { C += 1ULL << (shift + b - 1); shift -= b; }
+    C += 1ULL << (shift + b - 1);
+    c->ks_corrector = C % gl::kQ;
+    if (c->d_ksk) { hipFree(c->d_ksk); c->d_ksk = nullptr; }
+    if (c->d_corr) { hipFree(c->d_corr); c->d_corr = nullptr; }
+    if (hipMalloc(&c->d_ksk, words * 8) != hipSuccess || hipMalloc(&c->d_corr, W * 8) != hipSuccess)
+        return fail(TP_E_NOMEM, "ksk allocation failed");
+    TP_HIP(hipMemcpy(c->d_ksk, ksk, words * 8, hipMemcpyHostToDevice));
+    TP_HIP(hipMemcpy(c->d_corr, corr.data(), W * 8, hipMemcpyHostToDevice));
+    c->ksk_words = words;
+    return TP_OK;
+}
+
+int tp_ntt_forward_device(tp_context* c, uint64_t* dst, const uint64_t* src, size_t count) {
+    if (!c || !dst || !src) return fail(TP_E_INVALID, "null argument");
+    int rc = set_device(c);
+    if (rc) return rc;
+    hipError_t e = tp::launch_ntt_batch(c->logN, dst, src, count, c->d_tw, false, c->stream);
+    if (e != hipSuccess) return fail(TP_E_DEVICE, std::string("ntt launch: ") + hipGetErrorString(e));
+    return TP_OK;
+}
+
+int tp_ntt_inverse_device(tp_context* c, uint64_t* dst, const uint64_t* src, size_t count) {
+    if (!c || !dst || !src) return fail(TP_E_INVALID, "null argument");
+    int rc = set_device(c);
+    if (rc) return rc;
+    hipError_t e = tp::launch_ntt_batch(c->logN, dst, src, count, c->d_itw, true, c->stream);
+    if (e != hipSuccess) return fail(TP_E_DEVICE, std::string("ntt launch: ") + hipGetErrorString(e));
+    return TP_OK;
+}
+
+int tp_blind_rotate(tp_context* c, uint64_t* result, const uint64_t* lwe, const uint64_t* acc, int as_coef) {
+    if (!c || !result || !lwe || !acc) return fail(TP_E_INVALID, "null argument");
+    int rc = set_device(c);
+    if (rc) return rc;
+    const uint32_t N = c->N, n = c->p.lwe_dim;
+    for (uint32_t i = 0; i <= n; ++i)
+        if (lwe[i] >= 2 * (uint64_t)N) return fail(TP_E_INVALID, "lwe entries must lie in [0, 2N)");
+    rc = ensure_scratch(c, (size_t)n + 1 + 4 * N);
+    if (rc) return rc;
+    uint64_t* d_lwe = c->d_scratch;
+    uint64_t* d_acc = d_lwe + n + 1;
+    uint64_t* d_res = d_acc + 2 * N;
+    TP_HIP(hipMemcpyAsync(d_lwe, lwe, (n + 1) * 8, hipMemcpyHostToDevice, c->stream));
+    TP_HIP(hipMemcpyAsync(d_acc, acc, 2 * N * 8, hipMemcpyHostToDevice, c->stream));
+    rc = launch_eval(c, 1, d_lwe, d_acc, 0, nullptr, d_res, as_coef ? 0 : 1, 1);
+    if (rc) return rc;
+    TP_HIP(hipMemcpyAsync(result, d_res, 2 * N * 8, hipMemcpyDeviceToHost, c->stream));
+    TP_HIP(hipStreamSynchronize(c->stream));
+    return TP_OK;
+}
+
+int tp_eval_func_batch_device(tp_context* c, size_t count, const uint64_t* lwes, const uint64_t* tv,
+                              size_t tv_stride, uint64_t* out_lwe, uint64_t* out_rlwe, int rlwe_ntt) {
+    if (!c || !lwes || !tv) return fail(TP_E_INVALID, "null argument");
+    int rc = set_device(c);
+    if (rc) return rc;
+    return launch_eval(c, count, lwes, tv, tv_stride, out_lwe, out_rlwe, rlwe_ntt, 0);
+}
+
+int tp_lwe_keyswitch_batch_device(tp_context* c, size_t count, const uint64_t* in, size_t in_stride,
+                                  uint64_t* out) {
+    if (!c || !in || !out) return fail(TP_E_INVALID, "null argument");
+    if (!c->d_ksk) return fail(TP_E_STATE, "key-switching key not imported");
+    if (in_stride < (size_t)c->N + 1) return fail(TP_E_INVALID, "in_stride < N + 1");
+    if (count == 0) return TP_OK;
+    int rc = set_device(c);
+    if (rc) return rc;
+    tp::KsArgs A{};
+    A.ksk = c->d_ksk;
+    A.corr = c->d_corr;
+    A.in = in;
+    A.out = out;
+    A.in_stride = in_stride;
+    A.corrector = c->ks_corrector;
+    A.n_in = (int)c->N;
+    A.n_out = (int)c->p.ks_out_dim;
+    A.base_log2 = (int)c->p.ks_base_log2;
+    A.digits = (int)c->p.ks_digits;
+    A.count = (int)count;
+    hipError_t e = tp::launch_lwe_keyswitch(A, c->stream);
+    if (e != hipSuccess) return fail(TP_E_DEVICE, std::string("key switch launch: ") + hipGetErrorString(e));
+    return TP_OK;
+}
+
+int tp_bootstrap_batch_device(tp_context* c, size_t count, const uint64_t* lwes, const uint64_t* tv,
+                              size_t tv_stride, uint64_t* out) {
+    if (!c || !lwes || !tv || !out) return fail(TP_E_INVALID, "null argument");
+    if (!c->d_ksk) return fail(TP_E_STATE, "key-switching key not imported");
+    int rc = set_device(c);
+    if (rc) return rc;
+    rc = ensure_scratch(c, count * (c->N + 1));
+    if (rc) return rc;
+    rc = launch_eval(c, count, lwes, tv, tv_stride, c->d_scratch, nullptr, 0, 0);
+    if (rc) return rc;
+    return tp_lwe_keyswitch_batch_device(c, count, c->d_scratch, c->N + 1, out);
+}
+
+int tp_malloc(tp_context* c, uint64_t** ptr, size_t words) {
+    if (!c || !ptr) return fail(TP_E_INVALID, "null argument");
+    int rc = set_device(c);
+    if (rc) return rc;
+    if (hipMalloc(ptr, words ? words * 8 : 8) != hipSuccess) return fail(TP_E_NOMEM, "device allocation failed");
+    return TP_OK;
+}
+
+int tp_free(tp_context* c, uint64_t* ptr) {
+    if (!c) return fail(TP_E_INVALID, "null context");
+    int rc = set_device(c);
+    if (rc) return rc;
+    TP_HIP(hipStreamSynchronize(c->stream));
+    if (ptr) TP_HIP(hipFree(ptr));
+    return TP_OK;
+}
+
+int tp_copy_to_device(tp_context* c, uint64_t* dst, const uint64_t* src, size_t words) {
+    if (!c || !dst || !src) return fail(TP_E_INVALID, "null argument");
+    int rc = set_device(c);
+    if (rc) return rc;
+    TP_HIP(hipMemcpyAsync(dst, src, words * 8, hipMemcpyHostToDevice, c->stream));
+    TP_HIP(hipStreamSynchronize(c->stream));
+    return TP_OK;
+}
+
+int tp_copy_to_host(tp_context* c, uint64_t* dst, const uint64_t* src, size_t words) {
+    if (!c || !dst || !src) return fail(TP_E_INVALID, "null argument");
+    int rc = set_device(c);
+    if (rc) return rc;
+    TP_HIP(hipMemcpyAsync(dst, src, words * 8, hipMemcpyDeviceToHost, c->stream));
+    TP_HIP(hipStreamSynchronize(c->stream));
+    return TP_OK;
+}
+
+int tp_timer_mark(tp_context* c, int slot) {
+    if (!c || slot < 0 || slot >= 8) return fail(TP_E_INVALID, "bad timer slot");
+    TP_HIP(hipEventRecord(c->ev[slot], c->stream));
+    return TP_OK;
+}
+
+int tp_timer_elapsed(tp_context* c, int a, int b, float* ms) {
+    if (!c || !ms || a < 0 || a >= 8 || b < 0 || b >= 8) return fail(TP_E_INVALID, "bad timer slot");
+    TP_HIP(hipEventSynchronize(c->ev[b]));
+    TP_HIP(hipEventElapsedTime(ms, c->ev[a], c->ev[b]));
+    return TP_OK;
+}
+
+}  // extern "C"

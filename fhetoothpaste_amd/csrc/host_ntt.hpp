@@ -1,0 +1,51 @@
+// Host-side negacyclic NTT tables and a scalar forward transform (key generation only;
+// the hot path runs on the GPU).  Same convention as the reference
+// (lib/src/backend/solinas.cpp:168-261): psi = 7^((Q-1)/2N), psi_rev[i] = psi^brv(i),
+// natural order in, bit-reversed order out.
+#pragma once
+#include <stdint.h>
+#include <vector>
+
+#include "gl64.hpp"
+
+namespace tp {
+
+struct HostNtt {
+    uint32_t N = 0, logN = 0;
+    std::vector<uint64_t> psi_rev, psi_inv_rev;
+
+    explicit HostNtt(uint32_t n) : N(n) {
+        while ((1u << logN) < N) ++logN;
+        const uint64_t psi = gl::pow(7, (gl::kQ - 1) >> (logN + 1));
+        const uint64_t psi_inv = gl::inv(psi);
+        std::vector<uint64_t> pw(N), pwi(N);
+        pw[0] = pwi[0] = 1;
+        for (uint32_t i = 1; i < N; ++i) {
+            pw[i] = gl::mul(pw[i - 1], psi);
+            pwi[i] = gl::mul(pwi[i - 1], psi_inv);
+        }
+        psi_rev.resize(N);
+        psi_inv_rev.resize(N);
+        for (uint32_t i = 0; i < N; ++i) {
+            uint32_t r = 0, x = i;
+            for (uint32_t b = 0; b < logN; ++b) { r = (r << 1) | (x & 1); x >>= 1; }
+            psi_rev[i] = pw[r];
+            psi_inv_rev[i] = pwi[r];
+        }
+    }
+
+    void forward(uint64_t* a) const {
+        for (uint32_t m = 1, t = N / 2; m < N; m <<= 1, t >>= 1)
+            for (uint32_t i = 0; i < m; ++i) {
+                const uint64_t w = psi_rev[m + i];
+                uint64_t* p = a + 2 * i * t;
+                for (uint32_t j = 0; j < t; ++j) {
+                    const uint64_t u = p[j], v = gl::mul(p[j + t], w);
+                    p[j] = gl::add(u, v);
+                    p[j + t] = gl::sub(u, v);
+                }
+            }
+    }
+};
+
+}  // namespace tp

@@ -1,0 +1,374 @@
+// HIP kernels for gfx950: batched NTT, CGGI blind rotation + sample extraction,
+// batched LWE->LWE key switching.
+#include <hip/hip_runtime.h>
+#include "gl64.hpp"
+#include "ntt_wave.hpp"
+#include "kernels.hpp"
+
+namespace tp {
+
+// Position of NTT index 16 L + k inside a device key row: pairs (2c, 2c+1) of lane L are
+// adjacent and lanes are contiguous, so a wave reads one row with 16-byte coalesced loads.
+template <int LOGN>
+__device__ __forceinline__ int key_pos(int L, int k) {
+    return ((k >> 1) * (2 * Shape<LOGN>::P)) + 2 * L + (k & 1);
+}
+
+struct alignas(16) U64x2 {
+    uint64_t x, y;
+};
+
+template <int LOGN>
+__device__ __forceinline__ void load_key_row(uint64_t (&kv)[kE], const uint64_t* __restrict__ row, int L) {
+    const U64x2* r = reinterpret_cast<const U64x2*>(row);
+#pragma unroll
+    for (int c = 0; c < kE / 2; ++c) {
+        const U64x2 v = r[c * Shape<LOGN>::P + L];
+        kv[2 * c] = v.x;
+        kv[2 * c + 1] = v.y;
+    }
+}
+
+// NTT(X^e) - 1 at the lane's 16 NTT positions, e in [0, 2N): the table holds NTT(X^e) for
+// e < N in key-row order; X^e = -X^(e-N) for e >= N.
+template <int LOGN>
+__device__ __forceinline__ void load_mono(uint64_t (&w)[kE], const uint64_t* __restrict__ mono, uint32_t e, int L) {
+    constexpr int N = Shape<LOGN>::N;
+    const bool ng = e >= (uint32_t)N;
+    load_key_row<LOGN>(w, mono + (size_t)(e & (N - 1)) * N, L);
+    if (ng) {
+#pragma unroll
+        for (int k = 0; k < kE; ++k) w[k] = gl::neg(w[k]);
+    }
+}
+
+template <int LOGN>
+__device__ __forceinline__ void load_weight(uint64_t (&w)[kE], const uint64_t* __restrict__ mono, uint32_t e, int L) {
+    load_mono<LOGN>(w, mono, e, L);
+#pragma unroll
+    for (int k = 0; k < kE; ++k) w[k] = gl::sub(w[k], 1);
+}
+
+// BMMP third weight X^(a0+a1) - X^a0 - X^a1 + 1 (bmmp_blind_rotation.cpp:280-323).
+template <int LOGN>
+__device__ __forceinline__ void load_bmmp_w2(uint64_t (&w)[kE], const uint64_t* __restrict__ mono, uint32_t e0,
+                                             uint32_t e1, uint32_t e2, int L) {
+    uint64_t t[kE];
+    load_mono<LOGN>(w, mono, e2, L);
+    load_mono<LOGN>(t, mono, e0, L);
+#pragma unroll
+    for (int k = 0; k < kE; ++k) w[k] = gl::sub(w[k], t[k]);
+    load_mono<LOGN>(t, mono, e1, L);
+#pragma unroll
+    for (int k = 0; k < kE; ++k) w[k] = gl::add(gl::sub(w[k], t[k]), 1);
+}
+
+// ------------------------------------------------------------------ batched NTT
+
+template <int LOGN, bool INVERSE>
+__global__ void __launch_bounds__(256) ntt_batch_kernel(uint64_t* __restrict__ out, const uint64_t* __restrict__ in,
+                                                        size_t count, const uint64_t* __restrict__ tw) {
+    using S = Shape<LOGN>;
+    constexpr int POLYS = 256 / S::P;
+    __shared__ uint64_t lds[POLYS][S::LDS_WORDS];
+    const int g = threadIdx.x / S::P, L = threadIdx.x % S::P;
+    for (size_t base = (size_t)blockIdx.x * POLYS; base < count; base += (size_t)gridDim.x * POLYS) {
+        const size_t p = base + g;
+        const bool live = p < count;
+        const uint64_t* src = in + (live ? p : 0) * S::N;
+        uint64_t x[kE];
+        if (!INVERSE) {
+#pragma unroll
+            for (int k = 0; k < kE; ++k) x[k] = src[idx1<LOGN>(L, k)];
+            ntt_forward<LOGN>(x, lds[g], tw, L);
+            if (live)
+#pragma unroll
+                for (int k = 0; k < kE; ++k) out[p * S::N + idx3(L, k)] = x[k];
+        } else {
+#pragma unroll
+            for (int k = 0; k < kE; ++k) x[k] = src[idx3(L, k)];
+            ntt_inverse<LOGN>(x, lds[g], tw, L);
+            if (live)
+#pragma unroll
+                for (int k = 0; k < kE; ++k) out[p * S::N + idx1<LOGN>(L, k)] = x[k];
+        }
+    }
+}
+
+// ------------------------------------------------------------------ blind rotation
+
+// acc_h[k] = coefficient idx1(L,k) of X^e * src(X) (negacyclic), e in [0, 2N).
+template <int LOGN>
+__device__ __forceinline__ void rotate_from(uint64_t (&acc)[kE], const uint64_t* src, uint32_t e, int L,
+                                            bool lds_src) {
+    constexpr int N = Shape<LOGN>::N;
+    const bool flip = e >= (uint32_t)N;
+    const int r = (int)(e & (N - 1));
+#pragma unroll
+    for (int k = 0; k < kE; ++k) {
+        int j = idx1<LOGN>(L, k) - r;
+        bool ng = flip;
+        if (j < 0) { j += N; ng = !ng; }
+        const uint64_t v = lds_src ? src[phys(j)] : src[j];
+        acc[k] = ng ? gl::neg(v) : v;
+    }
+}
+
+// Blind rotation, one ciphertext per work-group of P = N/16 lanes (one wave at N = 1024).
+// The accumulator is kept in the coefficient domain, in registers (lane L holds
+// coefficients L + P k of both halves); every step is exact in Z_Q, so the result is
+// bit-identical to the reference's NTT-domain formulation:
+//
+// CGGI binary (cggi_blind_rotation.cpp:313-386, mux_operator.cpp:30-152):
+//   acc <- acc + INTT( sum_r NTT(udigit_r((X^-a_i - 1) acc)) * BRK_i[r] )   for a_i != 0
+//   (the reference multiplies NTT(acc) by NTT(X^-a_i) - NTT(1) before its INTT; here the
+//   monomial product is a signed rotation of the coefficient vector).
+// CGGI ternary (cggi_blind_rotation.cpp:388-472, mux_operator.cpp:156-202) and BMMP
+// (bmmp_blind_rotation.cpp:354-419, mux_operator.cpp:204-294):
+//   acc <- acc + INTT( sum_r NTT(sdigit_r(acc)) * sum_c w_c * RGSW_c[r] )
+//   with w = (X^-a - 1, X^a - 1) (ternary) or (X^a0 - 1, X^a1 - 1, X^(a0+a1) - X^a0 - X^a1 + 1)
+//   (BMMP); sum_c w_c (RGSW_c [x] acc) equals the reference's per-control products, weighted
+//   and summed, in Z_Q.
+template <int LOGN, int LD, int METHOD>
+__global__ void __launch_bounds__(Shape<LOGN>::P) br_kernel(BrArgs A) {
+    using S = Shape<LOGN>;
+    constexpr int N = S::N;
+    constexpr int NC = METHOD == 0 ? 1 : (METHOD == 1 ? 2 : 3);  // RGSWs per step
+    __shared__ uint64_t buf[S::LDS_WORDS];
+    const int L = threadIdx.x;
+    const uint64_t mask = (1ULL << A.base_log2) - 1;
+    const size_t rgsw_words = (size_t)4 * N * LD;
+
+    for (int ct = blockIdx.x; ct < A.count; ct += gridDim.x) {
+        const uint64_t* lwe = A.lwe + (size_t)ct * (A.n + 1);
+        const uint64_t* tv = A.tv + (size_t)ct * A.tv_stride;
+        uint64_t acc[2][kE];
+        const uint32_t bexp = (uint32_t)lwe[A.n];
+        rotate_from<LOGN>(acc[0], tv, bexp, L, false);
+        rotate_from<LOGN>(acc[1], tv + N, bexp, L, false);
+
+        const int steps = METHOD == 2 ? A.n / 2 : A.n;
+        for (int i = 0; i < steps; ++i) {
+            // monomial exponents of this step (NTT(X^e) rows of the table, e in [0, 2N))
+            uint32_t e0 = 0, e1 = 0, e2 = 0;
+            if constexpr (METHOD == 0 || METHOD == 1) {
+                const uint32_t ai = (uint32_t)lwe[i];
+                if (ai == 0) continue;
+                e0 = 2 * N - ai;
+                e1 = ai;
+            } else {
+                e0 = (2 * N - (uint32_t)lwe[2 * i]) & (2 * N - 1);
+                e1 = (2 * N - (uint32_t)lwe[2 * i + 1]) & (2 * N - 1);
+                e2 = (e0 + e1) & (2 * N - 1);
+            }
+            const uint64_t* key = A.brk + (size_t)i * NC * rgsw_words;
+            uint64_t out[2][kE];
+#pragma unroll
+            for (int k = 0; k < kE; ++k) out[0][k] = out[1][k] = 0;
+
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                uint64_t x[kE];
+                if constexpr (METHOD == 0) {
+                    // x = X^e0 acc_h - acc_h, through LDS
+                    poly_sync<S::WAVES>();
+#pragma unroll
+                    for (int k = 0; k < kE; ++k) buf[phys(idx1<LOGN>(L, k))] = acc[h][k];
+                    poly_sync<S::WAVES>();
+                    rotate_from<LOGN>(x, buf, e0, L, true);
+#pragma unroll
+                    for (int k = 0; k < kE; ++k) x[k] = gl::sub(x[k], acc[h][k]);
+                } else {
+                    // signed digits work on x + C mod Q (gadget_decomp.h:169-173)
+#pragma unroll
+                    for (int k = 0; k < kE; ++k) x[k] = gl::add(acc[h][k], A.corrector);
+                }
+                for (int j = 0; j < LD; ++j) {
+                    uint64_t d[kE];
+                    if constexpr (METHOD == 0) {
+                        const int shift = A.first_shift - j * A.base_log2;
+#pragma unroll
+                        for (int k = 0; k < kE; ++k) d[k] = (x[k] >> shift) & mask;
+                    } else {
+                        const bool last = (j == A.max_digits - 1);
+                        const int shift = last ? 0 : 64 - (j + 1) * A.base_log2;
+                        const uint64_t dm = last ? (2 * A.least - 1) : mask;
+                        const uint64_t delta = last ? A.least : (1ULL << (A.base_log2 - 1));
+#pragma unroll
+                        for (int k = 0; k < kE; ++k) d[k] = gl::sub((x[k] >> shift) & dm, delta);
+                    }
+                    ntt_forward<LOGN>(d, buf, A.tw, L);
+                    const size_t row = (size_t)(h * LD + j) * 2 * N;
+                    if constexpr (METHOD == 0) {
+                        uint64_t ka[kE], kb[kE];
+                        load_key_row<LOGN>(ka, key + row, L);
+                        load_key_row<LOGN>(kb, key + row + N, L);
+#pragma unroll
+                        for (int k = 0; k < kE; ++k) {
+                            out[0][k] = gl::add(out[0][k], gl::mul(d[k], ka[k]));
+                            out[1][k] = gl::add(out[1][k], gl::mul(d[k], kb[k]));
+                        }
+                    } else {
+#pragma unroll
+                        for (int p = 0; p < 2; ++p) {
+                            uint64_t kc[kE], w[kE], comb[kE];
+                            load_key_row<LOGN>(kc, key + row + p * N, L);
+                            load_weight<LOGN>(w, A.mono, e0, L);
+#pragma unroll
+                            for (int k = 0; k < kE; ++k) comb[k] = gl::mul(kc[k], w[k]);
+                            load_key_row<LOGN>(kc, key + rgsw_words + row + p * N, L);
+                            load_weight<LOGN>(w, A.mono, e1, L);
+#pragma unroll
+                            for (int k = 0; k < kE; ++k) comb[k] = gl::add(comb[k], gl::mul(kc[k], w[k]));
+                            if constexpr (NC == 3) {
+                                load_key_row<LOGN>(kc, key + 2 * rgsw_words + row + p * N, L);
+                                load_bmmp_w2<LOGN>(w, A.mono, e0, e1, e2, L);
+#pragma unroll
+                                for (int k = 0; k < kE; ++k) comb[k] = gl::add(comb[k], gl::mul(kc[k], w[k]));
+                            }
+#pragma unroll
+                            for (int k = 0; k < kE; ++k) out[p][k] = gl::add(out[p][k], gl::mul(d[k], comb[k]));
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                ntt_inverse<LOGN>(out[h], buf, A.itw, L);
+#pragma unroll
+                for (int k = 0; k < kE; ++k) acc[h][k] = gl::add(acc[h][k], out[h][k]);
+            }
+        }
+
+        // outputs: sample extraction at index 0 (sample_extraction.h:20-36) and optional RLWE
+        if (A.out_lwe) {
+            uint64_t* o = A.out_lwe + (size_t)ct * (N + 1);
+#pragma unroll
+            for (int k = 0; k < kE; ++k) {
+                const int j = idx1<LOGN>(L, k);
+                o[j == 0 ? 0 : N - j] = j == 0 ? acc[0][k] : gl::neg(acc[0][k]);
+                if (j == 0) o[N] = acc[1][k];
+            }
+        }
+        if (A.out_rlwe) {
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                uint64_t* o = A.out_rlwe + (size_t)ct * 2 * N + h * N;
+                if (A.out_rlwe_ntt) {
+                    ntt_forward<LOGN>(acc[h], buf, A.tw, L);
+#pragma unroll
+                    for (int k = 0; k < kE; ++k) o[idx3(L, k)] = acc[h][k];
+                } else {
+#pragma unroll
+                    for (int k = 0; k < kE; ++k) o[idx1<LOGN>(L, k)] = acc[h][k];
+                }
+            }
+        }
+        poly_sync<S::WAVES>();
+    }
+}
+
+// ------------------------------------------------------------------ LWE key switch
+
+// out[c][k] = -(sum_ij d_ij(c) KSK[i][j][k]) (k < n_out), out[c][n_out] = in[c][n_in] - (...)
+// (lwe_conversion.cpp:205-239).  With unsigned digits u_ij = d_ij + delta_j the sum is
+// sum u_ij K - corr[k], corr[k] = sum_ij delta_j K[i][j][k] precomputed per key.  u_ij < 2^b, so
+// u K_lo and u K_hi are accumulated exactly in 64 bits and reduced once at the end.
+// Block: 4 waves; wave w handles 16 ciphertexts, lane = one output column of a 64-wide tile.
+constexpr int kKsCts = 16;
+
+__global__ void __launch_bounds__(256) lwe_keyswitch_kernel(KsArgs A) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int col = blockIdx.x * 64 + lane;
+    const int ct0 = (blockIdx.y * 4 + w) * kKsCts;
+    const int W = A.n_out + 1;
+    const bool col_ok = col < W;
+    uint64_t lo[kKsCts], hi[kKsCts];
+#pragma unroll
+    for (int c = 0; c < kKsCts; ++c) lo[c] = hi[c] = 0;
+    const uint64_t mask = (1ULL << A.base_log2) - 1;
+    const uint64_t* in0 = A.in + (size_t)ct0 * A.in_stride;
+    for (int i = 0; i < A.n_in; ++i) {
+        uint64_t xs[kKsCts];
+#pragma unroll
+        for (int c = 0; c < kKsCts; ++c)
+            xs[c] = (ct0 + c < A.count) ? gl::add(in0[(size_t)c * A.in_stride + i], A.corrector) : 0;
+        const uint64_t* kr = A.ksk + (size_t)i * A.digits * W + (col_ok ? col : 0);
+        for (int j = 0; j < A.digits; ++j) {
+            const uint64_t kv = kr[(size_t)j * W];
+            const uint64_t k0 = (uint32_t)kv, k1 = kv >> 32;
+            const int shift = 64 - (j + 1) * A.base_log2;
+#pragma unroll
+            for (int c = 0; c < kKsCts; ++c) {
+                const uint64_t u = (xs[c] >> shift) & mask;
+                lo[c] += u * k0;
+                hi[c] += u * k1;
+            }
+        }
+    }
+    if (!col_ok) return;
+    const uint64_t corr = A.corr[col];
+#pragma unroll
+    for (int c = 0; c < kKsCts; ++c) {
+        const int ct = ct0 + c;
+        if (ct >= A.count) break;
+        // total = hi * 2^32 + lo, hi < 2^51, lo < 2^51
+        const uint64_t t_lo = (hi[c] << 32) + lo[c];
+        const uint64_t carry = t_lo < lo[c];
+        const uint64_t t_hi = (hi[c] >> 32) + carry;
+        const uint64_t acc = gl::sub(gl::reduce128(t_hi, t_lo), corr);
+        uint64_t* o = A.out + (size_t)ct * W;
+        if (col < A.n_out) o[col] = gl::neg(acc);
+        else o[col] = gl::sub(A.in[(size_t)ct * A.in_stride + A.n_in], acc);
+    }
+}
+
+// ------------------------------------------------------------------ launchers
+
+template <int LOGN>
+static void launch_ntt(uint64_t* out, const uint64_t* in, size_t count, const uint64_t* tw, bool inverse,
+                       hipStream_t s) {
+    constexpr int POLYS = 256 / Shape<LOGN>::P;
+    size_t blocks = (count + POLYS - 1) / POLYS;
+    if (blocks > 65535) blocks = 65535;
+    if (inverse)
+        hipLaunchKernelGGL((ntt_batch_kernel<LOGN, true>), dim3((unsigned)blocks), dim3(256), 0, s, out, in, count, tw);
+    else
+        hipLaunchKernelGGL((ntt_batch_kernel<LOGN, false>), dim3((unsigned)blocks), dim3(256), 0, s, out, in, count, tw);
+}
+
+hipError_t launch_ntt_batch(int logN, uint64_t* out, const uint64_t* in, size_t count, const uint64_t* tw,
+                            bool inverse, hipStream_t s) {
+    switch (logN) {
+        case 10: launch_ntt<10>(out, in, count, tw, inverse, s); break;
+        case 11: launch_ntt<11>(out, in, count, tw, inverse, s); break;
+        case 12: launch_ntt<12>(out, in, count, tw, inverse, s); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <int LOGN, int LD, int METHOD>
+static void launch_br(const BrArgs& A, int grid, hipStream_t s) {
+    hipLaunchKernelGGL((br_kernel<LOGN, LD, METHOD>), dim3(grid), dim3(Shape<LOGN>::P), 0, s, A);
+}
+
+hipError_t launch_blind_rotate(int logN, int method, int digits, const BrArgs& A, int grid, hipStream_t s) {
+#define TP_BR_CASE(LG, LD, M) \
+    if (logN == LG && digits == LD && method == M) { launch_br<LG, LD, M>(A, grid, s); return hipGetLastError(); }
+#define TP_BR_METHODS(LG, LD) TP_BR_CASE(LG, LD, 0) TP_BR_CASE(LG, LD, 1) TP_BR_CASE(LG, LD, 2)
+    TP_BR_METHODS(10, 2) TP_BR_METHODS(10, 3) TP_BR_METHODS(10, 4)
+    TP_BR_METHODS(11, 2) TP_BR_METHODS(11, 3) TP_BR_METHODS(11, 4)
+    TP_BR_METHODS(12, 3) TP_BR_METHODS(12, 4)
+#undef TP_BR_METHODS
+#undef TP_BR_CASE
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_lwe_keyswitch(const KsArgs& A, hipStream_t s) {
+    dim3 grid((A.n_out + 1 + 63) / 64, (A.count + 4 * kKsCts - 1) / (4 * kKsCts));
+    hipLaunchKernelGGL(lwe_keyswitch_kernel, grid, dim3(256), 0, s, A);
+    return hipGetLastError();
+}
+
+}  // namespace tp

@@ -1,0 +1,46 @@
+// Kernel argument blocks and launchers shared by the kernels and the C-ABI layer.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace tp {
+
+struct BrArgs {
+    const uint64_t* brk;   // device key layout (rows permuted lane-major, see key_pos)
+    const uint64_t* lwe;   // [count][n+1], values in Z_2N, b at index n
+    const uint64_t* tv;    // test vector / accumulator, coefficient domain [a|b]
+    size_t tv_stride;      // 0: one shared tv, 2N: one per ciphertext
+    uint64_t* out_lwe;     // [count][N+1] sample-extracted LWE, or null
+    uint64_t* out_rlwe;    // [count][2N] rotated accumulator, or null
+    int out_rlwe_ntt;      // 1: NTT domain (reference order), 0: coefficients
+    const uint64_t* tw;    // psi_rev
+    const uint64_t* itw;   // psi_inv_rev
+    int n;                 // LWE dimension
+    int base_log2;         // gadget base bits
+    int first_shift;       // unsigned digits: b (ceil(64/b) - 1)
+    int count;
+    // signed digits (ternary, BMMP): x' = x + corrector mod Q, digit j = ((x' >> (64 - b(j+1)))
+    // & mask) - 2^(b-1), except the digit j = max_digits - 1 (shift 0, half-range `least`)
+    uint64_t corrector;
+    uint64_t least;
+    int max_digits;
+    const uint64_t* mono;  // NTT(X^e), e < N, key-row order (ternary, BMMP)
+};
+
+struct KsArgs {
+    const uint64_t* ksk;   // [n_in][digits][n_out+1] (reference layout)
+    const uint64_t* corr;  // [n_out+1] sum_ij delta_j KSK[i][j][:]
+    const uint64_t* in;    // [count][in_stride], n_in+1 used
+    uint64_t* out;         // [count][n_out+1]
+    size_t in_stride;
+    uint64_t corrector;    // signed-to-unsigned corrector C mod Q
+    int n_in, n_out, base_log2, digits, count;
+};
+
+hipError_t launch_ntt_batch(int logN, uint64_t* out, const uint64_t* in, size_t count, const uint64_t* tw,
+                            bool inverse, hipStream_t s);
+hipError_t launch_blind_rotate(int logN, int method, int digits, const BrArgs& A, int grid, hipStream_t s);
+hipError_t launch_lwe_keyswitch(const KsArgs& A, hipStream_t s);
+
+}  // namespace tp

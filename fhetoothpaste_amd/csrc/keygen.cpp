@@ -1,0 +1,173 @@
+// Seeded host-side key generation in the reference byte layouts.
+//
+// The reference draws keys and encryption randomness from OpenFHE samplers that cannot be
+// seeded (lib/src/base_crypto.cpp:34-38,139-142).  This generator is seeded and fully
+// specified, so the GPU path and any CPU checker consume identical key bytes:
+//   stream(seed, tag): SplitMix64 with state seed ^ tag*0xD1B54A32D192ED03 ^ 0xA0761D6478BD642F;
+//   uniform mod Q: rejection of 64-bit draws >= Q;
+//   Gaussian(sigma): inversion of a 63-bit cumulative table of exp(-k^2 / 2 sigma^2) over
+//   |k| <= ceil(12 sigma), magnitude from the top 63 bits of a draw, sign from bit 0.
+// Layouts: RGSW rows r < l encrypt -s m g_r, rows l + r encrypt m g_r, produced in the order of
+// RLWEEncryptor::MakeRGSW (base_crypto.cpp:60-97); RLWE = [a | a s + NTT(e) + m] (:27-58);
+// LWE = [a | <a, s> + e + m] (:137-155).
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "../../include/toothpaste_amd.h"
+#include "gl64.hpp"
+#include "host_ntt.hpp"
+
+namespace {
+
+struct Rng {
+    uint64_t s;
+    Rng(uint64_t seed, uint64_t tag) : s(seed ^ (tag * 0xD1B54A32D192ED03ULL) ^ 0xA0761D6478BD642FULL) {}
+    uint64_t next() {
+        uint64_t z = (s += 0x9E3779B97F4A7C15ULL);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+        return z ^ (z >> 31);
+    }
+    uint64_t uniform_q() {
+        uint64_t x;
+        do { x = next(); } while (x >= gl::kQ);
+        return x;
+    }
+};
+
+struct Gauss {
+    uint32_t K = 0;
+    uint64_t T[256];
+    explicit Gauss(double sigma) {
+        if (sigma <= 0) { T[0] = 1ULL << 63; return; }
+        uint32_t k_max = (uint32_t)std::ceil(sigma * 12.0);
+        if (k_max > 255) k_max = 255;
+        double w[256], tot = 0;
+        for (uint32_t k = 0; k <= k_max; ++k) {
+            w[k] = std::exp(-(double)(k * k) / (2.0 * sigma * sigma));
+            tot += (k ? 2.0 * w[k] : w[k]);
+        }
+        double c = 0;
+        for (uint32_t k = 0; k <= k_max; ++k) {
+            c += (k ? 2.0 * w[k] : w[k]);
+            const double v = std::ldexp(c / tot, 63);
+            T[k] = (k == k_max || v >= 9223372036854775808.0) ? (1ULL << 63) : (uint64_t)v;
+        }
+        K = k_max;
+    }
+    uint64_t sample(Rng& g) const {
+        const uint64_t x = g.next();
+        const uint64_t u = x >> 1;
+        uint32_t k = 0;
+        while (k < K && u >= T[k]) ++k;
+        return (x & 1) ? gl::neg(k) : k;
+    }
+};
+
+uint32_t max_digits(uint32_t b) { return 64 / b + (64 % b ? 1 : 0); }
+
+void gadget(uint64_t* g, uint32_t b, uint32_t l, int kind) {
+    const uint32_t D = max_digits(b);
+    for (uint32_t j = 0; j < l; ++j) {
+        if (kind == 0) g[j] = 1ULL << (b * (D - 1 - j));
+        else g[j] = (j == D - 1) ? 1 : (1ULL << (64 - b * (j + 1)));
+    }
+}
+
+void make_rlwe(const tp::HostNtt& T, uint64_t* out, const uint64_t* msg_ntt, const uint64_t* s_ntt, Rng& g,
+               const Gauss& G, uint64_t* e) {
+    const uint32_t N = T.N;
+    for (uint32_t i = 0; i < N; ++i) out[i] = g.uniform_q();
+    for (uint32_t i = 0; i < N; ++i) e[i] = G.sample(g);
+    T.forward(e);
+    for (uint32_t i = 0; i < N; ++i) out[N + i] = gl::add(gl::add(gl::mul(out[i], s_ntt[i]), e[i]), msg_ntt[i]);
+}
+
+void make_rgsw_const(const tp::HostNtt& T, uint64_t* out, uint64_t m, const uint64_t* gad, uint32_t l,
+                     const uint64_t* s_ntt, Rng& g, const Gauss& G, std::vector<uint64_t>& ws) {
+    const uint32_t N = T.N;
+    uint64_t* msg = ws.data();
+    uint64_t* msgs = msg + N;
+    uint64_t* e = msgs + N;
+    for (uint32_t t = 0; t < l; ++t) {
+        const uint32_t r = l - 1 - t;
+        const uint64_t mg = gl::mul(m, gad[r]);
+        for (uint32_t i = 0; i < N; ++i) {
+            msg[i] = mg;
+            msgs[i] = gl::neg(gl::mul(mg, s_ntt[i]));
+        }
+        make_rlwe(T, out + (size_t)r * 2 * N, msgs, s_ntt, g, G, e);
+        make_rlwe(T, out + (size_t)r * 2 * N + (size_t)2 * N * l, msg, s_ntt, g, G, e);
+    }
+}
+
+}  // namespace
+
+extern "C" int tp_keygen_secret(uint64_t* out, uint32_t len, int dist, uint64_t seed, uint64_t tag) {
+    if (!out || (dist != 0 && dist != 1)) return TP_E_INVALID;
+    Rng g(seed, tag);
+    for (uint32_t i = 0; i < len; ++i) {
+        if (dist == 0) out[i] = g.next() >> 63;
+        else {
+            const uint64_t r = g.next() >> 62;
+            out[i] = r < 2 ? 0 : (r == 2 ? 1 : gl::kQ - 1);
+        }
+    }
+    return TP_OK;
+}
+
+extern "C" int tp_keygen_brk(uint64_t* brk, const uint64_t* lwe_key, const uint64_t* rlwe_key, uint32_t N,
+                             uint32_t n, uint32_t b, uint32_t l, int method, int gadget_kind, double sigma,
+                             uint64_t seed) {
+    if (!brk || !lwe_key || !rlwe_key || b == 0 || b > 32 || l == 0 || l > max_digits(b)) return TP_E_INVALID;
+    if (N < 16 || (N & (N - 1))) return TP_E_INVALID;
+    if (method == TP_METHOD_BMMP && (n & 1)) return TP_E_INVALID;
+    tp::HostNtt T(N);
+    Rng g(seed, 3);
+    Gauss G(sigma);
+    std::vector<uint64_t> s_ntt(rlwe_key, rlwe_key + N), ws(3 * N);
+    T.forward(s_ntt.data());
+    uint64_t gad[64];
+    gadget(gad, b, l, gadget_kind);
+    const size_t rg = (size_t)4 * N * l;
+    for (uint32_t i = 0; i < n; ++i) {
+        if (method == TP_METHOD_CGGI_BINARY) {
+            make_rgsw_const(T, brk + i * rg, lwe_key[i], gad, l, s_ntt.data(), g, G, ws);
+        } else if (method == TP_METHOD_CGGI_TERNARY) {
+            const uint64_t s = lwe_key[i];
+            // cggi_blind_rotation.cpp:294-305: 0 -> (0,0), 1 -> (1,0), anything else -> (0,1)
+            make_rgsw_const(T, brk + 2 * i * rg, s == 1, gad, l, s_ntt.data(), g, G, ws);
+            make_rgsw_const(T, brk + (2 * i + 1) * rg, s != 0 && s != 1, gad, l, s_ntt.data(), g, G, ws);
+        } else if (method == TP_METHOD_BMMP) {
+            if (i & 1) continue;
+            uint64_t* chunk = brk + (size_t)3 * (i >> 1) * rg;
+            make_rgsw_const(T, chunk, lwe_key[i], gad, l, s_ntt.data(), g, G, ws);
+            make_rgsw_const(T, chunk + rg, lwe_key[i + 1], gad, l, s_ntt.data(), g, G, ws);
+            make_rgsw_const(T, chunk + 2 * rg, gl::mul(lwe_key[i], lwe_key[i + 1]), gad, l, s_ntt.data(), g, G,
+                            ws);
+        } else {
+            return TP_E_INVALID;
+        }
+    }
+    return TP_OK;
+}
+
+extern "C" int tp_keygen_lwe_ksk(uint64_t* ksk, const uint64_t* src, const uint64_t* dst, uint32_t n_in,
+                                 uint32_t n_out, uint32_t b, uint32_t d, double sigma, uint64_t seed) {
+    if (!ksk || !src || !dst || b == 0 || b > 32 || d == 0) return TP_E_INVALID;
+    Rng g(seed, 4);
+    Gauss G(sigma);
+    for (uint32_t i = 0; i < n_in; ++i) {
+        uint64_t gij = 1ULL << (64 - b);
+        for (uint32_t j = 0; j < d; ++j) {
+            uint64_t* out = ksk + ((size_t)i * d + j) * (n_out + 1);
+            for (uint32_t k = 0; k < n_out; ++k) out[k] = g.uniform_q();
+            uint64_t bb = G.sample(g);
+            for (uint32_t k = 0; k < n_out; ++k) bb = gl::add(bb, gl::mul(out[k], dst[k]));
+            out[n_out] = gl::add(bb, gl::mul(gij, src[i]));
+            gij >>= b;
+        }
+    }
+    return TP_OK;
+}

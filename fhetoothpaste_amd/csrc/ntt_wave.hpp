@@ -1,0 +1,209 @@
+// Negacyclic Goldilocks NTT of one polynomial held in registers by P = N/16 lanes
+// (one wave for N = 1024, two waves for N = 2048), 16 coefficients per lane.
+//
+// Result convention (identical to the reference, lib/src/backend/solinas.cpp:228-313):
+//   forward:  natural order in, out[k] = a(psi^(2 brv(k) + 1)) (bit-reversed order) out,
+//             psi = 7^((Q-1)/2N);
+//   inverse:  bit-reversed in, natural out, scaled by N^-1.
+// The radix-2 Cooley-Tukey levels are grouped into three register passes with two
+// LDS transposes in between:
+//   pass 1  levels m = 1..8     lane L holds idx = L + P k          (twiddles are powers of 2)
+//   pass 2  levels m = 16..128  lane L holds idx = (L>>lb)<<(lb+4) | k<<lb | L&(2^lb-1)
+//   pass 3  the last lb levels  lane L holds idx = 16 L + k
+// with lb = log2(N) - 8.  The inverse runs the same passes backwards (Gentleman-Sande).
+// A level with m blocks uses twiddle psi_rev[m + block] (psi_inv_rev for the inverse).
+#pragma once
+#include <hip/hip_runtime.h>
+#include "gl64.hpp"
+
+namespace tp {
+
+constexpr int kE = 16;  // coefficients per lane
+
+template <int LOGN>
+struct Shape {
+    static constexpr int N = 1 << LOGN;
+    static constexpr int P = N / kE;        // lanes per polynomial
+    static constexpr int LB = LOGN - 8;     // levels in pass 3
+    static constexpr int WAVES = P / 64;    // waves per polynomial
+    static constexpr int LDS_WORDS = N + N / 16;
+};
+
+// LDS address of logical index idx: one pad word every 16 breaks the power-of-two strides.
+__device__ __forceinline__ int phys(int idx) { return idx + (idx >> 4); }
+
+template <int LOGN>
+__device__ __forceinline__ int idx1(int L, int k) { return L + (k << (LOGN - 4)); }
+template <int LOGN>
+__device__ __forceinline__ int idx2(int L, int k) {
+    constexpr int lb = LOGN - 8;
+    return ((L >> lb) << (lb + 4)) | (k << lb) | (L & ((1 << lb) - 1));
+}
+__device__ __forceinline__ int idx3(int L, int k) { return (L << 4) | k; }
+
+// Order LDS traffic between the lanes that share one polynomial.
+template <int WAVES>
+__device__ __forceinline__ void poly_sync() {
+    if constexpr (WAVES == 1) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+        __syncthreads();
+    }
+}
+
+enum class Layout { kPass1, kPass2, kPass3 };
+
+template <int LOGN, Layout LY>
+__device__ __forceinline__ int layout_idx(int L, int k) {
+    if constexpr (LY == Layout::kPass1) return idx1<LOGN>(L, k);
+    else if constexpr (LY == Layout::kPass2) return idx2<LOGN>(L, k);
+    else return idx3(L, k);
+}
+
+// Move the polynomial from layout A to layout B through LDS.  A sync precedes the
+// writes so that an earlier reader of the same buffer is done with it.
+template <int LOGN, Layout A, Layout B>
+__device__ __forceinline__ void transpose(uint64_t (&x)[kE], uint64_t* lds, int L) {
+    using S = Shape<LOGN>;
+    poly_sync<S::WAVES>();
+#pragma unroll
+    for (int k = 0; k < kE; ++k) lds[phys(layout_idx<LOGN, A>(L, k))] = x[k];
+    poly_sync<S::WAVES>();
+#pragma unroll
+    for (int k = 0; k < kE; ++k) x[k] = lds[phys(layout_idx<LOGN, B>(L, k))];
+}
+
+__device__ __forceinline__ void ct_bfly(uint64_t& u, uint64_t& v, uint64_t wv) {
+    const uint64_t a = u;
+    u = gl::add(a, wv);
+    v = gl::sub(a, wv);
+}
+
+__device__ __forceinline__ void gs_bfly(uint64_t& u, uint64_t& v, uint64_t w) {
+    const uint64_t a = u, b = v;
+    u = gl::add(a, b);
+    v = gl::mul(gl::sub(a, b), w);
+}
+
+// ---------------------------------------------------------------- forward
+
+template <int LOGN>
+__device__ __forceinline__ void fwd_pass1(uint64_t (&x)[kE]) {
+#pragma unroll
+    for (int lv = 0; lv < 4; ++lv) {
+        const int half = 8 >> lv;
+#pragma unroll
+        for (int k = 0; k < kE; ++k) {
+            if (k & half) continue;
+            const int blk = k >> (4 - lv);
+            ct_bfly(x[k], x[k + half], gl::mul_pow2(x[k + half], gl::kFwdPow2[(1 << lv) + blk]));
+        }
+    }
+}
+
+template <int LOGN>
+__device__ __forceinline__ void fwd_pass2(uint64_t (&x)[kE], const uint64_t* __restrict__ tw, int L) {
+    constexpr int lb = LOGN - 8;
+#pragma unroll
+    for (int lv = 4; lv < 8; ++lv) {
+        const int half = 8 >> (lv - 4);
+#pragma unroll
+        for (int k = 0; k < kE; ++k) {
+            if (k & half) continue;
+            const int blk = ((L >> lb) << (lv - 4)) | (k >> (8 - lv));
+            const uint64_t w = tw[(1 << lv) + blk];
+            ct_bfly(x[k], x[k + half], gl::mul(x[k + half], w));
+        }
+    }
+}
+
+template <int LOGN>
+__device__ __forceinline__ void fwd_pass3(uint64_t (&x)[kE], const uint64_t* __restrict__ tw, int L) {
+#pragma unroll
+    for (int lv = 8; lv < LOGN; ++lv) {
+        const int half = 1 << (LOGN - 1 - lv);
+#pragma unroll
+        for (int k = 0; k < kE; ++k) {
+            if (k & half) continue;
+            const int blk = ((L << 4) | k) >> (LOGN - lv);
+            const uint64_t w = tw[(1 << lv) + blk];
+            ct_bfly(x[k], x[k + half], gl::mul(x[k + half], w));
+        }
+    }
+}
+
+// Natural-order coefficients in pass-1 layout -> NTT values in pass-3 layout.
+template <int LOGN>
+__device__ __forceinline__ void ntt_forward(uint64_t (&x)[kE], uint64_t* lds, const uint64_t* __restrict__ tw,
+                                            int L) {
+    fwd_pass1<LOGN>(x);
+    transpose<LOGN, Layout::kPass1, Layout::kPass2>(x, lds, L);
+    fwd_pass2<LOGN>(x, tw, L);
+    transpose<LOGN, Layout::kPass2, Layout::kPass3>(x, lds, L);
+    fwd_pass3<LOGN>(x, tw, L);
+}
+
+// ---------------------------------------------------------------- inverse
+
+template <int LOGN>
+__device__ __forceinline__ void inv_pass3(uint64_t (&x)[kE], const uint64_t* __restrict__ itw, int L) {
+#pragma unroll
+    for (int lv = LOGN - 1; lv >= 8; --lv) {
+        const int half = 1 << (LOGN - 1 - lv);
+#pragma unroll
+        for (int k = 0; k < kE; ++k) {
+            if (k & half) continue;
+            const int blk = ((L << 4) | k) >> (LOGN - lv);
+            gs_bfly(x[k], x[k + half], itw[(1 << lv) + blk]);
+        }
+    }
+}
+
+template <int LOGN>
+__device__ __forceinline__ void inv_pass2(uint64_t (&x)[kE], const uint64_t* __restrict__ itw, int L) {
+    constexpr int lb = LOGN - 8;
+#pragma unroll
+    for (int lv = 7; lv >= 4; --lv) {
+        const int half = 8 >> (lv - 4);
+#pragma unroll
+        for (int k = 0; k < kE; ++k) {
+            if (k & half) continue;
+            const int blk = ((L >> lb) << (lv - 4)) | (k >> (8 - lv));
+            gs_bfly(x[k], x[k + half], itw[(1 << lv) + blk]);
+        }
+    }
+}
+
+// Levels m = 8..1 with power-of-two inverse twiddles, then the N^-1 scaling.
+template <int LOGN>
+__device__ __forceinline__ void inv_pass1(uint64_t (&x)[kE]) {
+#pragma unroll
+    for (int lv = 3; lv >= 0; --lv) {
+        const int half = 8 >> lv;
+#pragma unroll
+        for (int k = 0; k < kE; ++k) {
+            if (k & half) continue;
+            const int blk = k >> (4 - lv);
+            const uint64_t a = x[k], b = x[k + half];
+            x[k] = gl::add(a, b);
+            x[k + half] = gl::mul_pow2(gl::sub(a, b), gl::kInvPow2[(1 << lv) + blk]);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < kE; ++k) x[k] = gl::div_pow2(x[k], LOGN);
+}
+
+// NTT values in pass-3 layout -> natural-order coefficients in pass-1 layout.
+template <int LOGN>
+__device__ __forceinline__ void ntt_inverse(uint64_t (&x)[kE], uint64_t* lds, const uint64_t* __restrict__ itw,
+                                            int L) {
+    inv_pass3<LOGN>(x, itw, L);
+    transpose<LOGN, Layout::kPass3, Layout::kPass2>(x, lds, L);
+    inv_pass2<LOGN>(x, itw, L);
+    transpose<LOGN, Layout::kPass2, Layout::kPass1>(x, lds, L);
+    inv_pass1<LOGN>(x);
+}
+
+}  // namespace tp

@@ -1,0 +1,258 @@
+"""Python mirror of the reference operator API for the bootstrapping path.
+
+Names, argument meanings and error behaviour follow LSchild/FHEToothpaste:
+  CGGIBlindRotationContext / CGGIBlindRotator   lib/include/operators/cggi_blind_rotation.h:19-161
+  BMMPBlindRotationContext / BMMPBlindRotator   lib/include/operators/bmmp_blind_rotation.h:18-176
+  LWEConversionContext / LWEtoLWEConverter      lib/include/operators/endo_glwe_conversion.h:15-239
+  NegacyclicFunctionEvaluator                   lib/include/operators/negacyclic_function_evaluator.h:17-114
+  SolinasWorker (ForwardNTT / BackwardNTT)      lib/include/backend/solinas_worker.h:56-126
+Every call runs on the GPU through libtoothpaste_amd.so; there is no CPU path.
+Reference asserts and std::invalid_argument become ValueError / ToothpasteError.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib as T
+from ._lib import Q, check, lib, ptr
+
+BINARY, TERNARY = 0, 1
+
+
+def _max_digits(b):
+    return 64 // b + (1 if 64 % b else 0)
+
+
+class DeviceBuffer:
+    """Device array of uint64 owned by a context."""
+
+    def __init__(self, ctx, words):
+        self.ctx = ctx
+        self.words = int(words)
+        p = ctypes.c_void_p()
+        check(lib().tp_malloc(ctx.h, ctypes.byref(p), self.words))
+        self.p = p.value
+
+    @classmethod
+    def from_host(cls, ctx, a):
+        a = np.ascontiguousarray(a, dtype=np.uint64).reshape(-1)
+        b = cls(ctx, a.size)
+        check(lib().tp_copy_to_device(ctx.h, b.p, ptr(a), a.size))
+        return b
+
+    def to_host(self, words=None):
+        words = self.words if words is None else words
+        out = np.empty(words, dtype=np.uint64)
+        check(lib().tp_copy_to_host(self.ctx.h, ptr(out), self.p, words))
+        return out
+
+    def free(self):
+        if self.p is not None and self.ctx.h:
+            check(lib().tp_free(self.ctx.h, self.p))
+        self.p = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Context:
+    """One device context (stream, tables, resident keys) for one parameter set."""
+
+    def __init__(self, ring_dim, lwe_dim, base_log2, digits, method=T.METHOD_CGGI_BINARY, ks_out_dim=0,
+                 ks_base_log2=0, ks_digits=0, device=0):
+        self.params = T.Params(ring_dim, lwe_dim, base_log2, digits, method, ks_out_dim, ks_base_log2, ks_digits)
+        h = ctypes.c_void_p()
+        check(lib().tp_context_create(ctypes.byref(h), ctypes.byref(self.params), device))
+        self.h = h.value
+        self.N, self.n = ring_dim, lwe_dim
+
+    def close(self):
+        if self.h:
+            lib().tp_context_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sync(self):
+        check(lib().tp_context_sync(self.h))
+
+    def import_brk(self, brk):
+        brk = np.ascontiguousarray(brk, dtype=np.uint64)
+        check(lib().tp_import_brk(self.h, ptr(brk), brk.size))
+
+    def import_lwe_ksk(self, ksk):
+        ksk = np.ascontiguousarray(ksk, dtype=np.uint64)
+        check(lib().tp_import_lwe_ksk(self.h, ptr(ksk), ksk.size))
+
+    def ntt_forward(self, polys):
+        polys = np.ascontiguousarray(polys, dtype=np.uint64)
+        count = polys.size // self.N
+        d = DeviceBuffer.from_host(self, polys)
+        check(lib().tp_ntt_forward_device(self.h, d.p, d.p, count))
+        return d.to_host().reshape(polys.shape)
+
+    def ntt_inverse(self, polys):
+        polys = np.ascontiguousarray(polys, dtype=np.uint64)
+        count = polys.size // self.N
+        d = DeviceBuffer.from_host(self, polys)
+        check(lib().tp_ntt_inverse_device(self.h, d.p, d.p, count))
+        return d.to_host().reshape(polys.shape)
+
+    def blind_rotate(self, lwe, acc, as_coef=False):
+        lwe = np.ascontiguousarray(lwe, dtype=np.uint64)
+        acc = np.ascontiguousarray(acc, dtype=np.uint64)
+        out = np.empty(2 * self.N, dtype=np.uint64)
+        check(lib().tp_blind_rotate(self.h, ptr(out), ptr(lwe), ptr(acc), int(as_coef)))
+        return out
+
+    def eval_func_batch(self, lwes, tv, want_rlwe=False, rlwe_ntt=False):
+        """lwes [count, n+1] in Z_2N; tv [2N] shared or [count, 2N]. Returns extracted LWEs (and RLWEs)."""
+        lwes = np.ascontiguousarray(lwes, dtype=np.uint64).reshape(-1, self.n + 1)
+        tv = np.ascontiguousarray(tv, dtype=np.uint64)
+        count = lwes.shape[0]
+        d_in = DeviceBuffer.from_host(self, lwes)
+        d_tv = DeviceBuffer.from_host(self, tv)
+        d_out = DeviceBuffer(self, count * (self.N + 1))
+        d_rl = DeviceBuffer(self, count * 2 * self.N) if want_rlwe else None
+        stride = 0 if tv.size == 2 * self.N else 2 * self.N
+        check(lib().tp_eval_func_batch_device(self.h, count, d_in.p, d_tv.p, stride, d_out.p,
+                                              d_rl.p if d_rl else None, int(rlwe_ntt)))
+        out = d_out.to_host().reshape(count, self.N + 1)
+        if want_rlwe:
+            return out, d_rl.to_host().reshape(count, 2 * self.N)
+        return out
+
+    def lwe_keyswitch_batch(self, lwes):
+        lwes = np.ascontiguousarray(lwes, dtype=np.uint64).reshape(-1, self.N + 1)
+        count = lwes.shape[0]
+        W = self.params.ks_out_dim + 1
+        d_in = DeviceBuffer.from_host(self, lwes)
+        d_out = DeviceBuffer(self, count * W)
+        check(lib().tp_lwe_keyswitch_batch_device(self.h, count, d_in.p, self.N + 1, d_out.p))
+        return d_out.to_host().reshape(count, W)
+
+    def bootstrap_batch(self, lwes, tv):
+        lwes = np.ascontiguousarray(lwes, dtype=np.uint64).reshape(-1, self.n + 1)
+        tv = np.ascontiguousarray(tv, dtype=np.uint64)
+        count = lwes.shape[0]
+        W = self.params.ks_out_dim + 1
+        d_in = DeviceBuffer.from_host(self, lwes)
+        d_tv = DeviceBuffer.from_host(self, tv)
+        d_out = DeviceBuffer(self, count * W)
+        stride = 0 if tv.size == 2 * self.N else 2 * self.N
+        check(lib().tp_bootstrap_batch_device(self.h, count, d_in.p, d_tv.p, stride, d_out.p))
+        return d_out.to_host().reshape(count, W)
+
+
+# ---------------------------------------------------------------- key generation (host, seeded)
+
+def keygen_secret(length, dist, seed, tag):
+    out = np.empty(length, dtype=np.uint64)
+    check(lib().tp_keygen_secret(ptr(out), length, dist, seed, tag))
+    return out
+
+
+def keygen_brk(lwe_key, rlwe_key, N, n, base_log2, digits, method, gadget_kind, sigma, seed):
+    per = {0: 4 * N * digits * n, 1: 8 * N * digits * n, 2: 12 * N * digits * (n // 2)}[method]
+    out = np.empty(per, dtype=np.uint64)
+    check(lib().tp_keygen_brk(ptr(out), ptr(np.ascontiguousarray(lwe_key, np.uint64)),
+                              ptr(np.ascontiguousarray(rlwe_key, np.uint64)), N, n, base_log2, digits, method,
+                              gadget_kind, sigma, seed))
+    return out
+
+
+def keygen_lwe_ksk(src, dst, n_in, n_out, base_log2, digits, sigma, seed):
+    out = np.empty(n_in * digits * (n_out + 1), dtype=np.uint64)
+    check(lib().tp_keygen_lwe_ksk(ptr(out), ptr(np.ascontiguousarray(src, np.uint64)),
+                                  ptr(np.ascontiguousarray(dst, np.uint64)), n_in, n_out, base_log2, digits,
+                                  sigma, seed))
+    return out
+
+
+# ---------------------------------------------------------------- reference-named operators
+
+class GenericKey:
+    """GenericKey{label, data} (lib/include/static/keys.h:14-39)."""
+
+    def __init__(self, label, data):
+        self.label = label
+        self.data = np.ascontiguousarray(data, dtype=np.uint64)
+
+
+class CGGIBlindRotationContext:
+    """CGGIBlindRotationContext(distribution, Q, N, n, basis, digits, std)
+    (cggi_blind_rotation.cpp:14-25).  basis is L = 2^b as in the reference."""
+
+    def __init__(self, distribution, modulus, ring_dim, lwe_dim, basis, digits, std, seed=0):
+        if modulus != Q:
+            raise ValueError("only the Goldilocks modulus runs on the GPU path")
+        b = int(basis).bit_length() - 1
+        if basis != 1 << b:
+            raise ValueError("basis must be a power of two")
+        self.distribution, self.N, self.n, self.b, self.l, self.std = distribution, ring_dim, lwe_dim, b, digits, std
+        self.seed = seed
+
+    def GetRingDimension(self):
+        return self.N
+
+    def GetLWEDimension(self):
+        return self.n
+
+    def GetModulus(self):
+        return Q
+
+    def ConstructOperator(self, bundle):
+        """bundle = [lwe secret (n), RLWE secret (N, coefficients)] (cggi_blind_rotation.cpp:184-190)."""
+        return CGGIBlindRotator(self, bundle[0].data, bundle[1].data)
+
+
+class CGGIBlindRotator:
+    def __init__(self, ctx, lwe_key, rlwe_key):
+        self.ctx = ctx
+        method = T.METHOD_CGGI_BINARY if ctx.distribution == BINARY else T.METHOD_CGGI_TERNARY
+        self.dev = Context(ctx.N, ctx.n, ctx.b, ctx.l, method)
+        # binary keys follow the unsigned decomposition (MakeRGSW weights); ternary the signed one
+        kind = 0 if method == T.METHOD_CGGI_BINARY else 1
+        self.brk = keygen_brk(lwe_key, rlwe_key, ctx.N, ctx.n, ctx.b, ctx.l, method, kind, ctx.std, ctx.seed)
+        self.dev.import_brk(self.brk)
+
+    def BlindRotate(self, lwe_vec, accumulator, output_as_coefficients=False):
+        """BlindRotator::BlindRotate (blindrotation_operator.h:29-40); returns the result."""
+        return self.dev.blind_rotate(lwe_vec, accumulator, output_as_coefficients)
+
+
+class LWEtoLWEConverter:
+    """LWEtoLWEConverter (lwe_conversion.cpp:148-239).  Convert follows the reference's
+    accumulate-into-output semantics: output[:n_out] -= acc, output[n_out] = input[N] - acc."""
+
+    def __init__(self, dev_ctx, ksk):
+        self.dev = dev_ctx
+        self.dev.import_lwe_ksk(ksk)
+
+    def Convert(self, output, lwe_in):
+        res = self.dev.lwe_keyswitch_batch(np.asarray(lwe_in, dtype=np.uint64)[None, :])[0]
+        n_out = self.dev.params.ks_out_dim
+        out = np.asarray(output, dtype=np.uint64)
+        acc_neg = res[:n_out]
+        for k in range(n_out):
+            out[k] = (int(out[k]) + int(acc_neg[k])) % Q
+        out[n_out] = res[n_out]
+        return out
+
+
+def SampleExtract(rlwe_coef, N):
+    """SampleExtract(out, in, 0, N, Q) (static/sample_extraction.h:20-36), host helper."""
+    rlwe_coef = np.asarray(rlwe_coef, dtype=np.uint64)
+    out = np.zeros(N + 1, dtype=np.uint64)
+    out[0] = rlwe_coef[0]
+    a = rlwe_coef[1:N][::-1].astype(object)
+    out[1:N] = np.array([(Q - int(v)) % Q for v in a], dtype=np.uint64)
+    out[N] = rlwe_coef[N]
+    return out

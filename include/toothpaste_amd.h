@@ -1,0 +1,145 @@
+/*
+ * toothpaste_amd: C-ABI of the MI355X (gfx950) CGGI/BMMP bootstrapping path.
+ *
+ * This is the drop-in boundary for the hot path of LSchild/FHEToothpaste.  Every
+ * entry point names the reference interface it replaces (file:line in that
+ * repository).  Conventions:
+ *   - every function returns 0 (TP_OK) or a negative TP_E* status; the message of
+ *     the last failure on the calling thread is returned by tp_last_error();
+ *   - ring elements are uint64_t values in [0, Q), Q = 2^64 - 2^32 + 1 (Goldilocks);
+ *   - NTT-domain data uses the reference order: out[k] = a(psi^(2 brv(k) + 1)),
+ *     psi = 7^((Q-1)/2N) (lib/src/backend/solinas.cpp:228-261);
+ *   - key bytes use the reference layouts (lib/src/base_crypto.cpp:60-97,
+ *     lib/src/operators/cggi_blind_rotation.cpp:264-311,
+ *     lib/src/operators/bmmp_blind_rotation.cpp:229-262,
+ *     lib/src/operators/lwe_conversion.cpp:174-203) and are converted on import;
+ *   - "_device" entry points take device pointers and run asynchronously on the
+ *     context's stream (finish with tp_context_sync); the others take host
+ *     pointers and return when the result is in host memory;
+ *   - calls on one context are serialised by the caller; contexts are independent.
+ */
+#ifndef TOOTHPASTE_AMD_H
+#define TOOTHPASTE_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TP_OK 0
+#define TP_E_INVALID (-1)   /* bad argument (reference: assert / std::invalid_argument) */
+#define TP_E_DEVICE (-2)    /* HIP runtime error or no gfx950 device */
+#define TP_E_STATE (-3)     /* keys not imported yet (reference: assert(m_keys_generated)) */
+#define TP_E_NOMEM (-4)     /* device allocation failed */
+
+#define TP_Q 0xFFFFFFFF00000001ULL
+
+/* Blind-rotation method (lib/include/static/enum_ids.h KeyDistribution + operator kind). */
+#define TP_METHOD_CGGI_BINARY 0   /* CGGIBlindRotator, BINARY  (cggi_blind_rotation.cpp:313-386) */
+#define TP_METHOD_CGGI_TERNARY 1  /* CGGIBlindRotator, TERNARY (cggi_blind_rotation.cpp:388-472) */
+#define TP_METHOD_BMMP 2          /* BMMPBlindRotator, unroll 2 (bmmp_blind_rotation.cpp:354-419) */
+
+/* Parameters of one blind-rotation context; mirrors the arguments of
+ * CGGIBlindRotationContext(distribution, Q, N, n, basis, digits, std)
+ * (lib/src/operators/cggi_blind_rotation.cpp:14-25) and of
+ * LWEConversionContext (lib/include/operators/endo_glwe_conversion.h:15-60). */
+typedef struct tp_params {
+    uint32_t ring_dim;      /* N: 1024, 2048 or 4096 */
+    uint32_t lwe_dim;       /* n */
+    uint32_t br_base_log2;  /* b: gadget base 2^b */
+    uint32_t br_digits;     /* l */
+    uint32_t method;        /* TP_METHOD_* */
+    uint32_t ks_out_dim;    /* LWE key switch N -> ks_out_dim; 0 = no key switch */
+    uint32_t ks_base_log2;
+    uint32_t ks_digits;
+} tp_params;
+
+typedef struct tp_context tp_context;
+
+/* Last error message of the calling thread ("" if none). */
+const char* tp_last_error(void);
+
+/* Number of usable gfx950 devices. */
+int tp_device_count(int* count);
+
+/* Create / destroy a context on a device.  Replaces OperatorContext<Op>::ConstructOperator
+ * (lib/include/interfaces/operator_context.h:22-63) minus the key generation, which the
+ * reference does inside ConstructOperator and which is tp_import_* here. */
+int tp_context_create(tp_context** out, const tp_params* params, int device);
+int tp_context_destroy(tp_context* ctx);
+int tp_context_sync(tp_context* ctx);
+
+/* Import the blind-rotation key in the reference byte layout (host memory).  Replaces the
+ * key material built by CGGIBlindRotator::KeyGen (cggi_blind_rotation.cpp:215-311) and
+ * BMMPBlindRotator::KeyGen (bmmp_blind_rotation.cpp:222-262).  words must equal
+ * n*4Nl (binary), n*8Nl (ternary) or (n/2)*12Nl (BMMP). */
+int tp_import_brk(tp_context* ctx, const uint64_t* brk, size_t words);
+
+/* Import the LWE->LWE key-switching key [N][d][n_out+1] (lwe_conversion.cpp:174-203). */
+int tp_import_lwe_ksk(tp_context* ctx, const uint64_t* ksk, size_t words);
+
+/* Negacyclic NTT over Z_Q[X]/(X^N+1) of `count` polynomials (device pointers).
+ * Replace SolinasWorker::ForwardNTT / BackwardNTT (lib/src/backend/solinas.cpp:228-313);
+ * dst may equal src. */
+int tp_ntt_forward_device(tp_context* ctx, uint64_t* dst, const uint64_t* src, size_t count);
+int tp_ntt_inverse_device(tp_context* ctx, uint64_t* dst, const uint64_t* src, size_t count);
+
+/* One blind rotation, host buffers: BlindRotator::BlindRotate(result, lwe_vec, accumulator,
+ * output_as_coefficients) (lib/include/interfaces/blindrotation_operator.h:29-40).
+ * lwe: n+1 values in Z_2N; acc: 2N coefficients [a|b]; result: 2N (NTT domain unless
+ * as_coef); result may alias acc. */
+int tp_blind_rotate(tp_context* ctx, uint64_t* result, const uint64_t* lwe, const uint64_t* acc, int as_coef);
+
+/* Batched function evaluation: for each of `count` LWE inputs, blind rotation of the test
+ * vector followed by SampleExtract at index 0.  Replaces a loop over
+ * NegacyclicFunctionEvaluator::EvalFunc(result, input, lut_rlwe)
+ * (negacyclic_function_evaluator.cpp:92-101).  Device pointers:
+ *   lwes     [count][n+1] in Z_2N
+ *   tv       [2N] coefficients, shared (tv_stride 0) or [count][2N] (tv_stride 2N)
+ *   out_lwe  [count][N+1] extracted LWE (may be NULL)
+ *   out_rlwe [count][2N] rotated accumulator (may be NULL), NTT domain if rlwe_ntt. */
+int tp_eval_func_batch_device(tp_context* ctx, size_t count, const uint64_t* lwes, const uint64_t* tv,
+                              size_t tv_stride, uint64_t* out_lwe, uint64_t* out_rlwe, int rlwe_ntt);
+
+/* Batched LWE->LWE key switch N -> n_out (lwe_conversion.cpp:205-239), overwrite semantics:
+ * out = (0, in_b) - sum d_ij KSK[i][j].  in [count][in_stride] (first N+1 used), out
+ * [count][n_out+1], device pointers. */
+int tp_lwe_keyswitch_batch_device(tp_context* ctx, size_t count, const uint64_t* in, size_t in_stride,
+                                  uint64_t* out);
+
+/* Full bootstrap = tp_eval_func_batch_device + tp_lwe_keyswitch_batch_device, using a device
+ * scratch buffer owned by the context.  out [count][n_out+1]. */
+int tp_bootstrap_batch_device(tp_context* ctx, size_t count, const uint64_t* lwes, const uint64_t* tv,
+                              size_t tv_stride, uint64_t* out);
+
+/* Device memory helpers (so that callers need no HIP headers). */
+int tp_malloc(tp_context* ctx, uint64_t** ptr, size_t words);
+int tp_free(tp_context* ctx, uint64_t* ptr);
+int tp_copy_to_device(tp_context* ctx, uint64_t* dst, const uint64_t* src, size_t words);
+int tp_copy_to_host(tp_context* ctx, uint64_t* dst, const uint64_t* src, size_t words);
+
+/* Elapsed milliseconds between two points of the context stream, for benchmarks:
+ * tp_timer_mark(ctx, slot) records an event; tp_timer_elapsed(ctx, a, b, &ms). */
+int tp_timer_mark(tp_context* ctx, int slot);
+int tp_timer_elapsed(tp_context* ctx, int slot_a, int slot_b, float* ms);
+
+/* ------------------------------------------------------------------ seeded key generation
+ * Host-side key generation with a seeded sampler (the reference samples with OpenFHE, which
+ * has no seed API: lib/src/base_crypto.cpp:34-38,139-142).  Output bytes use the reference
+ * layouts, so they can be passed to tp_import_* and to any reference-layout consumer.
+ * dist: 0 binary, 1 ternary (P(0) = 1/2, P(+-1) = 1/4, -1 stored as Q-1).
+ * gadget_kind: 0 = MakeRGSW weights L^(D-1-j) (base_crypto.cpp:68,85-95),
+ *              1 = signed-decomposer weights 2^(64-b(j+1)) (gadget_decomp.h:176-193). */
+int tp_keygen_secret(uint64_t* out, uint32_t len, int dist, uint64_t seed, uint64_t tag);
+int tp_keygen_brk(uint64_t* brk, const uint64_t* lwe_key, const uint64_t* rlwe_key, uint32_t N, uint32_t n,
+                  uint32_t base_log2, uint32_t digits, int method, int gadget_kind, double sigma, uint64_t seed);
+int tp_keygen_lwe_ksk(uint64_t* ksk, const uint64_t* src, const uint64_t* dst, uint32_t n_in, uint32_t n_out,
+                      uint32_t base_log2, uint32_t digits, double sigma, uint64_t seed);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TOOTHPASTE_AMD_H */

@@ -1,0 +1,157 @@
+"""GPU parity: the HIP path (through the C-ABI) against the CPU oracle, bit for bit.
+
+Every case prints what it runs before touching the GPU, so a failing run names its case.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+Q = 0xFFFFFFFF00000001
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def tp():
+    import fhetoothpaste_amd as tp
+    assert tp.device_count() >= 1, "no gfx950 device"
+    return tp
+
+
+def _keys(oracle, N, n, b, l, method, dist, sigma, seed):
+    lwe_key = oracle.secret(n, dist, seed, 1)
+    rlwe_key = oracle.secret(N, 0, seed, 2)
+    kind = 0 if method == oracle.BINARY else 1
+    brk = oracle.brk(N, lwe_key, rlwe_key, n, b, l, method, kind, sigma, seed)
+    return lwe_key, rlwe_key, brk
+
+
+@pytest.mark.parametrize("N", [1024, 2048, 4096])
+def test_ntt_forward_inverse(tp, oracle, N):
+    say("ntt", N)
+    ctx = tp.Context(N, 2, 16, 4)
+    rng = np.random.default_rng(N)
+    a = rng.integers(0, Q, size=(37, N), dtype=np.uint64)
+    a[0] = 0
+    a[1] = Q - 1
+    R = oracle.Ring(N)
+    f = ctx.ntt_forward(a)
+    assert np.array_equal(f, R.ntt(a.reshape(-1)).reshape(a.shape))
+    inv = ctx.ntt_inverse(f)
+    assert np.array_equal(inv, a)
+    # inverse of arbitrary values against the oracle
+    assert np.array_equal(ctx.ntt_inverse(a), R.intt(a.reshape(-1)).reshape(a.shape))
+
+
+def test_keygen_matches_oracle(tp, oracle):
+    say("keygen bytes")
+    N, n = 1024, 8
+    for dist in (0, 1):
+        assert np.array_equal(tp.keygen_secret(n, dist, 77, 1), oracle.secret(n, dist, 77, 1))
+    lk = oracle.secret(n, 0, 5, 1)
+    rk = oracle.secret(N, 0, 5, 2)
+    for method, kind in [(0, 0), (1, 1), (2, 1)]:
+        a = tp.keygen_brk(lk, rk, N, n, 12, 3, method, kind, 3.2, 9)
+        b = oracle.brk(N, lk, rk, n, 12, 3, method, kind, 3.2, 9)
+        assert np.array_equal(a, b), method
+    src = oracle.secret(64, 0, 4, 2)
+    dst = oracle.secret(20, 1, 4, 1)
+    assert np.array_equal(tp.keygen_lwe_ksk(src, dst, 64, 20, 4, 14, 3.2, 6),
+                          oracle.lwe_ksk(src, dst, 64, 20, 4, 14, 3.2, 6))
+
+
+@pytest.mark.parametrize("N,n,b,l", [(1024, 12, 16, 4), (1024, 20, 12, 3), (2048, 10, 16, 4)])
+@pytest.mark.parametrize("as_coef", [False, True])
+def test_blind_rotate_single(tp, oracle, N, n, b, l, as_coef):
+    say("blind_rotate binary", N, n, b, l, as_coef)
+    lk, rk, brk = _keys(oracle, N, n, b, l, oracle.BINARY, 0, 3.2, 31 + n)
+    ctx = tp.Context(N, n, b, l, 0)
+    ctx.import_brk(brk)
+    rng = np.random.default_rng(n)
+    for trial in range(3):
+        lwe = rng.integers(0, 2 * N, size=n + 1, dtype=np.uint64)
+        if trial == 1:
+            lwe[::3] = 0  # zero coefficients are skipped by the reference loop
+        acc = rng.integers(0, Q, size=2 * N, dtype=np.uint64)
+        got = ctx.blind_rotate(lwe, acc, as_coef)
+        ref = oracle.blind_rotate(N, lwe, acc, brk, n, b, l, oracle.BINARY, as_coef)
+        assert np.array_equal(got, ref), trial
+
+
+def _sign_tv(N):
+    tv = np.zeros(2 * N, dtype=np.uint64)
+    tv[N:] = Q // 8
+    return tv
+
+
+@pytest.mark.parametrize("N,n,b,l,count", [(1024, 500, 16, 4, 3), (1024, 630, 12, 3, 5)])
+def test_eval_func_batch_vs_oracle(tp, oracle, N, n, b, l, count):
+    say("eval_func batch", N, n, b, l, count)
+    lk, rk, brk = _keys(oracle, N, n, b, l, oracle.BINARY, 0, 3.2, 7)
+    ctx = tp.Context(N, n, b, l, 0)
+    ctx.import_brk(brk)
+    lwes = oracle.random_mod_pow2(count * (n + 1), 2 * N, 7, 50).reshape(count, n + 1)
+    tv = _sign_tv(N)
+    got, rl = ctx.eval_func_batch(lwes, tv, want_rlwe=True, rlwe_ntt=False)
+    for c in range(count):
+        ref_acc = oracle.blind_rotate(N, lwes[c], tv, brk, n, b, l, oracle.BINARY, True)
+        assert np.array_equal(rl[c], ref_acc), c
+        assert np.array_equal(got[c], oracle.sample_extract(ref_acc, N)), c
+    # decryption of the evaluated function: sign of phase
+    for c in range(count):
+        ph = oracle.phase_lwe(got[c], rk, N)
+        ks = [int(x) for x in lk]
+        p = (int(lwes[c][n]) - sum(int(a) * s for a, s in zip(lwes[c][:n], ks))) % (2 * N)
+        want = Q // 8 if (p == 0 or p > N) else Q - Q // 8
+        err = (ph - want) % Q
+        err = err if err < Q // 2 else err - Q
+        assert abs(err) < Q // 16, (c, p)
+
+
+def test_keyswitch_batch_vs_oracle(tp, oracle):
+    N, n_out, b, d = 1024, 630, 4, 14
+    say("lwe keyswitch", N, n_out, b, d)
+    src = oracle.secret(N, 0, 3, 2)
+    dst = oracle.secret(n_out, 0, 3, 1)
+    ksk = oracle.lwe_ksk(src, dst, N, n_out, b, d, 3.2, 3)
+    ctx = tp.Context(N, 4, 12, 3, 0, n_out, b, d)
+    ctx.import_lwe_ksk(ksk)
+    rng = np.random.default_rng(5)
+    count = 37
+    lwes = rng.integers(0, Q, size=(count, N + 1), dtype=np.uint64)
+    lwes[0] = Q - 1
+    lwes[1] = 0
+    got = ctx.lwe_keyswitch_batch(lwes)
+    for c in range(count):
+        assert np.array_equal(got[c], oracle.lwe_keyswitch(lwes[c], ksk, N, n_out, b, d)), c
+
+
+def test_bootstrap_batch_cfg2_shape(tp, oracle):
+    """n=630, N=1024, b=12 l=3, KS 1024->630 b=4 d=14: bit-exact and decrypts."""
+    N, n, b, l, n_out, kb, kd = 1024, 630, 12, 3, 630, 4, 14
+    say("bootstrap batch cfg2 shape")
+    lk, rk, brk = _keys(oracle, N, n, b, l, oracle.BINARY, 0, 3.2, 11)
+    ksk = oracle.lwe_ksk(rk, lk, N, n_out, kb, kd, 3.2, 11)
+    ctx = tp.Context(N, n, b, l, 0, n_out, kb, kd)
+    ctx.import_brk(brk)
+    ctx.import_lwe_ksk(ksk)
+    count = 6
+    lwes = oracle.random_mod_pow2(count * (n + 1), 2 * N, 11, 60).reshape(count, n + 1)
+    tv = _sign_tv(N)
+    got = ctx.bootstrap_batch(lwes, tv)
+    ref, _ = oracle.bootstrap_batch(N, lwes.reshape(-1), tv, brk, n, b, l, oracle.BINARY, ksk, n_out, kb, kd, 1)
+    assert np.array_equal(got, ref)
+    ks = [int(x) for x in lk]
+    for c in range(count):
+        ph = oracle.phase_lwe(got[c], lk, n_out)
+        p = (int(lwes[c][n]) - sum(int(a) * s for a, s in zip(lwes[c][:n], ks))) % (2 * N)
+        want = Q // 8 if (p == 0 or p > N) else Q - Q // 8
+        err = (ph - want) % Q
+        err = err if err < Q // 2 else err - Q
+        assert abs(err) < Q // 16, (c, p)
