@@ -131,7 +131,7 @@ def main():
         ctx.sync()
         if i:
             plain_lat.append(time.perf_counter() - t)
-    plain_bs = 1.0 / (sum(plain_lat) / len(plain_lat))
+    plain_bs = 1.0 / (sum(plain_lat) / len(plain_lat)) if plain_lat else float("nan")
 
     # batch latency percentiles: every bootstrap of a batch completes with its batch
     lat_ms = sorted(br + ks for br, ks in zip(br_ms, ks_ms))
@@ -162,7 +162,7 @@ def main():
         "plain_bootstraps_per_s": round(plain_bs, 2),
         "batched_vs_plain": round(value / plain_bs, 2),
         "latency_ms": {"batch_p50": round(p50, 3), "batch_p99": round(p99, 3),
-                       "plain_mean": round(1000 * sum(plain_lat) / len(plain_lat), 3)},
+                       "plain_mean": round(1000 * sum(plain_lat) / max(len(plain_lat), 1), 3)},
         "kernel_ms": {"blind_rotate_avg": round(1000 * br_avg_s, 3),
                       "keyswitch_avg": round(sum(ks_ms) / len(ks_ms), 3)},
         "roofline": {"bound": "valu", "achieved": round(achieved, 4), "peak": peak, "unit": "Tmulmod/s",

@@ -82,6 +82,88 @@ GL_FN uint64_t mul_pow2(uint64_t x, int e) {
     return negate ? neg(r) : r;
 }
 
+// ------------------------------------------------------------------ lazy forms
+// "Lazy" values are any 64-bit integers standing for their residue mod Q; they let
+// butterflies skip canonicalisation.  Digit extraction and outputs use canonical values.
+
+// Any 128-bit (hi, lo) -> lazy r == x mod Q (no final canonicalisation).
+GL_FN uint64_t reduce128_lazy(uint64_t hi, uint64_t lo) {
+    const uint64_t h0 = (uint32_t)hi, h1 = hi >> 32;
+    uint64_t s, d;
+    const bool c = __builtin_add_overflow(h0 * kEps, lo, &s);  // + 2^64 == + eps
+    const bool b = __builtin_sub_overflow(s, h1, &d);          // - 2^64 == - eps
+    return d + (uint64_t)(((int64_t)c - (int64_t)b) * (int64_t)kEps);
+}
+
+GL_FN uint64_t mul_lazy(uint64_t a, uint64_t b) {
+    uint64_t hi, lo;
+    mul_wide(a, b, hi, lo);
+    return reduce128_lazy(hi, lo);
+}
+
+// a lazy, b < Q: one carry correction suffices (a + b < 2^64 + Q).
+GL_FN uint64_t add_lc(uint64_t a, uint64_t b) {
+    uint64_t s;
+    const bool c = __builtin_add_overflow(a, b, &s);
+    return s + (c ? kEps : 0);
+}
+
+// a lazy, b <= Q: one borrow correction suffices.
+GL_FN uint64_t sub_lc(uint64_t a, uint64_t b) {
+    uint64_t d;
+    const bool c = __builtin_sub_overflow(a, b, &d);
+    return d - (c ? kEps : 0);
+}
+
+// both lazy
+GL_FN uint64_t add_ll(uint64_t a, uint64_t b) {
+    uint64_t s, s2;
+    const bool c = __builtin_add_overflow(a, b, &s);
+    const bool c2 = __builtin_add_overflow(s, c ? kEps : 0, &s2);
+    return s2 + (c2 ? kEps : 0);
+}
+
+GL_FN uint64_t sub_ll(uint64_t a, uint64_t b) {
+    uint64_t d, d2;
+    const bool c = __builtin_sub_overflow(a, b, &d);
+    const bool c2 = __builtin_sub_overflow(d, c ? kEps : 0, &d2);
+    return d2 - (c2 ? kEps : 0);
+}
+
+// x * 2^e for 0 < e < 32, x lazy: (x << e) + (x >> (64 - e)) eps.
+GL_FN uint64_t mul_2e_small(uint64_t x, int e) {
+    uint64_t s;
+    const bool c = __builtin_add_overflow((x >> (64 - e)) * kEps, x << e, &s);
+    return s + (c ? kEps : 0);
+}
+
+// x * 2^32 = x1 eps + (x0 << 32), x lazy.
+GL_FN uint64_t mul_2_32(uint64_t x) {
+    uint64_t s;
+    const bool c = __builtin_add_overflow((x >> 32) * kEps, x << 32, &s);
+    return s + (c ? kEps : 0);
+}
+
+// x * 2^-s for 0 < s <= 32, x lazy (q + r 2^-s with r 2^-s = -r eps 2^(32-s)).
+GL_FN uint64_t div_2e_lazy(uint64_t x, int s) {
+    const uint64_t q = x >> s;
+    const uint64_t v = (x & ((1ULL << s) - 1)) << (32 - s);
+    return sub_lc(q, (v << 32) - v);
+}
+
+// Lazy x * 2^e for 0 <= e < 192 when e is a compile-time constant after unrolling; the sign
+// of 2^e (2^96 = -1) is returned separately so butterflies can swap outputs instead.
+GL_FN uint64_t mul_2e_unsigned(uint64_t x, int e, bool& negate) {
+    negate = false;
+    if (e >= 96) { e -= 96; negate = true; }
+    if (e == 0) return x;
+    if (e < 32) return mul_2e_small(x, e);
+    if (e == 32) return mul_2_32(x);
+    if (e < 64) return mul_2e_small(mul_2_32(x), e - 32);
+    negate = !negate;                 // 2^e = -2^-(96 - e)
+    return div_2e_lazy(x, 96 - e);
+}
+
 GL_FN uint64_t pow(uint64_t b, uint64_t e) {
     uint64_t r = 1;
     b = canon(b);

@@ -29,38 +29,30 @@ __device__ __forceinline__ void load_key_row(uint64_t (&kv)[kE], const uint64_t*
     }
 }
 
-// NTT(X^e) - 1 at the lane's 16 NTT positions, e in [0, 2N): the table holds NTT(X^e) for
-// e < N in key-row order; X^e = -X^(e-N) for e >= N.
+// NTT(X^e) at the lane's NTT positions (2c, 2c+1), e in [0, 2N): the table holds NTT(X^e)
+// for e < N in key-row order; X^e = -X^(e-N) for e >= N.
 template <int LOGN>
-__device__ __forceinline__ void load_mono(uint64_t (&w)[kE], const uint64_t* __restrict__ mono, uint32_t e, int L) {
+__device__ __forceinline__ U64x2 mono_pair(const uint64_t* __restrict__ mono, uint32_t e, int c, int L) {
     constexpr int N = Shape<LOGN>::N;
-    const bool ng = e >= (uint32_t)N;
-    load_key_row<LOGN>(w, mono + (size_t)(e & (N - 1)) * N, L);
-    if (ng) {
-#pragma unroll
-        for (int k = 0; k < kE; ++k) w[k] = gl::neg(w[k]);
-    }
+    const U64x2 v = reinterpret_cast<const U64x2*>(mono + (size_t)(e & (N - 1)) * N)[c * Shape<LOGN>::P + L];
+    if (e >= (uint32_t)N) return U64x2{gl::neg(v.x), gl::neg(v.y)};
+    return v;
 }
 
+// NTT(X^e) - 1
 template <int LOGN>
-__device__ __forceinline__ void load_weight(uint64_t (&w)[kE], const uint64_t* __restrict__ mono, uint32_t e, int L) {
-    load_mono<LOGN>(w, mono, e, L);
-#pragma unroll
-    for (int k = 0; k < kE; ++k) w[k] = gl::sub(w[k], 1);
+__device__ __forceinline__ U64x2 weight_pair(const uint64_t* __restrict__ mono, uint32_t e, int c, int L) {
+    const U64x2 v = mono_pair<LOGN>(mono, e, c, L);
+    return U64x2{gl::sub(v.x, 1), gl::sub(v.y, 1)};
 }
 
 // BMMP third weight X^(a0+a1) - X^a0 - X^a1 + 1 (bmmp_blind_rotation.cpp:280-323).
 template <int LOGN>
-__device__ __forceinline__ void load_bmmp_w2(uint64_t (&w)[kE], const uint64_t* __restrict__ mono, uint32_t e0,
-                                             uint32_t e1, uint32_t e2, int L) {
-    uint64_t t[kE];
-    load_mono<LOGN>(w, mono, e2, L);
-    load_mono<LOGN>(t, mono, e0, L);
-#pragma unroll
-    for (int k = 0; k < kE; ++k) w[k] = gl::sub(w[k], t[k]);
-    load_mono<LOGN>(t, mono, e1, L);
-#pragma unroll
-    for (int k = 0; k < kE; ++k) w[k] = gl::add(gl::sub(w[k], t[k]), 1);
+__device__ __forceinline__ U64x2 bmmp_w2_pair(const uint64_t* __restrict__ mono, uint32_t e0, uint32_t e1,
+                                              uint32_t e2, int c, int L) {
+    const U64x2 m2 = mono_pair<LOGN>(mono, e2, c, L), m0 = mono_pair<LOGN>(mono, e0, c, L),
+                m1 = mono_pair<LOGN>(mono, e1, c, L);
+    return U64x2{gl::add(gl::sub(gl::sub(m2.x, m0.x), m1.x), 1), gl::add(gl::sub(gl::sub(m2.y, m0.y), m1.y), 1)};
 }
 
 // ------------------------------------------------------------------ batched NTT
@@ -83,14 +75,14 @@ __global__ void __launch_bounds__(256) ntt_batch_kernel(uint64_t* __restrict__ o
             ntt_forward<LOGN>(x, lds[g], tw, L);
             if (live)
 #pragma unroll
-                for (int k = 0; k < kE; ++k) out[p * S::N + idx3(L, k)] = x[k];
+                for (int k = 0; k < kE; ++k) out[p * S::N + idx3(L, k)] = gl::canon(x[k]);
         } else {
 #pragma unroll
             for (int k = 0; k < kE; ++k) x[k] = src[idx3(L, k)];
             ntt_inverse<LOGN>(x, lds[g], tw, L);
             if (live)
 #pragma unroll
-                for (int k = 0; k < kE; ++k) out[p * S::N + idx1<LOGN>(L, k)] = x[k];
+                for (int k = 0; k < kE; ++k) out[p * S::N + idx1<LOGN>(L, k)] = gl::canon(x[k]);
         }
     }
 }
@@ -130,11 +122,13 @@ __device__ __forceinline__ void rotate_from(uint64_t (&acc)[kE], const uint64_t*
 //   (BMMP); sum_c w_c (RGSW_c [x] acc) equals the reference's per-control products, weighted
 //   and summed, in Z_Q.
 template <int LOGN, int LD, int METHOD>
-__global__ void __launch_bounds__(Shape<LOGN>::P) br_kernel(BrArgs A) {
+__global__ void __launch_bounds__(Shape<LOGN>::P) __attribute__((amdgpu_waves_per_eu(2)))
+br_kernel(BrArgs A) {
     using S = Shape<LOGN>;
     constexpr int N = S::N;
     constexpr int NC = METHOD == 0 ? 1 : (METHOD == 1 ? 2 : 3);  // RGSWs per step
     __shared__ uint64_t buf[S::LDS_WORDS];
+    __shared__ uint64_t src[METHOD == 0 ? S::LDS_WORDS : 1];
     const int L = threadIdx.x;
     const uint64_t mask = (1ULL << A.base_log2) - 1;
     const size_t rgsw_words = (size_t)4 * N * LD;
@@ -166,77 +160,100 @@ __global__ void __launch_bounds__(Shape<LOGN>::P) br_kernel(BrArgs A) {
 #pragma unroll
             for (int k = 0; k < kE; ++k) out[0][k] = out[1][k] = 0;
 
-#pragma unroll
+#pragma unroll 1
             for (int h = 0; h < 2; ++h) {
-                uint64_t x[kE];
                 if constexpr (METHOD == 0) {
-                    // x = X^e0 acc_h - acc_h, through LDS
+                    // the half being decomposed is parked in LDS so that every digit can re-read
+                    // its rotation X^e0 acc_h - acc_h (kept out of registers)
                     poly_sync<S::WAVES>();
 #pragma unroll
-                    for (int k = 0; k < kE; ++k) buf[phys(idx1<LOGN>(L, k))] = acc[h][k];
+                    for (int k = 0; k < kE; ++k) src[phys(idx1<LOGN>(L, k))] = acc[0][k];
                     poly_sync<S::WAVES>();
-                    rotate_from<LOGN>(x, buf, e0, L, true);
-#pragma unroll
-                    for (int k = 0; k < kE; ++k) x[k] = gl::sub(x[k], acc[h][k]);
-                } else {
-                    // signed digits work on x + C mod Q (gadget_decomp.h:169-173)
-#pragma unroll
-                    for (int k = 0; k < kE; ++k) x[k] = gl::add(acc[h][k], A.corrector);
                 }
+#pragma unroll 1
                 for (int j = 0; j < LD; ++j) {
                     uint64_t d[kE];
                     if constexpr (METHOD == 0) {
                         const int shift = A.first_shift - j * A.base_log2;
+                        rotate_from<LOGN>(d, src, e0, L, true);
 #pragma unroll
-                        for (int k = 0; k < kE; ++k) d[k] = (x[k] >> shift) & mask;
+                        for (int k = 0; k < kE; ++k) d[k] = (gl::sub(d[k], acc[0][k]) >> shift) & mask;
                     } else {
+                        // signed digits of x + C mod Q (gadget_decomp.h:169-198)
                         const bool last = (j == A.max_digits - 1);
                         const int shift = last ? 0 : 64 - (j + 1) * A.base_log2;
                         const uint64_t dm = last ? (2 * A.least - 1) : mask;
                         const uint64_t delta = last ? A.least : (1ULL << (A.base_log2 - 1));
 #pragma unroll
-                        for (int k = 0; k < kE; ++k) d[k] = gl::sub((x[k] >> shift) & dm, delta);
+                        for (int k = 0; k < kE; ++k)
+                            d[k] = gl::sub((gl::add(acc[0][k], A.corrector) >> shift) & dm, delta);
                     }
                     ntt_forward<LOGN>(d, buf, A.tw, L);
                     const size_t row = (size_t)(h * LD + j) * 2 * N;
                     if constexpr (METHOD == 0) {
-                        uint64_t ka[kE], kb[kE];
-                        load_key_row<LOGN>(ka, key + row, L);
-                        load_key_row<LOGN>(kb, key + row + N, L);
+                        // key pairs are read next to their use, not hoisted over the transform
+                        const U64x2* ka = reinterpret_cast<const U64x2*>(opaque(key + row));
+                        const U64x2* kb = reinterpret_cast<const U64x2*>(opaque(key + row + N));
 #pragma unroll
-                        for (int k = 0; k < kE; ++k) {
-                            out[0][k] = gl::add(out[0][k], gl::mul(d[k], ka[k]));
-                            out[1][k] = gl::add(out[1][k], gl::mul(d[k], kb[k]));
+                        for (int c = 0; c < kE / 2; ++c) {
+                            const U64x2 va = ka[c * S::P + L], vb = kb[c * S::P + L];
+                            out[0][2 * c] = gl::add_ll(out[0][2 * c], gl::mul_lazy(d[2 * c], va.x));
+                            out[0][2 * c + 1] = gl::add_ll(out[0][2 * c + 1], gl::mul_lazy(d[2 * c + 1], va.y));
+                            out[1][2 * c] = gl::add_ll(out[1][2 * c], gl::mul_lazy(d[2 * c], vb.x));
+                            out[1][2 * c + 1] = gl::add_ll(out[1][2 * c + 1], gl::mul_lazy(d[2 * c + 1], vb.y));
                         }
                     } else {
+                        // combined key sum_c w_c RGSW_c[row], formed pair by pair next to its use
+                        const uint64_t* krow = opaque(key + row);
+                        const uint64_t* mono = opaque_after(A.mono, d[0]);  // weights re-read per digit
 #pragma unroll
-                        for (int p = 0; p < 2; ++p) {
-                            uint64_t kc[kE], w[kE], comb[kE];
-                            load_key_row<LOGN>(kc, key + row + p * N, L);
-                            load_weight<LOGN>(w, A.mono, e0, L);
+                        for (int c = 0; c < kE / 2; ++c) {
+                            const U64x2 w0 = weight_pair<LOGN>(mono, e0, c, L);
+                            const U64x2 w1 = weight_pair<LOGN>(mono, e1, c, L);
+                            U64x2 w2{0, 0};
+                            if constexpr (NC == 3) w2 = bmmp_w2_pair<LOGN>(mono, e0, e1, e2, c, L);
 #pragma unroll
-                            for (int k = 0; k < kE; ++k) comb[k] = gl::mul(kc[k], w[k]);
-                            load_key_row<LOGN>(kc, key + rgsw_words + row + p * N, L);
-                            load_weight<LOGN>(w, A.mono, e1, L);
-#pragma unroll
-                            for (int k = 0; k < kE; ++k) comb[k] = gl::add(comb[k], gl::mul(kc[k], w[k]));
-                            if constexpr (NC == 3) {
-                                load_key_row<LOGN>(kc, key + 2 * rgsw_words + row + p * N, L);
-                                load_bmmp_w2<LOGN>(w, A.mono, e0, e1, e2, L);
-#pragma unroll
-                                for (int k = 0; k < kE; ++k) comb[k] = gl::add(comb[k], gl::mul(kc[k], w[k]));
+                            for (int p = 0; p < 2; ++p) {
+                                const U64x2* k0 = reinterpret_cast<const U64x2*>(krow + p * N);
+                                const U64x2* k1 = reinterpret_cast<const U64x2*>(krow + rgsw_words + p * N);
+                                const U64x2 a0 = k0[c * S::P + L], a1 = k1[c * S::P + L];
+                                uint64_t cx = gl::add_ll(gl::mul_lazy(a0.x, w0.x), gl::mul_lazy(a1.x, w1.x));
+                                uint64_t cy = gl::add_ll(gl::mul_lazy(a0.y, w0.y), gl::mul_lazy(a1.y, w1.y));
+                                if constexpr (NC == 3) {
+                                    const U64x2* k2 = reinterpret_cast<const U64x2*>(krow + 2 * rgsw_words + p * N);
+                                    const U64x2 a2 = k2[c * S::P + L];
+                                    cx = gl::add_ll(cx, gl::mul_lazy(a2.x, w2.x));
+                                    cy = gl::add_ll(cy, gl::mul_lazy(a2.y, w2.y));
+                                }
+                                if (p == 0) {
+                                    out[0][2 * c] = gl::add_ll(out[0][2 * c], gl::mul_lazy(d[2 * c], cx));
+                                    out[0][2 * c + 1] = gl::add_ll(out[0][2 * c + 1], gl::mul_lazy(d[2 * c + 1], cy));
+                                } else {
+                                    out[1][2 * c] = gl::add_ll(out[1][2 * c], gl::mul_lazy(d[2 * c], cx));
+                                    out[1][2 * c + 1] = gl::add_ll(out[1][2 * c + 1], gl::mul_lazy(d[2 * c + 1], cy));
+                                }
                             }
-#pragma unroll
-                            for (int k = 0; k < kE; ++k) out[p][k] = gl::add(out[p][k], gl::mul(d[k], comb[k]));
                         }
                     }
                 }
+                // the half being decomposed is always acc[0]: exchange the halves in registers
+#pragma unroll
+                for (int k = 0; k < kE; ++k) {
+                    const uint64_t t = acc[0][k];
+                    acc[0][k] = acc[1][k];
+                    acc[1][k] = t;
+                }
             }
-#pragma unroll
+#pragma unroll 1
             for (int h = 0; h < 2; ++h) {
-                ntt_inverse<LOGN>(out[h], buf, A.itw, L);
+                ntt_inverse<LOGN>(out[0], buf, A.itw, L);
 #pragma unroll
-                for (int k = 0; k < kE; ++k) acc[h][k] = gl::add(acc[h][k], out[h][k]);
+                for (int k = 0; k < kE; ++k) {
+                    const uint64_t t = gl::canon(gl::add_lc(out[0][k], acc[0][k]));
+                    acc[0][k] = acc[1][k];
+                    acc[1][k] = t;
+                    out[0][k] = out[1][k];
+                }
             }
         }
 
@@ -257,7 +274,7 @@ __global__ void __launch_bounds__(Shape<LOGN>::P) br_kernel(BrArgs A) {
                 if (A.out_rlwe_ntt) {
                     ntt_forward<LOGN>(acc[h], buf, A.tw, L);
 #pragma unroll
-                    for (int k = 0; k < kE; ++k) o[idx3(L, k)] = acc[h][k];
+                    for (int k = 0; k < kE; ++k) o[idx3(L, k)] = gl::canon(acc[h][k]);
                 } else {
 #pragma unroll
                     for (int k = 0; k < kE; ++k) o[idx1<LOGN>(L, k)] = acc[h][k];

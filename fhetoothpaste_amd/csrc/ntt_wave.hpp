@@ -75,16 +75,59 @@ __device__ __forceinline__ void transpose(uint64_t (&x)[kE], uint64_t* lds, int 
     for (int k = 0; k < kE; ++k) x[k] = lds[phys(layout_idx<LOGN, B>(L, k))];
 }
 
-__device__ __forceinline__ void ct_bfly(uint64_t& u, uint64_t& v, uint64_t wv) {
+// Butterflies on lazy values (any 64-bit representative): the product is made canonical
+// once, then one carry/borrow correction suffices for u + t and u - t.
+__device__ __forceinline__ void ct_bfly_t(uint64_t& u, uint64_t& v, uint64_t t_lazy, bool negate) {
+    const uint64_t t = gl::canon(t_lazy);
     const uint64_t a = u;
-    u = gl::add(a, wv);
-    v = gl::sub(a, wv);
+    const uint64_t s = gl::add_lc(a, t), d = gl::sub_lc(a, t);
+    u = negate ? d : s;  // u + (-t) = u - t: swap where the results go
+    v = negate ? s : d;
 }
 
-__device__ __forceinline__ void gs_bfly(uint64_t& u, uint64_t& v, uint64_t w) {
+__device__ __forceinline__ void ct_bfly_pow2(uint64_t& u, uint64_t& v, int e) {
+    bool ng;
+    const uint64_t t = gl::mul_2e_unsigned(v, e, ng);
+    ct_bfly_t(u, v, t, ng);
+}
+
+__device__ __forceinline__ void ct_bfly_mul(uint64_t& u, uint64_t& v, uint64_t w) {
+    ct_bfly_t(u, v, gl::mul_lazy(v, w), false);
+}
+
+__device__ __forceinline__ void gs_bfly_mul(uint64_t& u, uint64_t& v, uint64_t w) {
     const uint64_t a = u, b = v;
-    u = gl::add(a, b);
-    v = gl::mul(gl::sub(a, b), w);
+    u = gl::add_ll(a, b);
+    v = gl::mul_lazy(gl::sub_ll(a, b), w);
+}
+
+// (u, v) -> (u + v, (u - v) 2^e); a negative 2^e swaps the operands of the difference.
+__device__ __forceinline__ void gs_bfly_pow2(uint64_t& u, uint64_t& v, int e) {
+    const uint64_t a = u, b = v;
+    u = gl::add_ll(a, b);
+    const bool neg = (e >= 96) != (e % 96 >= 64);
+    const uint64_t diff = neg ? gl::sub_ll(b, a) : gl::sub_ll(a, b);
+    bool ng;
+    v = gl::mul_2e_unsigned(diff, e, ng);
+}
+
+// Twiddle tables are read inside the step loops rather than held in registers across them.
+__device__ __forceinline__ const uint64_t* opaque(const uint64_t* p) {
+    asm volatile("" : "+s"(p));
+    return p;
+}
+
+// Same, but ordered after the value v is computed: a level's twiddles are fetched when the
+// previous level is done, so only one level's twiddles are live at a time.
+__device__ __forceinline__ const uint64_t* opaque_after(const uint64_t* p, uint64_t v) {
+    asm volatile("" : "+s"(p) : "v"(v));
+    return p;
+}
+
+// Element idx of a table at a wave-uniform base: a 32-bit byte offset lets the load use the
+// scalar base register plus one offset register, with the constant part in the immediate.
+__device__ __forceinline__ uint64_t ld_u64(const uint64_t* base, uint32_t idx) {
+    return *reinterpret_cast<const uint64_t*>(reinterpret_cast<const char*>(base) + (size_t)(idx * 8u));
 }
 
 // ---------------------------------------------------------------- forward
@@ -98,7 +141,7 @@ __device__ __forceinline__ void fwd_pass1(uint64_t (&x)[kE]) {
         for (int k = 0; k < kE; ++k) {
             if (k & half) continue;
             const int blk = k >> (4 - lv);
-            ct_bfly(x[k], x[k + half], gl::mul_pow2(x[k + half], gl::kFwdPow2[(1 << lv) + blk]));
+            ct_bfly_pow2(x[k], x[k + half], gl::kFwdPow2[(1 << lv) + blk]);
         }
     }
 }
@@ -109,12 +152,12 @@ __device__ __forceinline__ void fwd_pass2(uint64_t (&x)[kE], const uint64_t* __r
 #pragma unroll
     for (int lv = 4; lv < 8; ++lv) {
         const int half = 8 >> (lv - 4);
+        const uint64_t* t = opaque_after(tw, x[0]);
 #pragma unroll
         for (int k = 0; k < kE; ++k) {
             if (k & half) continue;
-            const int blk = ((L >> lb) << (lv - 4)) | (k >> (8 - lv));
-            const uint64_t w = tw[(1 << lv) + blk];
-            ct_bfly(x[k], x[k + half], gl::mul(x[k + half], w));
+            const uint32_t blk = (((uint32_t)L >> lb) << (lv - 4)) + (k >> (8 - lv));
+            ct_bfly_mul(x[k], x[k + half], ld_u64(t, (1u << lv) + blk));
         }
     }
 }
@@ -124,25 +167,26 @@ __device__ __forceinline__ void fwd_pass3(uint64_t (&x)[kE], const uint64_t* __r
 #pragma unroll
     for (int lv = 8; lv < LOGN; ++lv) {
         const int half = 1 << (LOGN - 1 - lv);
+        const uint64_t* t = opaque_after(tw, x[0]);
 #pragma unroll
         for (int k = 0; k < kE; ++k) {
             if (k & half) continue;
-            const int blk = ((L << 4) | k) >> (LOGN - lv);
-            const uint64_t w = tw[(1 << lv) + blk];
-            ct_bfly(x[k], x[k + half], gl::mul(x[k + half], w));
+            const uint32_t blk = (((uint32_t)L << 4) + k) >> (LOGN - lv);
+            ct_bfly_mul(x[k], x[k + half], ld_u64(t, (1u << lv) + blk));
         }
     }
 }
 
-// Natural-order coefficients in pass-1 layout -> NTT values in pass-3 layout.
+// Natural-order coefficients (pass-1 layout, lazy or canonical) -> lazy NTT values in
+// pass-3 layout.
 template <int LOGN>
-__device__ __forceinline__ void ntt_forward(uint64_t (&x)[kE], uint64_t* lds, const uint64_t* __restrict__ tw,
-                                            int L) {
+__device__ __forceinline__ void ntt_forward(uint64_t (&x)[kE], uint64_t* lds, const uint64_t* tw, int L) {
+    tw = opaque(tw);
     fwd_pass1<LOGN>(x);
     transpose<LOGN, Layout::kPass1, Layout::kPass2>(x, lds, L);
-    fwd_pass2<LOGN>(x, tw, L);
+    fwd_pass2<LOGN>(x, opaque(tw), L);
     transpose<LOGN, Layout::kPass2, Layout::kPass3>(x, lds, L);
-    fwd_pass3<LOGN>(x, tw, L);
+    fwd_pass3<LOGN>(x, opaque(tw), L);
 }
 
 // ---------------------------------------------------------------- inverse
@@ -152,11 +196,12 @@ __device__ __forceinline__ void inv_pass3(uint64_t (&x)[kE], const uint64_t* __r
 #pragma unroll
     for (int lv = LOGN - 1; lv >= 8; --lv) {
         const int half = 1 << (LOGN - 1 - lv);
+        const uint64_t* t = opaque_after(itw, x[0]);
 #pragma unroll
         for (int k = 0; k < kE; ++k) {
             if (k & half) continue;
-            const int blk = ((L << 4) | k) >> (LOGN - lv);
-            gs_bfly(x[k], x[k + half], itw[(1 << lv) + blk]);
+            const uint32_t blk = (((uint32_t)L << 4) + k) >> (LOGN - lv);
+            gs_bfly_mul(x[k], x[k + half], ld_u64(t, (1u << lv) + blk));
         }
     }
 }
@@ -167,41 +212,50 @@ __device__ __forceinline__ void inv_pass2(uint64_t (&x)[kE], const uint64_t* __r
 #pragma unroll
     for (int lv = 7; lv >= 4; --lv) {
         const int half = 8 >> (lv - 4);
+        const uint64_t* t = opaque_after(itw, x[0]);
 #pragma unroll
         for (int k = 0; k < kE; ++k) {
             if (k & half) continue;
-            const int blk = ((L >> lb) << (lv - 4)) | (k >> (8 - lv));
-            gs_bfly(x[k], x[k + half], itw[(1 << lv) + blk]);
+            const uint32_t blk = (((uint32_t)L >> lb) << (lv - 4)) + (k >> (8 - lv));
+            gs_bfly_mul(x[k], x[k + half], ld_u64(t, (1u << lv) + blk));
         }
     }
 }
 
-// Levels m = 8..1 with power-of-two inverse twiddles, then the N^-1 scaling.
+// Levels m = 8..1 with power-of-two inverse twiddles; N^-1 = 2^(192 - LOGN) is merged into
+// the last level (both of its factors are powers of two).
 template <int LOGN>
 __device__ __forceinline__ void inv_pass1(uint64_t (&x)[kE]) {
 #pragma unroll
-    for (int lv = 3; lv >= 0; --lv) {
+    for (int lv = 3; lv >= 1; --lv) {
         const int half = 8 >> lv;
 #pragma unroll
         for (int k = 0; k < kE; ++k) {
             if (k & half) continue;
             const int blk = k >> (4 - lv);
-            const uint64_t a = x[k], b = x[k + half];
-            x[k] = gl::add(a, b);
-            x[k + half] = gl::mul_pow2(gl::sub(a, b), gl::kInvPow2[(1 << lv) + blk]);
+            gs_bfly_pow2(x[k], x[k + half], gl::kInvPow2[(1 << lv) + blk]);
         }
     }
+    constexpr int kScale = 192 - LOGN;
 #pragma unroll
-    for (int k = 0; k < kE; ++k) x[k] = gl::div_pow2(x[k], LOGN);
+    for (int k = 0; k < 8; ++k) {
+        const uint64_t a = x[k], b = x[k + 8];
+        x[k] = gl::div_2e_lazy(gl::add_ll(a, b), LOGN);
+        const int e = (gl::kInvPow2[1] + kScale) % 192;
+        const bool neg = (e >= 96) != (e % 96 >= 64);
+        const uint64_t diff = neg ? gl::sub_ll(b, a) : gl::sub_ll(a, b);
+        bool ng;
+        x[k + 8] = gl::mul_2e_unsigned(diff, e, ng);
+    }
 }
 
-// NTT values in pass-3 layout -> natural-order coefficients in pass-1 layout.
+// Lazy NTT values in pass-3 layout -> lazy natural-order coefficients in pass-1 layout.
 template <int LOGN>
-__device__ __forceinline__ void ntt_inverse(uint64_t (&x)[kE], uint64_t* lds, const uint64_t* __restrict__ itw,
-                                            int L) {
+__device__ __forceinline__ void ntt_inverse(uint64_t (&x)[kE], uint64_t* lds, const uint64_t* itw, int L) {
+    itw = opaque(itw);
     inv_pass3<LOGN>(x, itw, L);
     transpose<LOGN, Layout::kPass3, Layout::kPass2>(x, lds, L);
-    inv_pass2<LOGN>(x, itw, L);
+    inv_pass2<LOGN>(x, opaque(itw), L);
     transpose<LOGN, Layout::kPass2, Layout::kPass1>(x, lds, L);
     inv_pass1<LOGN>(x);
 }

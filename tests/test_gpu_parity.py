@@ -155,3 +155,48 @@ def test_bootstrap_batch_cfg2_shape(tp, oracle):
         err = (ph - want) % Q
         err = err if err < Q // 2 else err - Q
         assert abs(err) < Q // 16, (c, p)
+
+
+@pytest.mark.parametrize("method,N,n,b,l", [(1, 1024, 14, 16, 4), (1, 2048, 10, 16, 4), (2, 1024, 12, 14, 3),
+                                            (2, 2048, 10, 14, 3), (2, 1024, 8, 16, 4)])
+def test_blind_rotate_ternary_bmmp(tp, oracle, method, N, n, b, l):
+    say("blind_rotate method", method, N, n, b, l)
+    dist = 1 if method == oracle.TERNARY else 0
+    lk, rk, brk = _keys(oracle, N, n, b, l, method, dist, 3.2, 40 + n)
+    ctx = tp.Context(N, n, b, l, method)
+    ctx.import_brk(brk)
+    rng = np.random.default_rng(n + method)
+    for trial in range(3):
+        lwe = rng.integers(0, 2 * N, size=n + 1, dtype=np.uint64)
+        if trial == 1:
+            lwe[::2] = 0
+            lwe[1] = N
+        acc = rng.integers(0, Q, size=2 * N, dtype=np.uint64)
+        for as_coef in (False, True):
+            got = ctx.blind_rotate(lwe, acc, as_coef)
+            ref = oracle.blind_rotate(N, lwe, acc, brk, n, b, l, method, as_coef)
+            assert np.array_equal(got, ref), (trial, as_coef)
+
+
+@pytest.mark.parametrize("method,N,n,b,l,count", [(1, 2048, 800, 16, 4, 2), (2, 2048, 1024, 14, 3, 2)])
+def test_eval_func_ternary_bmmp_full_size(tp, oracle, method, N, n, b, l, count):
+    """Configs 3 and 4 parameter sets (ternary n=800 N=2048; BMMP n=1024 N=2048 b=14 l=3)."""
+    say("eval_func full size method", method, N, n)
+    dist = 1 if method == oracle.TERNARY else 0
+    lk, rk, brk = _keys(oracle, N, n, b, l, method, dist, 3.2, 90 + method)
+    ctx = tp.Context(N, n, b, l, method)
+    ctx.import_brk(brk)
+    lwes = oracle.random_mod_pow2(count * (n + 1), 2 * N, 13, 70 + method).reshape(count, n + 1)
+    tv = _sign_tv(N)
+    got, rl = ctx.eval_func_batch(lwes, tv, want_rlwe=True, rlwe_ntt=True)
+    for c in range(count):
+        ref = oracle.blind_rotate(N, lwes[c], tv, brk, n, b, l, method, False)
+        assert np.array_equal(rl[c], ref), c
+    ks = [int(x) if int(x) < 2 else -1 for x in lk]
+    for c in range(count):
+        ph = oracle.phase_lwe(got[c], rk, N)
+        p = (int(lwes[c][n]) - sum(int(a) * s for a, s in zip(lwes[c][:n], ks))) % (2 * N)
+        want = Q // 8 if (p == 0 or p > N) else Q - Q // 8
+        err = (ph - want) % Q
+        err = err if err < Q // 2 else err - Q
+        assert abs(err) < Q // 16, (c, p)
