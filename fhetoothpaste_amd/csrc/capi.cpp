@@ -282,9 +282,11 @@ int tp_import_lwe_ksk(tp_context* c, const uint64_t* ksk, size_t words) {
     c->ks_corrector = C % gl::kQ;
     if (c->d_ksk) { hipFree(c->d_ksk); c->d_ksk = nullptr; }
     if (c->d_corr) { hipFree(c->d_corr); c->d_corr = nullptr; }
-    if (hipMalloc(&c->d_ksk, words * 8) != hipSuccess || hipMalloc(&c->d_corr, W * 8) != hipSuccess)
+    if (hipMalloc(&c->d_ksk, (words + tp::kKsPadWords) * 8) != hipSuccess ||
+        hipMalloc(&c->d_corr, W * 8) != hipSuccess)
         return fail(TP_E_NOMEM, "ksk allocation failed");
     TP_HIP(hipMemcpy(c->d_ksk, ksk, words * 8, hipMemcpyHostToDevice));
+    TP_HIP(hipMemset(c->d_ksk + words, 0, tp::kKsPadWords * 8));
     TP_HIP(hipMemcpy(c->d_corr, corr.data(), W * 8, hipMemcpyHostToDevice));
     c->ksk_words = words;
     return TP_OK;

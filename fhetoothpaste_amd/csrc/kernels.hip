@@ -291,52 +291,51 @@ br_kernel(BrArgs A) {
 // (lwe_conversion.cpp:205-239).  With unsigned digits u_ij = d_ij + delta_j the sum is
 // sum u_ij K - corr[k], corr[k] = sum_ij delta_j K[i][j][k] precomputed per key.  u_ij < 2^b, so
 // u K_lo and u K_hi are accumulated exactly in 64 bits and reduced once at the end.
-// Block: 4 waves; wave w handles 16 ciphertexts, lane = one output column of a 64-wide tile.
-constexpr int kKsCts = 16;
+// GEMM layout: one ciphertext per lane (its digits are per-lane work), a tile of kKsCols key
+// columns per block; the key row segment K[i][j][col0 .. col0 + kKsCols) is wave-uniform and is
+// read through the scalar cache, so each MAC is two v_mad_u64_u32 with a scalar operand.
+constexpr int kKsCols = 16;
+constexpr int kKsBlock = 256;
 
-__global__ void __launch_bounds__(256) lwe_keyswitch_kernel(KsArgs A) {
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int col = blockIdx.x * 64 + lane;
-    const int ct0 = (blockIdx.y * 4 + w) * kKsCts;
+__global__ void __launch_bounds__(kKsBlock) lwe_keyswitch_kernel(KsArgs A) {
+    const int col0 = blockIdx.x * kKsCols;
+    const int ct = blockIdx.y * kKsBlock + threadIdx.x;
     const int W = A.n_out + 1;
-    const bool col_ok = col < W;
-    uint64_t lo[kKsCts], hi[kKsCts];
+    const bool live = ct < A.count;
+    const uint64_t* in = A.in + (size_t)(live ? ct : 0) * A.in_stride;
+    uint64_t lo[kKsCols], hi[kKsCols];
 #pragma unroll
-    for (int c = 0; c < kKsCts; ++c) lo[c] = hi[c] = 0;
+    for (int t = 0; t < kKsCols; ++t) lo[t] = hi[t] = 0;
     const uint64_t mask = (1ULL << A.base_log2) - 1;
-    const uint64_t* in0 = A.in + (size_t)ct0 * A.in_stride;
+    // the device key carries kKsCols words of zero padding, so a partial last tile reads in
+    // bounds; its extra columns are discarded
+    const uint64_t* krow = A.ksk + col0;
     for (int i = 0; i < A.n_in; ++i) {
-        uint64_t xs[kKsCts];
+        const uint64_t x = gl::add(in[i], A.corrector);
+        const uint64_t* kr = krow + (size_t)i * A.digits * W;
+        for (int j = 0; j < A.digits; ++j, kr += W) {
+            const uint32_t u = (uint32_t)((x >> (64 - (j + 1) * A.base_log2)) & mask);
 #pragma unroll
-        for (int c = 0; c < kKsCts; ++c)
-            xs[c] = (ct0 + c < A.count) ? gl::add(in0[(size_t)c * A.in_stride + i], A.corrector) : 0;
-        const uint64_t* kr = A.ksk + (size_t)i * A.digits * W + (col_ok ? col : 0);
-        for (int j = 0; j < A.digits; ++j) {
-            const uint64_t kv = kr[(size_t)j * W];
-            const uint64_t k0 = (uint32_t)kv, k1 = kv >> 32;
-            const int shift = 64 - (j + 1) * A.base_log2;
-#pragma unroll
-            for (int c = 0; c < kKsCts; ++c) {
-                const uint64_t u = (xs[c] >> shift) & mask;
-                lo[c] += u * k0;
-                hi[c] += u * k1;
+            for (int t = 0; t < kKsCols; ++t) {
+                const uint64_t kv = kr[t];
+                lo[t] += (uint64_t)u * (uint32_t)kv;
+                hi[t] += (uint64_t)u * (uint32_t)(kv >> 32);
             }
         }
     }
-    if (!col_ok) return;
-    const uint64_t corr = A.corr[col];
+    if (!live) return;
 #pragma unroll
-    for (int c = 0; c < kKsCts; ++c) {
-        const int ct = ct0 + c;
-        if (ct >= A.count) break;
-        // total = hi * 2^32 + lo, hi < 2^51, lo < 2^51
-        const uint64_t t_lo = (hi[c] << 32) + lo[c];
-        const uint64_t carry = t_lo < lo[c];
-        const uint64_t t_hi = (hi[c] >> 32) + carry;
-        const uint64_t acc = gl::sub(gl::reduce128(t_hi, t_lo), corr);
+    for (int t = 0; t < kKsCols; ++t) {
+        const int col = col0 + t;
+        if (col >= W) break;
+        // total = hi * 2^32 + lo, both < 2^62
+        const uint64_t t_lo = (hi[t] << 32) + lo[t];
+        const uint64_t carry = t_lo < lo[t];
+        const uint64_t t_hi = (hi[t] >> 32) + carry;
+        const uint64_t acc = gl::sub(gl::reduce128(t_hi, t_lo), A.corr[col]);
         uint64_t* o = A.out + (size_t)ct * W;
         if (col < A.n_out) o[col] = gl::neg(acc);
-        else o[col] = gl::sub(A.in[(size_t)ct * A.in_stride + A.n_in], acc);
+        else o[col] = gl::sub(in[A.n_in], acc);
     }
 }
 
@@ -383,8 +382,8 @@ hipError_t launch_blind_rotate(int logN, int method, int digits, const BrArgs& A
 }
 
 hipError_t launch_lwe_keyswitch(const KsArgs& A, hipStream_t s) {
-    dim3 grid((A.n_out + 1 + 63) / 64, (A.count + 4 * kKsCts - 1) / (4 * kKsCts));
-    hipLaunchKernelGGL(lwe_keyswitch_kernel, grid, dim3(256), 0, s, A);
+    dim3 grid((A.n_out + 1 + kKsCols - 1) / kKsCols, (A.count + kKsBlock - 1) / kKsBlock);
+    hipLaunchKernelGGL(lwe_keyswitch_kernel, grid, dim3(kKsBlock), 0, s, A);
     return hipGetLastError();
 }
 

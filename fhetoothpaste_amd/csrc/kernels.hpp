@@ -42,5 +42,6 @@ hipError_t launch_ntt_batch(int logN, uint64_t* out, const uint64_t* in, size_t 
                             bool inverse, hipStream_t s);
 hipError_t launch_blind_rotate(int logN, int method, int digits, const BrArgs& A, int grid, hipStream_t s);
 hipError_t launch_lwe_keyswitch(const KsArgs& A, hipStream_t s);
+constexpr int kKsPadWords = 16;  // zero words after the device key-switching key
 
 }  // namespace tp
