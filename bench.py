@@ -35,6 +35,22 @@ def algorithmic_mulmods():
     return w_br, w_ks
 
 
+def read_traffic(batch):
+    """HBM bytes per blind-rotation launch from the committed profiler summary (None if absent)."""
+    path = os.path.join(ROOT, "profiles", "br_kernel_n630_N1024_batch4096_traffic.txt")
+    if batch != BATCH or not os.path.exists(path):
+        return None
+    vals = {}
+    for line in open(path):
+        if "=" in line and not line.startswith("#"):
+            k, v = line.split("=", 1)
+            vals[k.strip()] = v.strip()
+    try:
+        return int(float(vals["hbm_bytes_per_launch"]))
+    except (KeyError, ValueError):
+        return None
+
+
 def cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -54,6 +70,8 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="issue one device-to-device copy of this size first (profiler byte-counter scale)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this path runs on one GPU; --gpus %d is not supported (sharding is described in "
@@ -101,6 +119,14 @@ def main():
         if mark:
             chk(lib.tp_timer_mark(ctx.h, 2))
 
+    if args.copy_calibration_mb:
+        words = args.copy_calibration_mb * (1 << 20) // 8
+        c_src, c_dst = DeviceBuffer(ctx, words), DeviceBuffer(ctx, words)
+        chk(lib.tp_copy_on_device(ctx.h, c_dst.p, c_src.p, words))
+        ctx.sync()
+        c_src.free()
+        c_dst.free()
+
     for _ in range(args.warmup):
         step()
     ctx.sync()
@@ -141,8 +167,11 @@ def main():
     w_br, w_ks = algorithmic_mulmods()
     br_avg_s = 1e-3 * sum(br_ms) / len(br_ms)
     achieved = (w_br * B) / br_avg_s / 1e12  # Tmulmod/s in the dominant kernel
-    peak = float(os.environ.get("TP_VALU_PEAK_TMULMOD", "2.4576"))
-
+    # INT64 multiply bound of the chip: 256 CUs x 64 lanes x 2.4 GHz of full-rate v_mad_u64_u32,
+    # four per 64x64->128 product
+    peak = 256 * 64 * 2.4e9 / 4 / 1e12
+    traffic = read_traffic(B)
+    key_bytes = (n * 4 * N * L + N * KS_D * (KS_OUT + 1)) * 8
     result = {
         "metric": "CGGI bootstraps/sec at n=630,N=1024 (batched vs plain); p50/p99 latency",
         "value": round(value, 2),
@@ -165,9 +194,10 @@ def main():
                        "plain_mean": round(1000 * sum(plain_lat) / max(len(plain_lat), 1), 3)},
         "kernel_ms": {"blind_rotate_avg": round(1000 * br_avg_s, 3),
                       "keyswitch_avg": round(sum(ks_ms) / len(ks_ms), 3)},
-        "roofline": {"bound": "valu", "achieved": round(achieved, 4), "peak": peak, "unit": "Tmulmod/s",
-                     "frac": round(achieved / peak, 4), "traffic": None,
-                     "kernel": "br_binary_kernel", "work_per_launch_mulmod": w_br * B},
+        "roofline": {"bound": "valu", "achieved": round(achieved, 4), "peak": round(peak, 4), "unit": "Tmulmod/s",
+                     "frac": round(achieved / peak, 4), "traffic": traffic,
+                     "kernel": "br_kernel<10,3,binary>", "work_per_launch_mulmod": w_br * B,
+                     "hbm_key_stream_GBps": round(key_bytes / (br_avg_s + 1e-3 * sum(ks_ms) / len(ks_ms)) / 1e9, 2)},
         "keygen_s": round(keygen_s, 2),
     }
 

@@ -49,6 +49,7 @@ SIGNATURES = {
     "tp_free": [vp, vp],
     "tp_copy_to_device": [vp, vp, u64p, SZ],
     "tp_copy_to_host": [vp, u64p, vp, SZ],
+    "tp_copy_on_device": [vp, vp, vp, SZ],
     "tp_timer_mark": [vp, INT],
     "tp_timer_elapsed": [vp, INT, INT, ctypes.POINTER(ctypes.c_float)],
     "tp_keygen_secret": [u64p, U32, INT, U64, U64],

@@ -41,6 +41,9 @@ def build(verbose=True):
     tmp = OUT + ".tmp"
     subprocess.check_call([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs)
     os.replace(tmp, OUT)
+    probe = os.path.join(HERE, "issue_rate_probe")
+    subprocess.check_call([HIPCC, "--offload-arch=gfx950", "-O3", "-o", probe,
+                           os.path.join(CSRC, "issue_rate_probe.hip")])
     if verbose:
         print("built", OUT)
     return OUT

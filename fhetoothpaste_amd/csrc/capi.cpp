@@ -412,6 +412,14 @@ int tp_copy_to_host(tp_context* c, uint64_t* dst, const uint64_t* src, size_t wo
     return TP_OK;
 }
 
+int tp_copy_on_device(tp_context* c, uint64_t* dst, const uint64_t* src, size_t words) {
+    if (!c || !dst || !src) return fail(TP_E_INVALID, "null argument");
+    int rc = set_device(c);
+    if (rc) return rc;
+    TP_HIP(hipMemcpyAsync(dst, src, words * 8, hipMemcpyDeviceToDevice, c->stream));
+    return TP_OK;
+}
+
 int tp_timer_mark(tp_context* c, int slot) {
     if (!c || slot < 0 || slot >= 8) return fail(TP_E_INVALID, "bad timer slot");
     TP_HIP(hipEventRecord(c->ev[slot], c->stream));

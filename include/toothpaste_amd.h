@@ -119,6 +119,7 @@ int tp_malloc(tp_context* ctx, uint64_t** ptr, size_t words);
 int tp_free(tp_context* ctx, uint64_t* ptr);
 int tp_copy_to_device(tp_context* ctx, uint64_t* dst, const uint64_t* src, size_t words);
 int tp_copy_to_host(tp_context* ctx, uint64_t* dst, const uint64_t* src, size_t words);
+int tp_copy_on_device(tp_context* ctx, uint64_t* dst, const uint64_t* src, size_t words);
 
 /* Elapsed milliseconds between two points of the context stream, for benchmarks:
  * tp_timer_mark(ctx, slot) records an event; tp_timer_elapsed(ctx, a, b, &ms). */
