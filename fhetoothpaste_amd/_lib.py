@@ -28,7 +28,8 @@ class ToothpasteError(RuntimeError):
 
 class Params(ctypes.Structure):
     _fields_ = [("ring_dim", U32), ("lwe_dim", U32), ("br_base_log2", U32), ("br_digits", U32),
-                ("method", U32), ("ks_out_dim", U32), ("ks_base_log2", U32), ("ks_digits", U32)]
+                ("method", U32), ("ks_out_dim", U32), ("ks_base_log2", U32), ("ks_digits", U32),
+                ("rlwe_ks_base_log2", U32), ("rlwe_ks_digits", U32)]
 
 
 # name -> argtypes (all return int status)
@@ -55,6 +56,11 @@ SIGNATURES = {
     "tp_keygen_secret": [u64p, U32, INT, U64, U64],
     "tp_keygen_brk": [u64p, u64p, u64p, U32, U32, U32, U32, INT, INT, ctypes.c_double, U64],
     "tp_keygen_lwe_ksk": [u64p, u64p, u64p, U32, U32, U32, U32, ctypes.c_double, U64],
+    "tp_keygen_auto_keys": [u64p, u64p, U32, U32, U32, ctypes.c_double, U64],
+    "tp_import_auto_keys": [vp, u64p, SZ],
+    "tp_automorphism_device": [vp, SZ, U32, vp, vp],
+    "tp_trace_device": [vp, SZ, vp, vp],
+    "tp_pack_device": [vp, SZ, vp, vp, vp],
 }
 
 _lib = None

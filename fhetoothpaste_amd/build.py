@@ -20,6 +20,7 @@ HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall"]
 
 SOURCES = [
     ("kernels.hip", DEVICE_FLAGS),
+    ("kernels_pack.hip", DEVICE_FLAGS),
     ("capi.cpp", HOST_FLAGS),
     ("keygen.cpp", HOST_FLAGS),
     ("dropin.cpp", HOST_FLAGS),

@@ -56,10 +56,37 @@ struct tp_context {
     uint64_t* d_io = nullptr;  // staging of the host-pointer entry points
     uint64_t* d_mono = nullptr;  // NTT(X^e) table (ternary, BMMP)
     uint64_t br_corrector = 0, br_least = 0;
+    uint64_t* d_auto = nullptr;  // automorphism keys, log2 N x d x 2N, key-row order
+    uint64_t* d_nodes = nullptr; // packing tree nodes
+    size_t nodes_words = 0;
+    uint32_t* d_perm = nullptr;
+    size_t perm_len = 0;
     hipEvent_t ev[8] = {};
 };
 
 namespace {
+
+// Signed-digit corrector and least-digit correction (gadget_decomp.h:16-28).
+uint64_t signed_corrector(uint32_t b, uint64_t* least) {
+    uint64_t C = 0;
+    int64_t shift = 64 - (int64_t)b;
+    while (shift > 0) { C += 1ULL << (shift + b - 1); shift -= b; }
+    *least = 1ULL << (shift + b - 1);
+    C += *least;
+    return C % gl::kQ;
+}
+
+// Reference key rows (N words each, NTT order) -> lane-major device order: NTT index 16 L + k
+// goes to ((k >> 1) * 2P + 2L + (k & 1)).
+std::vector<uint64_t> to_device_rows(const uint64_t* src, size_t words, uint32_t N) {
+    const uint32_t P = N / 16;
+    std::vector<uint64_t> dev(words);
+    for (size_t r = 0; r < words / N; ++r)
+        for (uint32_t L = 0; L < P; ++L)
+            for (uint32_t k = 0; k < 16; ++k)
+                dev[r * N + (k >> 1) * 2 * P + 2 * L + (k & 1)] = src[r * N + 16 * L + k];
+    return dev;
+}
 
 int set_device(tp_context* c) {
     TP_HIP(hipSetDevice(c->device));
@@ -145,6 +172,10 @@ int tp_context_create(tp_context** out, const tp_params* p, int device) {
         p->br_digits > max_digits(p->br_base_log2))
         return fail(TP_E_INVALID, "bad gadget parameters");
     if (p->method == TP_METHOD_BMMP && (p->lwe_dim & 1)) return fail(TP_E_INVALID, "BMMP needs an even lwe_dim");
+    if (p->rlwe_ks_digits) {
+        const uint32_t b = p->rlwe_ks_base_log2, d = p->rlwe_ks_digits;
+        if (b == 0 || b > 32 || d > max_digits(b)) return fail(TP_E_INVALID, "bad automorphism-key gadget parameters");
+    }
     if (p->ks_out_dim) {
         const uint32_t b = p->ks_base_log2, d = p->ks_digits;
         if (b == 0 || b > 16 || d == 0 || d > max_digits(b) || (d == max_digits(b) && 64 % b))
@@ -215,10 +246,12 @@ int tp_context_destroy(tp_context* c) {
     if (!c) return TP_OK;
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
-    for (uint64_t* p : {c->d_tw, c->d_itw, c->d_brk, c->d_ksk, c->d_corr, c->d_scratch, c->d_io, c->d_mono})
+    for (uint64_t* p : {c->d_tw, c->d_itw, c->d_brk, c->d_ksk, c->d_corr, c->d_scratch, c->d_io, c->d_mono,
+                        c->d_auto, c->d_nodes})
         if (p) hipFree(p);
     for (auto& ev : c->ev)
         if (ev) hipEventDestroy(ev);
+    if (c->d_perm) hipFree(c->d_perm);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
     return TP_OK;
@@ -289,6 +322,82 @@ int tp_import_lwe_ksk(tp_context* c, const uint64_t* ksk, size_t words) {
     TP_HIP(hipMemset(c->d_ksk + words, 0, tp::kKsPadWords * 8));
     TP_HIP(hipMemcpy(c->d_corr, corr.data(), W * 8, hipMemcpyHostToDevice));
     c->ksk_words = words;
+    return TP_OK;
+}
+
+namespace {
+tp::KeySwitchArgs auto_args(tp_context* c) {
+    tp::KeySwitchArgs A{};
+    A.tw = c->d_tw;
+    A.itw = c->d_itw;
+    A.base_log2 = (int)c->p.rlwe_ks_base_log2;
+    A.digits = (int)c->p.rlwe_ks_digits;
+    A.max_digits = (int)max_digits(c->p.rlwe_ks_base_log2);
+    A.corrector = signed_corrector(c->p.rlwe_ks_base_log2, &A.least);
+    return A;
+}
+}  // namespace
+
+int tp_import_auto_keys(tp_context* c, const uint64_t* keys, size_t words) {
+    if (!c || !keys) return fail(TP_E_INVALID, "null argument");
+    if (!c->p.rlwe_ks_digits) return fail(TP_E_INVALID, "context has no automorphism gadget (rlwe_ks_digits = 0)");
+    const size_t want = (size_t)c->logN * c->p.rlwe_ks_digits * 2 * c->N;
+    if (words != want) return fail(TP_E_INVALID, "automorphism keys size mismatch: expected " + std::to_string(want));
+    int rc = set_device(c);
+    if (rc) return rc;
+    std::vector<uint64_t> dev = to_device_rows(keys, words, c->N);
+    if (c->d_auto) { hipFree(c->d_auto); c->d_auto = nullptr; }
+    if (hipMalloc(&c->d_auto, words * 8) != hipSuccess) return fail(TP_E_NOMEM, "automorphism key allocation failed");
+    TP_HIP(hipMemcpy(c->d_auto, dev.data(), words * 8, hipMemcpyHostToDevice));
+    return TP_OK;
+}
+
+int tp_automorphism_device(tp_context* c, size_t count, uint32_t t, const uint64_t* in, uint64_t* out) {
+    if (!c || !in || !out) return fail(TP_E_INVALID, "null argument");
+    if (!c->d_auto) return fail(TP_E_STATE, "automorphism keys not imported");
+    int j = 0;
+    while (j <= c->logN && ((1u << j) + 1) != t) ++j;
+    if (j < 1 || j > c->logN) return fail(TP_E_INVALID, "automorphism index must be 2^j + 1, 1 <= j <= log2 N");
+    if (count == 0) return TP_OK;
+    if (count > 0x7fffffff) return fail(TP_E_INVALID, "count too large");
+    int rc = set_device(c);
+    if (rc) return rc;
+    const size_t key_words = (size_t)c->p.rlwe_ks_digits * 2 * c->N;
+    hipError_t e = tp::launch_automorphism(c->logN, auto_args(c), in, out, (int)count, t,
+                                           c->d_auto + (size_t)(j - 1) * key_words, c->stream);
+    if (e != hipSuccess) return fail(TP_E_DEVICE, std::string("automorphism launch: ") + hipGetErrorString(e));
+    return TP_OK;
+}
+
+int tp_trace_device(tp_context* c, size_t count, const uint64_t* in, uint64_t* out) {
+    if (!c || !in || !out) return fail(TP_E_INVALID, "null argument");
+    if (!c->d_auto) return fail(TP_E_STATE, "automorphism keys not imported");
+    if (count == 0) return TP_OK;
+    if (count > 0x7fffffff) return fail(TP_E_INVALID, "count too large");
+    int rc = set_device(c);
+    if (rc) return rc;
+    hipError_t e = tp::launch_trace(c->logN, auto_args(c), in, out, (int)count, c->d_auto, c->stream);
+    if (e != hipSuccess) return fail(TP_E_DEVICE, std::string("trace launch: ") + hipGetErrorString(e));
+    return TP_OK;
+}
+
+int tp_pack_device(tp_context* c, size_t len, const uint64_t* lwes, const uint32_t* perm, uint64_t* out) {
+    if (!c || !lwes || !out) return fail(TP_E_INVALID, "null argument");
+    if (!c->d_auto) return fail(TP_E_STATE, "automorphism keys not imported");
+    if (len < 2 || len > c->N || (len & (len - 1))) return fail(TP_E_INVALID, "len must be a power of two in [2, N]");
+    int rc = set_device(c);
+    if (rc) return rc;
+    const size_t words = len * 2 * c->N;
+    if (c->nodes_words < words) {
+        if (c->d_nodes) TP_HIP(hipFree(c->d_nodes));
+        c->d_nodes = nullptr;
+        c->nodes_words = 0;
+        if (hipMalloc(&c->d_nodes, words * 8) != hipSuccess) return fail(TP_E_NOMEM, "packing buffer allocation failed");
+        c->nodes_words = words;
+    }
+    hipError_t e = tp::launch_pack(c->logN, auto_args(c), lwes, perm, (int)len, c->d_nodes, c->d_auto, c->stream);
+    if (e != hipSuccess) return fail(TP_E_DEVICE, std::string("packing launch: ") + hipGetErrorString(e));
+    TP_HIP(hipMemcpyAsync(out, c->d_nodes, 2 * c->N * 8, hipMemcpyDeviceToDevice, c->stream));
     return TP_OK;
 }
 

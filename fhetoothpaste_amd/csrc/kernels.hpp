@@ -38,6 +38,24 @@ struct KsArgs {
     int n_in, n_out, base_log2, digits, count;
 };
 
+// RLWE key switching with signed digits (automorphism, trace and packing keys).
+struct KeySwitchArgs {
+    const uint64_t* tw;
+    const uint64_t* itw;
+    uint64_t corrector;   // signed-digit corrector for base_log2
+    uint64_t least;
+    int base_log2;
+    int digits;
+    int max_digits;
+};
+
+hipError_t launch_automorphism(int logN, const KeySwitchArgs& A, const uint64_t* in, uint64_t* out, int count,
+                               uint32_t t, const uint64_t* ksk, hipStream_t s);
+hipError_t launch_trace(int logN, const KeySwitchArgs& A, const uint64_t* in, uint64_t* out, int count,
+                        const uint64_t* keys, hipStream_t s);
+hipError_t launch_pack(int logN, const KeySwitchArgs& A, const uint64_t* lwes, const uint32_t* perm, int len,
+                       uint64_t* nodes, const uint64_t* keys, hipStream_t s);
+
 hipError_t launch_ntt_batch(int logN, uint64_t* out, const uint64_t* in, size_t count, const uint64_t* tw,
                             bool inverse, hipStream_t s);
 hipError_t launch_blind_rotate(int logN, int method, int digits, const BrArgs& A, int grid, hipStream_t s);

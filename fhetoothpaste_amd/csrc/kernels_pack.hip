@@ -1,0 +1,333 @@
+// HIP kernels for gfx950: RLWE key switching, automorphisms, trace and n-LWE -> RLWE packing.
+// One RLWE per work-group of P = N/16 lanes (lane L holds coefficients L + P k, k < 16).
+#include <hip/hip_runtime.h>
+#include "gl64.hpp"
+#include "ntt_wave.hpp"
+#include "kernels.hpp"
+
+namespace tp {
+
+namespace {
+
+struct alignas(16) V2 {
+    uint64_t x, y;
+};
+
+// x[k] <- coefficient idx1(L,k) of tau_t(p), p given by the same lanes' registers (coefficients,
+// canonical): out[(i t) mod N] = +-p[i] (utils/math_utils.cpp:70-85), through LDS.
+template <int LOGN>
+__device__ __forceinline__ void automorphism_regs(uint64_t (&x)[kE], uint64_t* lds, uint32_t t, int L) {
+    using S = Shape<LOGN>;
+    constexpr uint32_t N = S::N;
+    poly_sync<S::WAVES>();
+#pragma unroll
+    for (int k = 0; k < kE; ++k) {
+        const uint32_t i = idx1<LOGN>(L, k);
+        const uint32_t e = (i * t) & (2 * N - 1);
+        lds[phys(e & (N - 1))] = e >= N ? gl::neg(x[k]) : x[k];
+    }
+    poly_sync<S::WAVES>();
+#pragma unroll
+    for (int k = 0; k < kE; ++k) x[k] = lds[phys(idx1<LOGN>(L, k))];
+}
+
+// x[k] <- coefficient idx1(L,k) of X^s p (negacyclic), 0 <= s < 2N, through LDS.
+template <int LOGN>
+__device__ __forceinline__ void rotate_regs(uint64_t (&x)[kE], uint64_t* lds, uint32_t s, int L) {
+    using S = Shape<LOGN>;
+    constexpr int N = S::N;
+    poly_sync<S::WAVES>();
+#pragma unroll
+    for (int k = 0; k < kE; ++k) lds[phys(idx1<LOGN>(L, k))] = x[k];
+    poly_sync<S::WAVES>();
+    const bool flip = s >= (uint32_t)N;
+    const int r = (int)(s & (N - 1));
+#pragma unroll
+    for (int k = 0; k < kE; ++k) {
+        int j = idx1<LOGN>(L, k) - r;
+        bool ng = flip;
+        if (j < 0) { j += N; ng = !ng; }
+        const uint64_t v = lds[phys(j)];
+        x[k] = ng ? gl::neg(v) : v;
+    }
+}
+
+// RLWEtoRLWEConverter::Convert core (rlwe_conversion.cpp:224-249): out_p = sum_j
+// NTT(sdigit_j(a)) * KSK_j.p, p = a, b (lazy, NTT domain, pass-3 layout).  a: coefficients,
+// canonical, pass-1 layout.  ksk: d rows of [2N] in key-row order.
+template <int LOGN>
+__device__ __forceinline__ void rlwe_ks_core(uint64_t (&out_a)[kE], uint64_t (&out_b)[kE], const uint64_t (&a)[kE],
+                                             uint64_t* lds, const KeySwitchArgs& A, const uint64_t* ksk, int L) {
+    using S = Shape<LOGN>;
+    constexpr int N = S::N;
+    const uint64_t mask = (1ULL << A.base_log2) - 1;
+#pragma unroll
+    for (int k = 0; k < kE; ++k) out_a[k] = out_b[k] = 0;
+#pragma unroll 1
+    for (int j = 0; j < A.digits; ++j) {
+        const bool last = (j == A.max_digits - 1);
+        const int shift = last ? 0 : 64 - (j + 1) * A.base_log2;
+        const uint64_t dm = last ? (2 * A.least - 1) : mask;
+        const uint64_t delta = last ? A.least : (1ULL << (A.base_log2 - 1));
+        uint64_t d[kE];
+#pragma unroll
+        for (int k = 0; k < kE; ++k) d[k] = gl::sub((gl::add(a[k], A.corrector) >> shift) & dm, delta);
+        ntt_forward<LOGN>(d, lds, A.tw, L);
+        const V2* ka = reinterpret_cast<const V2*>(opaque(ksk + (size_t)j * 2 * N));
+        const V2* kb = reinterpret_cast<const V2*>(opaque(ksk + (size_t)j * 2 * N + N));
+#pragma unroll
+        for (int c = 0; c < kE / 2; ++c) {
+            const V2 va = ka[c * S::P + L], vb = kb[c * S::P + L];
+            out_a[2 * c] = gl::add_ll(out_a[2 * c], gl::mul_lazy(d[2 * c], va.x));
+            out_a[2 * c + 1] = gl::add_ll(out_a[2 * c + 1], gl::mul_lazy(d[2 * c + 1], va.y));
+            out_b[2 * c] = gl::add_ll(out_b[2 * c], gl::mul_lazy(d[2 * c], vb.x));
+            out_b[2 * c + 1] = gl::add_ll(out_b[2 * c + 1], gl::mul_lazy(d[2 * c + 1], vb.y));
+        }
+    }
+}
+
+// AutomorphismEvaluator::Eval on registers (automorphism_evaluation.cpp:150-159):
+// [a|b] coefficients -> [a'|b'] NTT domain (canonical, pass-3 layout) with
+// a' = -sum NTT(sdigit(tau(a))) K.a,  b' = NTT(tau(b)) - sum NTT(sdigit(tau(a))) K.b.
+template <int LOGN>
+__device__ __forceinline__ void automorphism_eval(uint64_t (&a)[kE], uint64_t (&b)[kE], uint64_t* lds, uint32_t t,
+                                                  const KeySwitchArgs& A, const uint64_t* ksk, int L) {
+    automorphism_regs<LOGN>(a, lds, t, L);
+    automorphism_regs<LOGN>(b, lds, t, L);
+    ntt_forward<LOGN>(b, lds, A.tw, L);
+    uint64_t oa[kE], ob[kE];
+    rlwe_ks_core<LOGN>(oa, ob, a, lds, A, ksk, L);
+#pragma unroll
+    for (int k = 0; k < kE; ++k) {
+        a[k] = gl::neg(gl::canon(oa[k]));
+        b[k] = gl::sub(gl::canon(b[k]), gl::canon(ob[k]));
+    }
+}
+
+}  // namespace
+
+// Batched automorphism: in [count][2N] coefficients, out [count][2N] NTT domain.
+template <int LOGN>
+__global__ void __launch_bounds__(Shape<LOGN>::P) automorphism_kernel(KeySwitchArgs A, const uint64_t* in,
+                                                                      uint64_t* out, int count, uint32_t t,
+                                                                      const uint64_t* ksk) {
+    using S = Shape<LOGN>;
+    constexpr int N = S::N;
+    __shared__ uint64_t lds[S::LDS_WORDS];
+    const int L = threadIdx.x;
+    for (int ct = blockIdx.x; ct < count; ct += gridDim.x) {
+        uint64_t a[kE], b[kE];
+        const uint64_t* src = in + (size_t)ct * 2 * N;
+#pragma unroll
+        for (int k = 0; k < kE; ++k) {
+            a[k] = src[idx1<LOGN>(L, k)];
+            b[k] = src[N + idx1<LOGN>(L, k)];
+        }
+        automorphism_eval<LOGN>(a, b, lds, t, A, ksk, L);
+        uint64_t* dst = out + (size_t)ct * 2 * N;
+#pragma unroll
+        for (int k = 0; k < kE; ++k) {
+            dst[idx3(L, k)] = a[k];
+            dst[N + idx3(L, k)] = b[k];
+        }
+        poly_sync<S::WAVES>();
+    }
+}
+
+// Batched trace (trace_evaluation.cpp:135-162): o = in / N; for t = N+1, N/2+1, ..., 3:
+// o += INTT(Auto_t(o)); out = NTT(o).  in coefficients, out NTT domain.  keys: log2 N key blocks,
+// block j-1 for t = 2^j + 1.
+template <int LOGN>
+__global__ void __launch_bounds__(Shape<LOGN>::P) trace_kernel(KeySwitchArgs A, const uint64_t* in, uint64_t* out,
+                                                               int count, const uint64_t* keys) {
+    using S = Shape<LOGN>;
+    constexpr int N = S::N;
+    __shared__ uint64_t lds[S::LDS_WORDS];
+    const int L = threadIdx.x;
+    const size_t key_words = (size_t)A.digits * 2 * N;
+    for (int ct = blockIdx.x; ct < count; ct += gridDim.x) {
+        uint64_t o[2][kE];
+        const uint64_t* src = in + (size_t)ct * 2 * N;
+#pragma unroll
+        for (int k = 0; k < kE; ++k) {
+            o[0][k] = gl::div_pow2(src[idx1<LOGN>(L, k)], LOGN);
+            o[1][k] = gl::div_pow2(src[N + idx1<LOGN>(L, k)], LOGN);
+        }
+#pragma unroll 1
+        for (int j = LOGN; j >= 1; --j) {
+            uint64_t a[kE], b[kE];
+#pragma unroll
+            for (int k = 0; k < kE; ++k) {
+                a[k] = o[0][k];
+                b[k] = o[1][k];
+            }
+            automorphism_eval<LOGN>(a, b, lds, (1u << j) + 1, A, keys + (size_t)(j - 1) * key_words, L);
+            ntt_inverse<LOGN>(a, lds, A.itw, L);
+            ntt_inverse<LOGN>(b, lds, A.itw, L);
+#pragma unroll
+            for (int k = 0; k < kE; ++k) {
+                o[0][k] = gl::canon(gl::add_lc(a[k], o[0][k]));
+                o[1][k] = gl::canon(gl::add_lc(b[k], o[1][k]));
+            }
+        }
+        uint64_t* dst = out + (size_t)ct * 2 * N;
+#pragma unroll 1
+        for (int h = 0; h < 2; ++h) {
+            uint64_t y[kE];
+#pragma unroll
+            for (int k = 0; k < kE; ++k) y[k] = h ? o[1][k] : o[0][k];
+            ntt_forward<LOGN>(y, lds, A.tw, L);
+#pragma unroll
+            for (int k = 0; k < kE; ++k) dst[h * N + idx3(L, k)] = gl::canon(y[k]);
+        }
+        poly_sync<S::WAVES>();
+    }
+}
+
+// Packing leaves (lwe_to_rlwe_packing.cpp:238-251): node k = L0Setup(lwes[perm((len-1) ^ brv(k))]):
+// a_0 = lwe_0, a_i = -lwe_(N-i), b_0 = lwe_N, all scaled by N^-1, other b coefficients 0.
+template <int LOGN>
+__global__ void __launch_bounds__(Shape<LOGN>::P) pack_leaf_kernel(const uint64_t* lwes, const uint32_t* perm,
+                                                                   uint64_t* nodes, int len, int log_len) {
+    using S = Shape<LOGN>;
+    constexpr int N = S::N;
+    const int L = threadIdx.x;
+    const uint64_t ninv = gl::inv(N);
+    for (int leaf = blockIdx.x; leaf < len; leaf += gridDim.x) {
+        uint32_t r = 0, x = (uint32_t)leaf;
+        for (int b = 0; b < log_len; ++b) { r = (r << 1) | (x & 1); x >>= 1; }
+        uint32_t src = (uint32_t)(len - 1) ^ r;
+        if (perm) src = perm[src];
+        const uint64_t* lwe = lwes + (size_t)src * (N + 1);
+        uint64_t* dst = nodes + (size_t)leaf * 2 * N;
+#pragma unroll
+        for (int k = 0; k < kE; ++k) {
+            const int i = idx1<LOGN>(L, k);
+            const uint64_t a = i == 0 ? lwe[0] : gl::neg(lwe[N - i]);
+            dst[i] = gl::mul(a, ninv);
+            dst[N + i] = i == 0 ? gl::mul(lwe[N], ninv) : 0;
+        }
+    }
+}
+
+// One level of the packing tree: node j of level lvl = (R + X^s L) + tau_(2^lvl+1)(R - X^s L),
+// s = N >> lvl, with L, R the level lvl-1 nodes 2j, 2j+1 (stored at leaf slots 2j 2^(lvl-1),
+// (2j+1) 2^(lvl-1)); the result goes to the left slot (lwe_to_rlwe_packing.cpp:170-224).
+template <int LOGN>
+__global__ void __launch_bounds__(Shape<LOGN>::P) pack_merge_kernel(KeySwitchArgs A, uint64_t* nodes, int merges,
+                                                                    int lvl, const uint64_t* ksk) {
+    using S = Shape<LOGN>;
+    constexpr int N = S::N;
+    __shared__ uint64_t lds[S::LDS_WORDS];
+    const int L = threadIdx.x;
+    const uint32_t shift = (uint32_t)N >> lvl;
+    for (int j = blockIdx.x; j < merges; j += gridDim.x) {
+        uint64_t* left = nodes + ((size_t)(2 * j) << (lvl - 1)) * 2 * N;
+        const uint64_t* right = nodes + ((size_t)(2 * j + 1) << (lvl - 1)) * 2 * N;
+        uint64_t p[2][kE], m[2][kE];
+#pragma unroll 1
+        for (int h = 0; h < 2; ++h) {
+            uint64_t x[kE];
+#pragma unroll
+            for (int k = 0; k < kE; ++k) x[k] = left[h * N + idx1<LOGN>(L, k)];
+            rotate_regs<LOGN>(x, lds, shift, L);
+#pragma unroll
+            for (int k = 0; k < kE; ++k) {
+                const uint64_t r = right[h * N + idx1<LOGN>(L, k)];
+                const uint64_t s = gl::add(r, x[k]), d = gl::sub(r, x[k]);
+                if (h) { p[1][k] = s; m[1][k] = d; } else { p[0][k] = s; m[0][k] = d; }
+            }
+        }
+        automorphism_eval<LOGN>(m[0], m[1], lds, (1u << lvl) + 1, A, ksk, L);
+        ntt_inverse<LOGN>(m[0], lds, A.itw, L);
+        ntt_inverse<LOGN>(m[1], lds, A.itw, L);
+        poly_sync<S::WAVES>();
+#pragma unroll
+        for (int k = 0; k < kE; ++k) {
+            left[idx1<LOGN>(L, k)] = gl::canon(gl::add_lc(m[0][k], p[0][k]));
+            left[N + idx1<LOGN>(L, k)] = gl::canon(gl::add_lc(m[1][k], p[1][k]));
+        }
+    }
+}
+
+// Trace steps after the tree when len < N: o += INTT(Auto_(2^lvl+1)(o)), lvl = logN .. log len + 1.
+template <int LOGN>
+__global__ void __launch_bounds__(Shape<LOGN>::P) pack_tail_kernel(KeySwitchArgs A, uint64_t* node, int lvl_stop,
+                                                                   const uint64_t* keys) {
+    using S = Shape<LOGN>;
+    constexpr int N = S::N;
+    __shared__ uint64_t lds[S::LDS_WORDS];
+    const int L = threadIdx.x;
+    const size_t key_words = (size_t)A.digits * 2 * N;
+    uint64_t o[2][kE];
+#pragma unroll
+    for (int k = 0; k < kE; ++k) {
+        o[0][k] = node[idx1<LOGN>(L, k)];
+        o[1][k] = node[N + idx1<LOGN>(L, k)];
+    }
+#pragma unroll 1
+    for (int lvl = LOGN; lvl > lvl_stop; --lvl) {
+        uint64_t a[kE], b[kE];
+#pragma unroll
+        for (int k = 0; k < kE; ++k) {
+            a[k] = o[0][k];
+            b[k] = o[1][k];
+        }
+        automorphism_eval<LOGN>(a, b, lds, (1u << lvl) + 1, A, keys + (size_t)(lvl - 1) * key_words, L);
+        ntt_inverse<LOGN>(a, lds, A.itw, L);
+        ntt_inverse<LOGN>(b, lds, A.itw, L);
+#pragma unroll
+        for (int k = 0; k < kE; ++k) {
+            o[0][k] = gl::canon(gl::add_lc(a[k], o[0][k]));
+            o[1][k] = gl::canon(gl::add_lc(b[k], o[1][k]));
+        }
+    }
+    poly_sync<S::WAVES>();
+#pragma unroll
+    for (int k = 0; k < kE; ++k) {
+        node[idx1<LOGN>(L, k)] = o[0][k];
+        node[N + idx1<LOGN>(L, k)] = o[1][k];
+    }
+}
+
+// ------------------------------------------------------------------ launchers
+
+#define TP_LOGN_SWITCH(LOGN_VAR, CALL)                    \
+    switch (LOGN_VAR) {                                   \
+        case 10: { constexpr int LG = 10; CALL; break; }  \
+        case 11: { constexpr int LG = 11; CALL; break; }  \
+        case 12: { constexpr int LG = 12; CALL; break; }  \
+        default: return hipErrorInvalidValue;             \
+    }
+
+hipError_t launch_automorphism(int logN, const KeySwitchArgs& A, const uint64_t* in, uint64_t* out, int count,
+                               uint32_t t, const uint64_t* ksk, hipStream_t s) {
+    TP_LOGN_SWITCH(logN, hipLaunchKernelGGL(automorphism_kernel<LG>, dim3(count), dim3(Shape<LG>::P), 0, s, A, in,
+                                            out, count, t, ksk));
+    return hipGetLastError();
+}
+
+hipError_t launch_trace(int logN, const KeySwitchArgs& A, const uint64_t* in, uint64_t* out, int count,
+                        const uint64_t* keys, hipStream_t s) {
+    TP_LOGN_SWITCH(logN, hipLaunchKernelGGL(trace_kernel<LG>, dim3(count), dim3(Shape<LG>::P), 0, s, A, in, out,
+                                            count, keys));
+    return hipGetLastError();
+}
+
+hipError_t launch_pack(int logN, const KeySwitchArgs& A, const uint64_t* lwes, const uint32_t* perm, int len,
+                       uint64_t* nodes, const uint64_t* keys, hipStream_t s) {
+    int log_len = 0;
+    while ((1 << log_len) < len) ++log_len;
+    const size_t key_words = (size_t)A.digits * 2 * ((size_t)1 << logN);
+    TP_LOGN_SWITCH(logN, {
+        hipLaunchKernelGGL(pack_leaf_kernel<LG>, dim3(len), dim3(Shape<LG>::P), 0, s, lwes, perm, nodes, len, log_len);
+        for (int lvl = 1; lvl <= log_len; ++lvl)
+            hipLaunchKernelGGL(pack_merge_kernel<LG>, dim3(len >> lvl), dim3(Shape<LG>::P), 0, s, A, nodes,
+                               len >> lvl, lvl, keys + (size_t)(lvl - 1) * key_words);
+        if (log_len < logN)
+            hipLaunchKernelGGL(pack_tail_kernel<LG>, dim3(1), dim3(Shape<LG>::P), 0, s, A, nodes, log_len, keys);
+    });
+    return hipGetLastError();
+}
+
+}  // namespace tp

@@ -171,3 +171,32 @@ extern "C" int tp_keygen_lwe_ksk(uint64_t* ksk, const uint64_t* src, const uint6
     }
     return TP_OK;
 }
+
+// Automorphism keys (automorphism_evaluation.cpp:43-63 with rlwe_conversion.cpp:187-218): key j-1
+// is RLWEtoRLWEConverter's key from s(X^t) to s(X), t = 2^j + 1; row i = RLWE(g_i s(X^t)),
+// g_i = 2^(64 - b(i+1)) (kept >= 1).  One stream (tag 5) over j = 1..log2 N in order.
+extern "C" int tp_keygen_auto_keys(uint64_t* keys, const uint64_t* s, uint32_t N, uint32_t b, uint32_t d,
+                                   double sigma, uint64_t seed) {
+    if (!keys || !s || b == 0 || b > 32 || d == 0 || N < 16 || (N & (N - 1))) return TP_E_INVALID;
+    tp::HostNtt T(N);
+    Rng g(seed, 5);
+    Gauss G(sigma);
+    std::vector<uint64_t> t_ntt(s, s + N), s_auto(N), msg(N), e(N);
+    T.forward(t_ntt.data());
+    for (uint32_t j = 1; j <= T.logN; ++j) {
+        const uint64_t t = (1ULL << j) + 1;
+        for (uint64_t i = 0; i < N; ++i) {
+            const uint64_t p = (i * t) & (2 * (uint64_t)N - 1);
+            s_auto[p & (N - 1)] = p >= N ? gl::neg(s[i]) : s[i];
+        }
+        uint64_t gij = 1ULL << (64 - b);
+        for (uint32_t i = 0; i < d; ++i) {
+            for (uint32_t k = 0; k < N; ++k) msg[k] = gl::mul(s_auto[k], gij);
+            T.forward(msg.data());
+            make_rlwe(T, keys + ((size_t)(j - 1) * d + i) * 2 * N, msg.data(), t_ntt.data(), g, G, e.data());
+            gij >>= b;
+            if (gij == 0) gij = 1;
+        }
+    }
+    return TP_OK;
+}

@@ -62,8 +62,9 @@ class Context:
     """One device context (stream, tables, resident keys) for one parameter set."""
 
     def __init__(self, ring_dim, lwe_dim, base_log2, digits, method=T.METHOD_CGGI_BINARY, ks_out_dim=0,
-                 ks_base_log2=0, ks_digits=0, device=0):
-        self.params = T.Params(ring_dim, lwe_dim, base_log2, digits, method, ks_out_dim, ks_base_log2, ks_digits)
+                 ks_base_log2=0, ks_digits=0, device=0, auto_base_log2=0, auto_digits=0):
+        self.params = T.Params(ring_dim, lwe_dim, base_log2, digits, method, ks_out_dim, ks_base_log2, ks_digits,
+                               auto_base_log2, auto_digits)
         h = ctypes.c_void_p()
         check(lib().tp_context_create(ctypes.byref(h), ctypes.byref(self.params), device))
         self.h = h.value
@@ -138,6 +139,38 @@ class Context:
         check(lib().tp_lwe_keyswitch_batch_device(self.h, count, d_in.p, self.N + 1, d_out.p))
         return d_out.to_host().reshape(count, W)
 
+    def import_auto_keys(self, keys):
+        keys = np.ascontiguousarray(keys, dtype=np.uint64)
+        check(lib().tp_import_auto_keys(self.h, ptr(keys), keys.size))
+
+    def automorphism(self, rlwes_coef, t):
+        """Batched AutomorphismEvaluator::Eval: [count, 2N] coefficients -> [count, 2N] NTT domain."""
+        x = np.ascontiguousarray(rlwes_coef, dtype=np.uint64).reshape(-1, 2 * self.N)
+        d_in = DeviceBuffer.from_host(self, x)
+        d_out = DeviceBuffer(self, x.size)
+        check(lib().tp_automorphism_device(self.h, x.shape[0], t, d_in.p, d_out.p))
+        return d_out.to_host().reshape(x.shape)
+
+    def trace(self, rlwes_coef):
+        x = np.ascontiguousarray(rlwes_coef, dtype=np.uint64).reshape(-1, 2 * self.N)
+        d_in = DeviceBuffer.from_host(self, x)
+        d_out = DeviceBuffer(self, x.size)
+        check(lib().tp_trace_device(self.h, x.shape[0], d_in.p, d_out.p))
+        return d_out.to_host().reshape(x.shape)
+
+    def pack(self, lwes, perm=None):
+        """LWEtoRLWEPacker::Pack of [len, N+1] LWEs (optionally reordered by perm) -> [2N] coefficients."""
+        x = np.ascontiguousarray(lwes, dtype=np.uint64).reshape(-1, self.N + 1)
+        d_in = DeviceBuffer.from_host(self, x)
+        d_out = DeviceBuffer(self, 2 * self.N)
+        d_perm = None
+        if perm is not None:
+            p32 = np.ascontiguousarray(perm, dtype=np.uint32)
+            d_perm = DeviceBuffer.from_host(self, np.frombuffer(
+                np.concatenate([p32, np.zeros(p32.size % 2, np.uint32)]).tobytes(), dtype=np.uint64))
+        check(lib().tp_pack_device(self.h, x.shape[0], d_in.p, d_perm.p if d_perm else None, d_out.p))
+        return d_out.to_host()
+
     def bootstrap_batch(self, lwes, tv):
         lwes = np.ascontiguousarray(lwes, dtype=np.uint64).reshape(-1, self.n + 1)
         tv = np.ascontiguousarray(tv, dtype=np.uint64)
@@ -165,6 +198,14 @@ def keygen_brk(lwe_key, rlwe_key, N, n, base_log2, digits, method, gadget_kind, 
     check(lib().tp_keygen_brk(ptr(out), ptr(np.ascontiguousarray(lwe_key, np.uint64)),
                               ptr(np.ascontiguousarray(rlwe_key, np.uint64)), N, n, base_log2, digits, method,
                               gadget_kind, sigma, seed))
+    return out
+
+
+def keygen_auto_keys(s, N, base_log2, digits, sigma, seed):
+    logN = N.bit_length() - 1
+    out = np.empty(logN * digits * 2 * N, dtype=np.uint64)
+    check(lib().tp_keygen_auto_keys(ptr(out), ptr(np.ascontiguousarray(s, np.uint64)), N, base_log2, digits, sigma,
+                                    seed))
     return out
 
 

@@ -54,6 +54,8 @@ typedef struct tp_params {
     uint32_t ks_out_dim;    /* LWE key switch N -> ks_out_dim; 0 = no key switch */
     uint32_t ks_base_log2;
     uint32_t ks_digits;
+    uint32_t rlwe_ks_base_log2;  /* automorphism / trace / packing keys (RLWEtoRLWEConverter gadget) */
+    uint32_t rlwe_ks_digits;     /* 0 = no automorphism keys */
 } tp_params;
 
 typedef struct tp_context tp_context;
@@ -79,6 +81,28 @@ int tp_import_brk(tp_context* ctx, const uint64_t* brk, size_t words);
 
 /* Import the LWE->LWE key-switching key [N][d][n_out+1] (lwe_conversion.cpp:174-203). */
 int tp_import_lwe_ksk(tp_context* ctx, const uint64_t* ksk, size_t words);
+
+/* Import the automorphism keys for t = 2^j + 1, j = 1..log2 N, in the layout the reference
+ * builds them (AutomorphismContext::ConstructOperator, automorphism_evaluation.cpp:43-63, one
+ * RLWEtoRLWEConverter key each, rlwe_conversion.cpp:187-218): [log2 N][d][2N], NTT domain,
+ * key j-1 switching from s(X^t) to s(X).  These are the keys TraceEvaluationContext and
+ * LWEtoRLWEPackingContext construct (trace_evaluation.cpp:50-62, lwe_to_rlwe_packing.cpp:46-58). */
+int tp_import_auto_keys(tp_context* ctx, const uint64_t* keys, size_t words);
+
+/* Batched AutomorphismEvaluator::Eval (automorphism_evaluation.cpp:150-159): in [count][2N]
+ * coefficients [a|b], out [count][2N] NTT domain, t = 2^j + 1 (1 <= j <= log2 N).  Overwrite
+ * semantics (the reference subtracts into a zeroed output). */
+int tp_automorphism_device(tp_context* ctx, size_t count, uint32_t t, const uint64_t* in, uint64_t* out);
+
+/* Batched TraceEvaluator::Eval (trace_evaluation.cpp:135-162): in [count][2N] coefficients,
+ * out [count][2N] NTT domain. */
+int tp_trace_device(tp_context* ctx, size_t count, const uint64_t* in, uint64_t* out);
+
+/* LWEtoRLWEPacker::Pack (lwe_to_rlwe_packing.cpp:129-236): len LWEs of dimension N
+ * ([len][N+1], mod Q; len a power of two, 2 <= len <= N) -> one RLWE in coefficients, out [2N].
+ * perm (may be NULL) reorders the inputs: input i is lwes[perm[i]].  Uses len*2N words of
+ * context scratch; the tree levels are launched in order on the context stream. */
+int tp_pack_device(tp_context* ctx, size_t len, const uint64_t* lwes, const uint32_t* perm, uint64_t* out);
 
 /* Negacyclic NTT over Z_Q[X]/(X^N+1) of `count` polynomials (device pointers).
  * Replace SolinasWorker::ForwardNTT / BackwardNTT (lib/src/backend/solinas.cpp:228-313);
@@ -138,6 +162,10 @@ int tp_keygen_brk(uint64_t* brk, const uint64_t* lwe_key, const uint64_t* rlwe_k
                   uint32_t base_log2, uint32_t digits, int method, int gadget_kind, double sigma, uint64_t seed);
 int tp_keygen_lwe_ksk(uint64_t* ksk, const uint64_t* src, const uint64_t* dst, uint32_t n_in, uint32_t n_out,
                       uint32_t base_log2, uint32_t digits, double sigma, uint64_t seed);
+/* Automorphism keys for t = 2^j + 1, j = 1..log2 N, under RLWE key s (coefficients):
+ * keys [log2 N][d][2N]. */
+int tp_keygen_auto_keys(uint64_t* keys, const uint64_t* s, uint32_t N, uint32_t base_log2, uint32_t digits,
+                        double sigma, uint64_t seed);
 
 #ifdef __cplusplus
 }

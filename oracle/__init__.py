@@ -66,6 +66,13 @@ def lib():
             "or_gauss_samples": (None, [u64p, ctypes.c_size_t, ctypes.c_double, U64, U64]),
             "or_phase_rlwe": (None, [ctypes.c_void_p, u64p, u64p, u64p]),
             "or_phase_lwe": (U64, [u64p, u64p, U32]),
+            "or_keygen_rlwe_ksk": (None, [ctypes.c_void_p, u64p, u64p, u64p, U32, U32, ctypes.c_double, U64, U64]),
+            "or_automorphism_poly": (None, [u64p, u64p, U64, U32]),
+            "or_keygen_auto_keys": (None, [ctypes.c_void_p, u64p, u64p, U32, U32, ctypes.c_double, U64]),
+            "or_rlwe_keyswitch": (None, [ctypes.c_void_p, u64p, u64p, u64p, U32, U32]),
+            "or_automorphism": (None, [ctypes.c_void_p, u64p, u64p, U64, u64p, U32, U32]),
+            "or_trace": (None, [ctypes.c_void_p, u64p, u64p, u64p, U32, U32]),
+            "or_pack": (ctypes.c_int, [ctypes.c_void_p, u64p, u64p, U32, u64p, U32, U32]),
             "or_bootstrap_batch": (ctypes.c_double, [ctypes.c_void_p, u64p, u64p, ctypes.c_size_t, u64p, u64p,
                                                      U32, U32, U32, ctypes.c_int, u64p, U32, U32, U32,
                                                      ctypes.c_int]),
@@ -213,3 +220,48 @@ def bootstrap_batch(N, lwes, tv, brk_bytes, n, b, l, method, ksk=None, n_out=0, 
     secs = lib().or_bootstrap_batch(Ring(N).h, P(out), P(lwes), count, P(tv), P(brk_bytes), n, b, l, method,
                                     P(ksk), n_out, ks_b, ks_d, threads)
     return out.reshape(count, width), secs
+
+
+def auto_keys(N, s, b, d, sigma, seed):
+    """Automorphism keys for t = 2^j + 1, j = 1..log2 N: [log2 N][d][2N], NTT domain."""
+    logN = N.bit_length() - 1
+    out = arr(logN * d * 2 * N)
+    lib().or_keygen_auto_keys(Ring(N).h, P(out), P(s), b, d, sigma, seed)
+    return out
+
+
+def rlwe_ksk(N, src, dst, b, d, sigma, seed, tag=6):
+    out = arr(d * 2 * N)
+    lib().or_keygen_rlwe_ksk(Ring(N).h, P(out), P(src), P(dst), b, d, sigma, seed, tag)
+    return out
+
+
+def automorphism_poly(a, t, N):
+    out = arr(N)
+    lib().or_automorphism_poly(P(out), P(np.ascontiguousarray(a, np.uint64)), t, N)
+    return out
+
+
+def automorphism(N, rlwe_coef, t, ksk, b, d):
+    out = arr(2 * N)
+    lib().or_automorphism(Ring(N).h, P(out), P(np.ascontiguousarray(rlwe_coef, np.uint64)), t, P(ksk), b, d)
+    return out
+
+
+def rlwe_keyswitch(N, out, inp, ksk, b, d):
+    out = np.ascontiguousarray(out, np.uint64).copy()
+    lib().or_rlwe_keyswitch(Ring(N).h, P(out), P(np.ascontiguousarray(inp, np.uint64)), P(ksk), b, d)
+    return out
+
+
+def trace(N, rlwe_coef, keys, b, d):
+    out = arr(2 * N)
+    lib().or_trace(Ring(N).h, P(out), P(np.ascontiguousarray(rlwe_coef, np.uint64)), P(keys), b, d)
+    return out
+
+
+def pack(N, lwes, length, keys, b, d):
+    out = arr(2 * N)
+    rc = lib().or_pack(Ring(N).h, P(out), P(np.ascontiguousarray(lwes, np.uint64)), length, P(keys), b, d)
+    assert rc == 0
+    return out

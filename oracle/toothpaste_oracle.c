@@ -730,6 +730,180 @@ OR_EXPORT uint64_t or_phase_lwe(const uint64_t* lwe, const uint64_t* s, uint32_t
 }
 
 /* ------------------------------------------------------------------------ */
+/* RLWE key switching, automorphism, trace, packing                         */
+/* ------------------------------------------------------------------------ */
+
+/* RLWEtoRLWEConverter key (rlwe_conversion.cpp:187-218): d rows [2N], row i = RLWE_t(g_i s_src),
+ * g_i = 2^(64 - b(i+1)) (kept >= 1), message in coefficients, NTT-domain output. */
+static void keygen_rlwe_ksk_rng(const or_ring* R, uint64_t* ksk, const uint64_t* src, const uint64_t* t_ntt,
+                                uint32_t b, uint32_t d, or_rng* g, const or_gauss* G) {
+    const uint32_t N = R->N;
+    uint64_t* ws = (uint64_t*)malloc(sizeof(uint64_t) * 2 * N);
+    uint64_t gij = 1ULL << (64 - b);
+    for (uint32_t i = 0; i < d; ++i) {
+        for (uint32_t k = 0; k < N; ++k) ws[k] = or_mul(src[k], gij);
+        ntt_fwd(R, ws);                       /* NTT(e + m) = NTT(e) + NTT(m) */
+        make_rlwe(R, ksk + (size_t)i * 2 * N, ws, t_ntt, g, G, ws + N);
+        gij >>= b;
+        if (gij == 0) gij = 1;
+    }
+    free(ws);
+}
+
+OR_EXPORT void or_keygen_rlwe_ksk(const or_ring* R, uint64_t* ksk, const uint64_t* src, const uint64_t* dst,
+                                  uint32_t b, uint32_t d, double sigma, uint64_t seed, uint64_t tag) {
+    or_rng g = rng_make(seed, tag);
+    or_gauss G;
+    gauss_init(&G, sigma);
+    uint64_t* t = (uint64_t*)malloc(sizeof(uint64_t) * R->N);
+    memcpy(t, dst, sizeof(uint64_t) * R->N);
+    ntt_fwd(R, t);
+    keygen_rlwe_ksk_rng(R, ksk, src, t, b, d, &g, &G);
+    free(t);
+}
+
+/* EvalNegacyclicAutomorphism (utils/math_utils.cpp:70-85): out[(i t) mod N] = +-in[i],
+ * negated when (i t mod 2N) >= N. */
+OR_EXPORT void or_automorphism_poly(uint64_t* out, const uint64_t* in, uint64_t t, uint32_t N) {
+    for (uint64_t i = 0; i < N; ++i) {
+        const uint64_t p = (i * t) & (2 * (uint64_t)N - 1);
+        out[p & (N - 1)] = p >= N ? neg(in[i]) : in[i];
+    }
+}
+
+/* Automorphism keys for t = 2^j + 1, j = 1..log2 N (trace_evaluation.cpp:50-62): source s(X^t),
+ * target s(X), one seeded stream, keys in order of j.  keys: log2(N) x d x 2N. */
+OR_EXPORT void or_keygen_auto_keys(const or_ring* R, uint64_t* keys, const uint64_t* s, uint32_t b, uint32_t d,
+                                   double sigma, uint64_t seed) {
+    const uint32_t N = R->N;
+    or_rng g = rng_make(seed, 5);
+    or_gauss G;
+    gauss_init(&G, sigma);
+    uint64_t* t_ntt = (uint64_t*)malloc(sizeof(uint64_t) * 2 * N);
+    uint64_t* s_auto = t_ntt + N;
+    memcpy(t_ntt, s, sizeof(uint64_t) * N);
+    ntt_fwd(R, t_ntt);
+    for (uint32_t j = 1; j <= R->logN; ++j) {
+        or_automorphism_poly(s_auto, s, (1ULL << j) + 1, N);
+        keygen_rlwe_ksk_rng(R, keys + (size_t)(j - 1) * d * 2 * N, s_auto, t_ntt, b, d, &g, &G);
+    }
+    free(t_ntt);
+}
+
+/* RLWEtoRLWEConverter::Convert (rlwe_conversion.cpp:224-249): in = [a (coefficients) | b (NTT)];
+ * out -= sum_i NTT(sdigit_i(a)) * KSK_i (both halves); out_b += b. */
+OR_EXPORT void or_rlwe_keyswitch(const or_ring* R, uint64_t* out, const uint64_t* in, const uint64_t* ksk,
+                                 uint32_t b, uint32_t d) {
+    const uint32_t N = R->N;
+    uint64_t* dig = (uint64_t*)malloc(sizeof(uint64_t) * 2 * N * d + sizeof(uint64_t) * 2 * N);
+    uint64_t* acc = dig + (size_t)2 * N * d;
+    signed_rep2_ntt(R, dig, in, b, d);
+    vmul(dig, dig, ksk, 2 * N * d);
+    memcpy(acc, dig, sizeof(uint64_t) * 2 * N);
+    for (uint32_t i = 1; i < d; ++i) vadd(acc, acc, dig + (size_t)i * 2 * N, 2 * N);
+    vsub(out, out, acc, 2 * N);
+    vadd(out + N, out + N, in + N, N);
+    free(dig);
+}
+
+/* AutomorphismEvaluator::Eval (automorphism_evaluation.cpp:150-159, 183-207): input [a|b] in
+ * coefficients; a' = tau_t(a), b' = NTT(tau_t(b)); out (zeroed here) = Convert([a'|b']). */
+OR_EXPORT void or_automorphism(const or_ring* R, uint64_t* out, const uint64_t* in, uint64_t t, const uint64_t* ksk,
+                               uint32_t b, uint32_t d) {
+    const uint32_t N = R->N;
+    uint64_t* buf = (uint64_t*)malloc(sizeof(uint64_t) * 2 * N);
+    or_automorphism_poly(buf, in, t, N);
+    or_automorphism_poly(buf + N, in + N, t, N);
+    ntt_fwd(R, buf + N);
+    memset(out, 0, sizeof(uint64_t) * 2 * N);
+    or_rlwe_keyswitch(R, out, buf, ksk, b, d);
+    free(buf);
+}
+
+/* TraceEvaluator::Eval (trace_evaluation.cpp:135-162): in coefficients, out NTT:
+ * o = in / N; for t = N+1, N/2+1, ..., 3: o += INTT(Auto_t(o)); out = NTT(o). */
+OR_EXPORT void or_trace(const or_ring* R, uint64_t* out, const uint64_t* in, const uint64_t* keys, uint32_t b,
+                        uint32_t d) {
+    const uint32_t N = R->N;
+    uint64_t* buf = (uint64_t*)malloc(sizeof(uint64_t) * 2 * N);
+    for (uint32_t i = 0; i < 2 * N; ++i) out[i] = or_mul(in[i], R->n_inv);
+    for (int j = (int)R->logN; j >= 1; --j) {
+        or_automorphism(R, buf, out, (1ULL << j) + 1, keys + (size_t)(j - 1) * d * 2 * N, b, d);
+        ntt_inv(R, buf);
+        ntt_inv(R, buf + N);
+        vadd(out, out, buf, 2 * N);
+    }
+    ntt_fwd(R, out);
+    ntt_fwd(R, out + N);
+}
+
+/* LWEtoRLWEPacker::L0Setup (lwe_to_rlwe_packing.cpp:238-251): LWE (dimension N) -> RLWE in
+ * coefficients with b in the constant coefficient, scaled by N^-1. */
+static void pack_l0(const or_ring* R, uint64_t* out, const uint64_t* lwe) {
+    const uint32_t N = R->N;
+    memset(out, 0, sizeof(uint64_t) * 2 * N);
+    out[0] = lwe[0];
+    for (uint32_t i = 1; i < N; ++i) out[i] = neg(lwe[N - i]);
+    out[N] = lwe[N];
+    for (uint32_t i = 0; i <= N; ++i) out[i] = or_mul(out[i], R->n_inv);
+}
+
+/* X^s p for 0 <= s < N, both halves (coefficients). */
+static void rot_poly2(uint64_t* out, const uint64_t* in, uint32_t s, uint32_t N) {
+    for (int h = 0; h < 2; ++h)
+        for (uint32_t i = 0; i < N; ++i) {
+            const uint32_t k = i + s;
+            if (k < N) out[h * N + k] = in[h * N + i];
+            else out[h * N + k - N] = neg(in[h * N + i]);
+        }
+}
+
+/* LWEtoRLWEPacker::Pack (lwe_to_rlwe_packing.cpp:129-236): len LWEs (dimension N, len a power of
+ * two >= 2) -> one RLWE in coefficients.  Leaf k is input[(len-1) ^ brv(k)]; two nodes of level
+ * l (left = earlier, right = later) merge into
+ *   (R + X^(N/2^(l+1)) L) + tau_(2^(l+1)+1)(R - X^(N/2^(l+1)) L);
+ * then the trace steps t = N+1, ..., 2^(log len + 1) + 1 follow when len < N. */
+OR_EXPORT int or_pack(const or_ring* R, uint64_t* out, const uint64_t* lwes, uint32_t len, const uint64_t* keys,
+                      uint32_t b, uint32_t d) {
+    const uint32_t N = R->N;
+    uint32_t log_l = 0;
+    while ((1u << log_l) < len) ++log_l;
+    if (len < 2 || (1u << log_l) != len || len > N) return -1;
+    uint64_t* nodes = (uint64_t*)malloc(sizeof(uint64_t) * 2 * N * len);
+    uint64_t* rot = (uint64_t*)malloc(sizeof(uint64_t) * 6 * N);
+    uint64_t* plus = rot + 2 * N;
+    uint64_t* aut = rot + 4 * N;
+    for (uint32_t k = 0; k < len; ++k) {
+        const uint32_t src = (len - 1) ^ brv(k, log_l);
+        pack_l0(R, nodes + (size_t)k * 2 * N, lwes + (size_t)src * (N + 1));
+    }
+    for (uint32_t lvl = 1, cnt = len / 2; lvl <= log_l; ++lvl, cnt /= 2) {
+        const uint32_t shift = N >> lvl;
+        for (uint32_t j = 0; j < cnt; ++j) {
+            uint64_t* L = nodes + (size_t)(2 * j) * 2 * N * (1u << (lvl - 1));
+            uint64_t* Rn = nodes + (size_t)(2 * j + 1) * 2 * N * (1u << (lvl - 1));
+            rot_poly2(rot, L, shift, N);
+            vadd(plus, Rn, rot, 2 * N);
+            vsub(Rn, Rn, rot, 2 * N);
+            or_automorphism(R, aut, Rn, (1ULL << lvl) + 1, keys + (size_t)(lvl - 1) * d * 2 * N, b, d);
+            ntt_inv(R, aut);
+            ntt_inv(R, aut + N);
+            vadd(L, plus, aut, 2 * N);  /* the node keeps the left slot */
+        }
+    }
+    memcpy(out, nodes, sizeof(uint64_t) * 2 * N);
+    for (uint32_t lvl = R->logN; lvl > log_l; --lvl) {
+        or_automorphism(R, aut, out, (1ULL << lvl) + 1, keys + (size_t)(lvl - 1) * d * 2 * N, b, d);
+        ntt_inv(R, aut);
+        ntt_inv(R, aut + N);
+        vadd(out, aut, out, 2 * N);
+    }
+    free(nodes);
+    free(rot);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
 /* Batched bootstrap for the CPU baseline: EvalFunc (negacyclic_function_    */
 /* evaluator.cpp:92-101) + LWE key switch, one rotator per thread.          */
 /* ------------------------------------------------------------------------ */

@@ -277,3 +277,77 @@ def test_gadget_kinds(oracle):
     assert list(oracle.gadget(12, 3, 1)) == [2**52, 2**40, 2**28]
     assert list(oracle.gadget(16, 4, 0)) == list(oracle.gadget(16, 4, 1)) == [2**48, 2**32, 2**16, 1]
     assert list(oracle.gadget(14, 3, 0)) == [2**56, 2**42, 2**28]
+
+
+# --------------------------------------------------------------------------
+# RLWE key switching, automorphism, trace, packing: the reference tests' noise-free invariants
+# (tests/automorphism_test.cpp:80-82, trace_tests.cpp:71-74, packing_tests.cpp:65-67,99-104)
+# --------------------------------------------------------------------------
+
+def _phase_coef(oracle, N, ct_ntt, s):
+    return [int(x) for x in oracle.phase_rlwe(N, ct_ntt, s)]
+
+
+def test_automorphism_poly_closed_form(oracle):
+    N = 64
+    rng = np.random.default_rng(1)
+    a = rng.integers(0, Q, size=N, dtype=np.uint64)
+    for t in (3, 5, 17, 2 * N - 1, 257 % (2 * N)):
+        out = oracle.automorphism_poly(a, t, N)
+        # tau_t(a)(X) = a(X^t) mod X^N + 1, checked monomial by monomial
+        ref = [0] * N
+        for i in range(N):
+            e = (i * t) % (2 * N)
+            if e < N:
+                ref[e] = (ref[e] + int(a[i])) % Q
+            else:
+                ref[e - N] = (ref[e - N] - int(a[i])) % Q
+        assert [int(x) for x in out] == ref
+
+
+@pytest.mark.parametrize("t", [3, 5, 129, 257])
+def test_automorphism_noise_free(oracle, t):
+    N, b, d = 128, 16, 4
+    s = oracle.secret(N, 0, 4, 2)
+    R = oracle.Ring(N)
+    s_t = oracle.automorphism_poly(s, t, N)
+    ksk = oracle.rlwe_ksk(N, s_t, s, b, d, 0.0, 4)
+    rng = np.random.default_rng(t)
+    m = rng.integers(0, Q, size=N, dtype=np.uint64)
+    ct = R.intt(oracle.encrypt_rlwe(N, m, s, 0.0, 4, 8))  # coefficients
+    out = oracle.automorphism(N, ct, t, ksk, b, d)
+    assert _phase_coef(oracle, N, out, s) == [int(x) for x in oracle.automorphism_poly(m, t, N)]
+
+
+def test_trace_noise_free(oracle):
+    N, b, d = 128, 16, 4
+    s = oracle.secret(N, 0, 9, 2)
+    keys = oracle.auto_keys(N, s, b, d, 0.0, 9)
+    rng = np.random.default_rng(9)
+    m = rng.integers(0, Q, size=N, dtype=np.uint64)
+    ct = oracle.Ring(N).intt(oracle.encrypt_rlwe(N, m, s, 0.0, 9, 8))
+    out = oracle.trace(N, ct, keys, b, d)
+    ph = _phase_coef(oracle, N, out, s)
+    assert ph[0] == int(m[0]) and not any(ph[1:])
+
+
+@pytest.mark.parametrize("length", [128, 16, 2])
+def test_pack_noise_free(oracle, length):
+    N, b, d = 128, 16, 4
+    s = oracle.secret(N, 0, 12, 2)
+    keys = oracle.auto_keys(N, s, b, d, 0.0, 12)
+    rng = np.random.default_rng(length)
+    msgs = rng.integers(0, Q, size=length, dtype=np.uint64)
+    lwes = np.zeros((length, N + 1), dtype=np.uint64)
+    for i in range(length):
+        a = rng.integers(0, Q, size=N, dtype=np.uint64)
+        bb = (int(msgs[i]) + sum(int(x) * int(y) for x, y in zip(a, s))) % Q
+        lwes[i, :N] = a
+        lwes[i, N] = bb
+    out = oracle.pack(N, lwes, length, keys, b, d)
+    R = oracle.Ring(N)
+    ph = _phase_coef(oracle, N, np.concatenate([R.ntt(out[:N]), R.ntt(out[N:])]), s)
+    stride = N // length
+    for j in range(N):
+        want = int(msgs[j // stride]) if j % stride == 0 else 0
+        assert ph[j] == want, j
