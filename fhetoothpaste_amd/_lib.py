@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libtoothpaste_amd.so")
 
 Q = 0xFFFFFFFF00000001
-TP_OK, TP_E_INVALID, TP_E_DEVICE, TP_E_STATE, TP_E_NOMEM = 0, -1, -2, -3, -4
+TP_OK, TP_E_INVALID, TP_E_DEVICE, TP_E_STATE, TP_E_NOMEM, TP_E_TIMEOUT = 0, -1, -2, -3, -4, -5
 METHOD_CGGI_BINARY, METHOD_CGGI_TERNARY, METHOD_BMMP = 0, 1, 2
 
 u64p = ctypes.POINTER(ctypes.c_uint64)
@@ -30,6 +30,10 @@ class Params(ctypes.Structure):
     _fields_ = [("ring_dim", U32), ("lwe_dim", U32), ("br_base_log2", U32), ("br_digits", U32),
                 ("method", U32), ("ks_out_dim", U32), ("ks_base_log2", U32), ("ks_digits", U32),
                 ("rlwe_ks_base_log2", U32), ("rlwe_ks_digits", U32)]
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("chunk_steps", U32), ("max_slots_per_set", U32), ("hbm_budget_bytes", SZ)]
 
 
 # name -> argtypes (all return int status)
@@ -61,6 +65,24 @@ SIGNATURES = {
     "tp_automorphism_device": [vp, SZ, U32, vp, vp],
     "tp_trace_device": [vp, SZ, vp, vp],
     "tp_pack_device": [vp, SZ, vp, vp, vp],
+    "tp_sched_create": [ctypes.POINTER(vp), U32, ctypes.POINTER(SZ), SZ, ctypes.POINTER(U32)],
+    "tp_sched_destroy": [vp],
+    "tp_sched_submit": [vp, U32, SZ, ctypes.POINTER(U64)],
+    "tp_sched_admit": [vp, U32, ctypes.POINTER(U32), ctypes.POINTER(U64), ctypes.POINTER(U64), SZ,
+                       ctypes.POINTER(SZ)],
+    "tp_sched_release": [vp, U32, ctypes.POINTER(U32), ctypes.POINTER(U64), SZ, ctypes.POINTER(INT)],
+    "tp_sched_cancel": [vp, U64],
+    "tp_sched_forget": [vp, U64],
+    "tp_sched_state": [vp, U32, ctypes.POINTER(SZ), ctypes.POINTER(SZ), ctypes.POINTER(SZ)],
+    "tp_server_create": [ctypes.POINTER(vp), ctypes.POINTER(vp), U32, ctypes.POINTER(ServerConfig)],
+    "tp_server_destroy": [vp],
+    "tp_server_submit": [vp, U32, SZ, u64p, u64p, u64p, ctypes.POINTER(U64)],
+    "tp_server_poll": [vp, U64, ctypes.POINTER(INT)],
+    "tp_server_wait": [vp, U64, ctypes.c_double],
+    "tp_server_cancel": [vp, U64],
+    "tp_server_release": [vp, U64],
+    "tp_server_times": [vp, U64, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)],
+    "tp_server_stats": [vp, ctypes.POINTER(U64), ctypes.POINTER(U64), ctypes.POINTER(U64)],
 }
 
 _lib = None

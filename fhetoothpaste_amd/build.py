@@ -23,6 +23,7 @@ SOURCES = [
     ("kernels_pack.hip", DEVICE_FLAGS),
     ("capi.cpp", HOST_FLAGS),
     ("keygen.cpp", HOST_FLAGS),
+    ("serve.cpp", HOST_FLAGS),
     ("dropin.cpp", HOST_FLAGS),
 ]
 

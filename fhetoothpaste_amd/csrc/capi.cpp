@@ -9,6 +9,7 @@
 
 #include "../../include/toothpaste_amd.h"
 #include "gl64.hpp"
+#include "context_internal.hpp"
 #include "host_ntt.hpp"
 #include "kernels.hpp"
 
@@ -113,13 +114,8 @@ int ensure_scratch(tp_context* c, size_t words) {
     return TP_OK;
 }
 
-int launch_eval(tp_context* c, size_t count, const uint64_t* lwes, const uint64_t* tv, size_t tv_stride,
-                uint64_t* out_lwe, uint64_t* out_rlwe, int rlwe_ntt, int grid) {
-    if (!c->d_brk) return fail(TP_E_STATE, "blind-rotation key not imported");
-    if (count == 0) return TP_OK;
-    if (count > 0x7fffffff) return fail(TP_E_INVALID, "count too large");
-    if (tv_stride != 0 && tv_stride != 2 * (size_t)c->N) return fail(TP_E_INVALID, "tv_stride must be 0 or 2N");
-    tp::BrArgs A{};
+void fill_br_args(tp_context* c, tp::BrArgs& A, const uint64_t* lwes, const uint64_t* tv, size_t tv_stride,
+                  uint64_t* out_lwe, uint64_t* out_rlwe, int rlwe_ntt, size_t count) {
     A.brk = c->d_brk;
     A.lwe = lwes;
     A.tv = tv;
@@ -137,6 +133,16 @@ int launch_eval(tp_context* c, size_t count, const uint64_t* lwes, const uint64_
     A.least = c->br_least;
     A.max_digits = (int)max_digits(c->p.br_base_log2);
     A.mono = c->d_mono;
+}
+
+int launch_eval(tp_context* c, size_t count, const uint64_t* lwes, const uint64_t* tv, size_t tv_stride,
+                uint64_t* out_lwe, uint64_t* out_rlwe, int rlwe_ntt, int grid) {
+    if (!c->d_brk) return fail(TP_E_STATE, "blind-rotation key not imported");
+    if (count == 0) return TP_OK;
+    if (count > 0x7fffffff) return fail(TP_E_INVALID, "count too large");
+    if (tv_stride != 0 && tv_stride != 2 * (size_t)c->N) return fail(TP_E_INVALID, "tv_stride must be 0 or 2N");
+    tp::BrArgs A{};
+    fill_br_args(c, A, lwes, tv, tv_stride, out_lwe, out_rlwe, rlwe_ntt, count);
     if (grid <= 0) grid = (int)count;
     hipError_t e = tp::launch_blind_rotate(c->logN, (int)c->p.method, (int)c->p.br_digits, A, grid, c->stream);
     if (e != hipSuccess) return fail(TP_E_DEVICE, std::string("blind rotation launch: ") + hipGetErrorString(e));
@@ -451,12 +457,23 @@ int tp_eval_func_batch_device(tp_context* c, size_t count, const uint64_t* lwes,
 int tp_lwe_keyswitch_batch_device(tp_context* c, size_t count, const uint64_t* in, size_t in_stride,
                                   uint64_t* out) {
     if (!c || !in || !out) return fail(TP_E_INVALID, "null argument");
+    int rc = set_device(c);
+    if (rc) return rc;
+    return tp::launch_ks_rows(c, count, in, in_stride, nullptr, out);
+}
+
+}  // extern "C"
+
+namespace tp {
+
+int launch_ks_rows(tp_context* c, size_t count, const uint64_t* in, size_t in_stride, const uint32_t* rows,
+                   uint64_t* out) {
     if (!c->d_ksk) return fail(TP_E_STATE, "key-switching key not imported");
     if (in_stride < (size_t)c->N + 1) return fail(TP_E_INVALID, "in_stride < N + 1");
     if (count == 0) return TP_OK;
-    int rc = set_device(c);
-    if (rc) return rc;
+    if (count > 0x7fffffff) return fail(TP_E_INVALID, "count too large");
     tp::KsArgs A{};
+    A.rows = rows;
     A.ksk = c->d_ksk;
     A.corr = c->d_corr;
     A.in = in;
@@ -472,6 +489,34 @@ int tp_lwe_keyswitch_batch_device(tp_context* c, size_t count, const uint64_t* i
     if (e != hipSuccess) return fail(TP_E_DEVICE, std::string("key switch launch: ") + hipGetErrorString(e));
     return TP_OK;
 }
+
+int launch_chunk(tp_context* c, size_t count, const uint32_t* active, uint32_t* step, uint64_t* state,
+                 const uint64_t* lwes, const uint64_t* tv, const uint32_t* tv_index, uint64_t* ext,
+                 uint32_t chunk_steps) {
+    if (!c->d_brk) return fail(TP_E_STATE, "blind-rotation key not imported");
+    if (count == 0) return TP_OK;
+    if (count > 0x7fffffff || chunk_steps > 0x7fffffff) return fail(TP_E_INVALID, "count too large");
+    tp::BrArgs A{};
+    fill_br_args(c, A, lwes, tv, 0, ext, nullptr, 0, count);
+    A.active = active;
+    A.slot_step = step;
+    A.state = state;
+    A.tv_index = tv_index;
+    A.chunk_steps = (int)chunk_steps;
+    hipError_t e = tp::launch_blind_rotate(c->logN, (int)c->p.method, (int)c->p.br_digits, A, (int)count, c->stream);
+    if (e != hipSuccess) return fail(TP_E_DEVICE, std::string("blind rotation launch: ") + hipGetErrorString(e));
+    return TP_OK;
+}
+
+int set_error(int code, const char* msg) { return fail(code, msg); }
+hipStream_t context_stream(tp_context* c) { return c->stream; }
+void context_params(tp_context* c, tp_params* p) { *p = c->p; }
+int context_ready(tp_context* c) { return c->d_brk && c->d_ksk; }
+int bind_device(tp_context* c) { return set_device(c); }
+
+}  // namespace tp
+
+extern "C" {
 
 int tp_bootstrap_batch_device(tp_context* c, size_t count, const uint64_t* lwes, const uint64_t* tv,
                               size_t tv_stride, uint64_t* out) {

@@ -134,15 +134,27 @@ br_kernel(BrArgs A) {
     const size_t rgsw_words = (size_t)4 * N * LD;
 
     for (int ct = blockIdx.x; ct < A.count; ct += gridDim.x) {
-        const uint64_t* lwe = A.lwe + (size_t)ct * (A.n + 1);
-        const uint64_t* tv = A.tv + (size_t)ct * A.tv_stride;
+        const int slot = A.active ? (int)A.active[ct] : ct;
+        const uint64_t* lwe = A.lwe + (size_t)slot * (A.n + 1);
+        const uint64_t* tv = A.tv + (A.tv_index ? (size_t)A.tv_index[slot] * 2 * N : (size_t)slot * A.tv_stride);
+        const int total = METHOD == 2 ? A.n / 2 : A.n;
+        const int i0 = A.slot_step ? (int)A.slot_step[slot] : 0;
+        const int i1 = (A.chunk_steps > 0 && i0 + A.chunk_steps < total) ? i0 + A.chunk_steps : total;
         uint64_t acc[2][kE];
-        const uint32_t bexp = (uint32_t)lwe[A.n];
-        rotate_from<LOGN>(acc[0], tv, bexp, L, false);
-        rotate_from<LOGN>(acc[1], tv + N, bexp, L, false);
+        if (i0 == 0) {
+            const uint32_t bexp = (uint32_t)lwe[A.n];
+            rotate_from<LOGN>(acc[0], tv, bexp, L, false);
+            rotate_from<LOGN>(acc[1], tv + N, bexp, L, false);
+        } else {
+            const uint64_t* st = A.state + (size_t)slot * 2 * N;
+#pragma unroll
+            for (int k = 0; k < kE; ++k) {
+                acc[0][k] = st[idx1<LOGN>(L, k)];
+                acc[1][k] = st[N + idx1<LOGN>(L, k)];
+            }
+        }
 
-        const int steps = METHOD == 2 ? A.n / 2 : A.n;
-        for (int i = 0; i < steps; ++i) {
+        for (int i = i0; i < i1; ++i) {
             // monomial exponents of this step (NTT(X^e) rows of the table, e in [0, 2N))
             uint32_t e0 = 0, e1 = 0, e2 = 0;
             if constexpr (METHOD == 0 || METHOD == 1) {
@@ -257,9 +269,22 @@ br_kernel(BrArgs A) {
             }
         }
 
+        if (A.slot_step && L == 0) A.slot_step[slot] = (uint32_t)i1;
+        if (i1 < total) {
+            // chunk boundary: park the accumulator for the next launch
+            uint64_t* st = A.state + (size_t)slot * 2 * N;
+#pragma unroll
+            for (int k = 0; k < kE; ++k) {
+                st[idx1<LOGN>(L, k)] = acc[0][k];
+                st[N + idx1<LOGN>(L, k)] = acc[1][k];
+            }
+            poly_sync<S::WAVES>();
+            continue;
+        }
+
         // outputs: sample extraction at index 0 (sample_extraction.h:20-36) and optional RLWE
         if (A.out_lwe) {
-            uint64_t* o = A.out_lwe + (size_t)ct * (N + 1);
+            uint64_t* o = A.out_lwe + (size_t)slot * (N + 1);
 #pragma unroll
             for (int k = 0; k < kE; ++k) {
                 const int j = idx1<LOGN>(L, k);
@@ -270,7 +295,7 @@ br_kernel(BrArgs A) {
         if (A.out_rlwe) {
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
-                uint64_t* o = A.out_rlwe + (size_t)ct * 2 * N + h * N;
+                uint64_t* o = A.out_rlwe + (size_t)slot * 2 * N + h * N;
                 if (A.out_rlwe_ntt) {
                     ntt_forward<LOGN>(acc[h], buf, A.tw, L);
 #pragma unroll
@@ -299,10 +324,11 @@ constexpr int kKsBlock = 256;
 
 __global__ void __launch_bounds__(kKsBlock) lwe_keyswitch_kernel(KsArgs A) {
     const int col0 = blockIdx.x * kKsCols;
-    const int ct = blockIdx.y * kKsBlock + threadIdx.x;
+    const int c = blockIdx.y * kKsBlock + threadIdx.x;
     const int W = A.n_out + 1;
-    const bool live = ct < A.count;
-    const uint64_t* in = A.in + (size_t)(live ? ct : 0) * A.in_stride;
+    const bool live = c < A.count;
+    const int ct = live ? (A.rows ? (int)A.rows[c] : c) : 0;
+    const uint64_t* in = A.in + (size_t)ct * A.in_stride;
     uint64_t lo[kKsCols], hi[kKsCols];
 #pragma unroll
     for (int t = 0; t < kKsCols; ++t) lo[t] = hi[t] = 0;
@@ -333,7 +359,7 @@ __global__ void __launch_bounds__(kKsBlock) lwe_keyswitch_kernel(KsArgs A) {
         const uint64_t carry = t_lo < lo[t];
         const uint64_t t_hi = (hi[t] >> 32) + carry;
         const uint64_t acc = gl::sub(gl::reduce128(t_hi, t_lo), A.corr[col]);
-        uint64_t* o = A.out + (size_t)ct * W;
+        uint64_t* o = A.out + (size_t)c * W;  // compact: output row c reads input row rows[c]
         if (col < A.n_out) o[col] = gl::neg(acc);
         else o[col] = gl::sub(in[A.n_in], acc);
     }

@@ -26,6 +26,12 @@ struct BrArgs {
     uint64_t least;
     int max_digits;
     const uint64_t* mono;  // NTT(X^e), e < N, key-row order (ternary, BMMP)
+    // chunked (continuous-batching) mode; all null / 0 for one launch over whole rotations
+    const uint32_t* active;    // [count] slot processed by work-group ct (null: slot = ct)
+    uint32_t* slot_step;       // [slots] next blind-rotation step of each slot (null: start at 0)
+    uint64_t* state;           // [slots][2N] accumulator between launches (coefficients)
+    const uint32_t* tv_index;  // [slots] test-vector row of each slot (null: tv_stride rule)
+    int chunk_steps;           // steps per launch (0: run to the end)
 };
 
 struct KsArgs {
@@ -36,6 +42,7 @@ struct KsArgs {
     size_t in_stride;
     uint64_t corrector;    // signed-to-unsigned corrector C mod Q
     int n_in, n_out, base_log2, digits, count;
+    const uint32_t* rows;  // [count] input row of ciphertext c (null: row c); output row is c
 };
 
 // RLWE key switching with signed digits (automorphism, trace and packing keys).

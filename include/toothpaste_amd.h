@@ -33,6 +33,7 @@ extern "C" {
 #define TP_E_DEVICE (-2)    /* HIP runtime error or no gfx950 device */
 #define TP_E_STATE (-3)     /* keys not imported yet (reference: assert(m_keys_generated)) */
 #define TP_E_NOMEM (-4)     /* device allocation failed */
+#define TP_E_TIMEOUT (-5)   /* tp_server_wait: request not complete in time */
 
 #define TP_Q 0xFFFFFFFF00000001ULL
 
@@ -149,6 +150,56 @@ int tp_copy_on_device(tp_context* ctx, uint64_t* dst, const uint64_t* src, size_
  * tp_timer_mark(ctx, slot) records an event; tp_timer_elapsed(ctx, a, b, &ms). */
 int tp_timer_mark(tp_context* ctx, int slot);
 int tp_timer_elapsed(tp_context* ctx, int slot_a, int slot_b, float* ms);
+
+/* ------------------------------------------------------------------ serving (continuous batching)
+ * No counterpart in the reference, which evaluates one ciphertext per call
+ * (negacyclic_function_evaluator.cpp:92-101 then lwe_conversion.cpp:205-239): a server keeps
+ * the keys of several parameter sets resident and bootstraps the ciphertexts of concurrent
+ * requests in shared launches.  Every ciphertext's output equals what
+ * tp_bootstrap_batch_device produces for it alone, bit for bit.
+ *
+ * Scheduler core (host only, usable without a GPU): per parameter set a pool of max_slots[s]
+ * slots of slot_bytes[s] bytes of HBM each; tp_sched_admit hands out slots to the ciphertexts
+ * of queued requests in FIFO order while slots are free and the admitted footprint stays
+ * within budget_bytes; tp_sched_release returns them and reports completed requests. */
+typedef struct tp_sched tp_sched;
+int tp_sched_create(tp_sched** out, uint32_t nsets, const size_t* slot_bytes, size_t budget_bytes,
+                    const uint32_t* max_slots);
+int tp_sched_destroy(tp_sched* s);
+int tp_sched_submit(tp_sched* s, uint32_t set, size_t count, uint64_t* id);
+int tp_sched_admit(tp_sched* s, uint32_t set, uint32_t* slots, uint64_t* req_ids, uint64_t* indices, size_t cap,
+                   size_t* n_admitted);
+int tp_sched_release(tp_sched* s, uint32_t set, const uint32_t* slots, const uint64_t* req_ids, size_t n,
+                     int* completed);
+int tp_sched_cancel(tp_sched* s, uint64_t id);
+int tp_sched_forget(tp_sched* s, uint64_t id);
+int tp_sched_state(tp_sched* s, uint32_t set, size_t* free_slots, size_t* queued_requests, size_t* used_bytes);
+
+/* Server: one worker thread; each parameter set (a context with brk and LWE ksk imported)
+ * runs on its context's stream.  Each iteration admits queued ciphertexts, advances every
+ * active slot by chunk_steps blind-rotation steps, key-switches the slots that finished and
+ * copies their outputs to the requests' host buffers.
+ * tp_server_submit: count LWE ciphertexts (host, count x (n+1), entries in [0, 2N), kept
+ * readable until completion), one test vector (host, 2N coefficients), output buffer
+ * (host, count x (ks_out_dim+1)).  tp_server_wait returns TP_E_TIMEOUT when the request is
+ * still running at the deadline; tp_server_release forgets a completed request.
+ * tp_server_times: submit and completion instants of a request (seconds, steady clock). */
+typedef struct tp_server tp_server;
+typedef struct tp_server_config {
+    uint32_t chunk_steps;        /* blind-rotation steps per launch (0: 64) */
+    uint32_t max_slots_per_set;  /* slot pool size per parameter set (0: 4096) */
+    size_t hbm_budget_bytes;     /* slot footprint limit over all sets (0: 90% of free HBM) */
+} tp_server_config;
+int tp_server_create(tp_server** out, tp_context* const* ctxs, uint32_t nsets, const tp_server_config* cfg);
+int tp_server_destroy(tp_server* srv);
+int tp_server_submit(tp_server* srv, uint32_t set, size_t count, const uint64_t* lwes, const uint64_t* tv,
+                     uint64_t* out, uint64_t* id);
+int tp_server_poll(tp_server* srv, uint64_t id, int* done);
+int tp_server_wait(tp_server* srv, uint64_t id, double timeout_s);
+int tp_server_cancel(tp_server* srv, uint64_t id);
+int tp_server_release(tp_server* srv, uint64_t id);
+int tp_server_times(tp_server* srv, uint64_t id, double* submitted, double* completed);
+int tp_server_stats(tp_server* srv, uint64_t* iterations, uint64_t* launches, uint64_t* ciphertexts);
 
 /* ------------------------------------------------------------------ seeded key generation
  * Host-side key generation with a seeded sampler (the reference samples with OpenFHE, which
