@@ -61,6 +61,90 @@ def cpu_model():
     return platform.processor()
 
 
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+# second resident parameter set of the serving leg: ternary n=800, N=2048, exact gadget 2^16 x 4,
+# key switch 2048 -> 800 with base 2^5 and 12 digits
+TER = dict(N=2048, n=800, b=16, l=4, method=1, n_out=800, kb=5, kd=12)
+
+
+def serving_leg(args, tp, ctx_bin):
+    """Poisson stream over {binary n=630 N=1024, ternary n=800 N=2048}, random 3-bit LUTs,
+    lognormal counts (median 32, sigma_ln 1.0, clipped to [1, 512]); lambda = 0.8 x peak."""
+    from fhetoothpaste_amd.serving import Server, replay
+    from fhetoothpaste_amd.workload import lognormal_counts, poisson_arrivals, random_3bit_lut
+
+    t0 = time.time()
+    seed = SEED + 1
+    lk3 = tp.keygen_secret(TER["n"], 1, seed, 1)
+    rk3 = tp.keygen_secret(TER["N"], 0, seed, 2)
+    ctx_ter = tp.Context(TER["N"], TER["n"], TER["b"], TER["l"], TER["method"], TER["n_out"], TER["kb"], TER["kd"])
+    ctx_ter.import_brk(tp.keygen_brk(lk3, rk3, TER["N"], TER["n"], TER["b"], TER["l"], TER["method"], 1, 3.2, seed))
+    ctx_ter.import_lwe_ksk(tp.keygen_lwe_ksk(rk3, lk3, TER["N"], TER["n_out"], TER["kb"], TER["kd"], 3.2, seed))
+    log("serving keys ready (%.1f s)" % (time.time() - t0))
+
+    rng = np.random.default_rng(SEED + 5)
+    dims = [(N, n), (TER["N"], TER["n"])]
+    pools = [rng.integers(0, 2 * NN, size=(4096, nn + 1), dtype=np.uint64) for NN, nn in dims]
+
+    def draw(k):
+        sets = rng.integers(0, 2, size=k)
+        counts = lognormal_counts(rng, k)
+        reqs = []
+        for si, cnt in zip(sets, counts):
+            off = int(rng.integers(0, 4096 - cnt + 1))
+            tv, _ = random_3bit_lut(dims[si][0], rng)
+            reqs.append([0.0, int(si), pools[si][off:off + cnt], tv])
+        return reqs
+
+    def summarize(records):
+        lat = np.array([r["done"] - r["arrival"] for r in records])
+        cts = sum(r["count"] for r in records)
+        span = max(r["done"] for r in records) - min(r["arrival"] for r in records)
+        return lat, cts, span
+
+    mean_count = float(lognormal_counts(np.random.default_rng(7), 200000).mean())
+    with Server([ctx_bin, ctx_ter], chunk_steps=args.serve_chunk) as srv:
+        # peak: a burst of the same request mix, all arriving at once
+        burst = draw(400)
+        rec = replay(srv, burst)
+        _, cts, span = summarize(rec)
+        peak = cts / span
+        log("serving peak %.0f bootstraps/s (%d ciphertexts in %.2f s)" % (peak, cts, span))
+        lam = 0.8 * peak / mean_count
+        arrivals = poisson_arrivals(rng, lam, args.serve_seconds)
+        stream = draw(len(arrivals))
+        for r, t in zip(stream, arrivals):
+            r[0] = float(t)
+        log("serving stream: %d requests over %.0f s (lambda %.2f/s)" % (len(stream), args.serve_seconds, lam))
+        rec = replay(srv, stream, progress=log)
+        st = srv.stats()
+    ctx_ter.close()
+    lat, cts, span = summarize(rec)
+    out = {
+        "sets": ["binary n=630 N=1024 b=2^12 l=3, KS 1024->630 b=2^4 d=14",
+                 "ternary n=800 N=2048 b=2^16 l=4, KS 2048->800 b=2^5 d=12"],
+        "requests": len(rec), "ciphertexts": int(cts), "duration_s": round(span, 3),
+        "peak_bootstraps_per_s": round(peak, 1), "lambda_requests_per_s": round(lam, 3),
+        "mean_count": round(mean_count, 2),
+        "offered_bootstraps_per_s": round(lam * mean_count, 1),
+        "throughput_bootstraps_per_s": round(cts / span, 1),
+        "p50_ms": round(1000 * float(np.percentile(lat, 50)), 2),
+        "p99_ms": round(1000 * float(np.percentile(lat, 99)), 2),
+        "chunk_steps": args.serve_chunk, "launches": st["launches"], "iterations": st["iterations"],
+        "counts": "lognormal median 32 sigma_ln 1.0 clipped to [1, 512]",
+        "lut": "random 3-bit, f(p) = v_floor(8p/N) * floor(q/16)",
+    }
+    for si in (0, 1):
+        l_s = np.array([r["done"] - r["arrival"] for r in rec if r["set"] == si])
+        if l_s.size:
+            out["set%d_p50_ms" % si] = round(1000 * float(np.percentile(l_s, 50)), 2)
+            out["set%d_p99_ms" % si] = round(1000 * float(np.percentile(l_s, 99)), 2)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -70,12 +154,15 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=60.0,
+                    help="length of the Poisson request stream of the serving leg (0 skips it)")
+    ap.add_argument("--serve-chunk", type=int, default=64, help="blind-rotation steps per serving launch")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="issue one device-to-device copy of this size first (profiler byte-counter scale)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this path runs on one GPU; --gpus %d is not supported (sharding is described in "
-              "DESIGN.md)" % args.gpus, file=sys.stderr)
+        print("bench.py: this path runs on one GPU; --gpus %d is not supported (ciphertexts shard across "
+              "GPUs with replicated keys, which this build does not run)" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
     import fhetoothpaste_amd as tp
@@ -200,6 +287,11 @@ def main():
                      "hbm_key_stream_GBps": round(key_bytes / (br_avg_s + 1e-3 * sum(ks_ms) / len(ks_ms)) / 1e9, 2)},
         "keygen_s": round(keygen_s, 2),
     }
+
+    if args.serve_seconds > 0:
+        result["serving"] = serving_leg(args, tp, ctx)
+        result["latency_ms"]["serve_p50"] = result["serving"]["p50_ms"]
+        result["latency_ms"]["serve_p99"] = result["serving"]["p99_ms"]
 
     if not args.no_cpu:
         import oracle as O

@@ -138,3 +138,56 @@ class Server:
         it, la, ct = T.U64(), T.U64(), T.U64()
         check(lib().tp_server_stats(self.h, ctypes.byref(it), ctypes.byref(la), ctypes.byref(ct)))
         return {"iterations": it.value, "launches": la.value, "ciphertexts": ct.value}
+
+
+def replay(server, requests, progress=None):
+    """Submit requests at their arrival instants and collect completion times.
+
+    requests: list of (arrival_s, set_index, lwes, tv) sorted by arrival_s (seconds from the
+    start of the replay).  Returns a list of dicts {set, count, arrival, submitted, done}, all in
+    seconds of the steady clock the server stamps completions with (time.monotonic on Linux).
+    Latency of a request = done - arrival (arrival is the scheduled instant, so a late submission
+    counts against the server).  Completed requests are released as they are collected.
+    """
+    import queue
+    import threading
+    import time
+
+    pending = queue.Queue()
+    records = []
+    errors = []
+
+    def collect():
+        while True:
+            item = pending.get()
+            if item is None:
+                return
+            h, rec = item
+            try:
+                h.wait(600.0)
+                rec["submitted"], rec["done"] = h.times()
+                h.release()
+            except Exception as e:  # noqa: BLE001 - reported to the caller below
+                errors.append(e)
+                return
+            records.append(rec)
+
+    th = threading.Thread(target=collect, daemon=True)
+    th.start()
+    t0 = time.monotonic()
+    next_report = 10.0
+    for arrival, set_index, lwes, tv in requests:
+        now = time.monotonic() - t0
+        if arrival > now:
+            time.sleep(arrival - now)
+        h = server.submit(set_index, lwes, tv)
+        pending.put((h, {"set": set_index, "count": int(lwes.shape[0]), "arrival": t0 + arrival}))
+        if progress and arrival >= next_report:
+            progress("replay %.0f s: %d requests submitted, %d collected" % (arrival, len(records) + pending.qsize(),
+                                                                            len(records)))
+            next_report += 10.0
+    pending.put(None)
+    th.join()
+    if errors:
+        raise errors[0]
+    return records
