@@ -9,11 +9,11 @@
 //     input, accumulator, extracted LWE, output and test-vector row in HBM.  Ciphertexts of
 //     queued requests are admitted FIFO whenever a slot is free and the HBM footprint of all
 //     admitted slots stays within the budget.
-//   * Server loop (one thread): every iteration admits new ciphertexts, then advances every
-//     active slot of every parameter set by `chunk_steps` blind-rotation steps (one launch per
-//     set, each set on its own stream so the sets overlap on the device), key-switches the slots
-//     that finished, copies their results out and frees the slots.  Requests that arrive while
-//     others are on the device join at the next iteration.
+//   * Server loop (one thread per parameter set, each on its set's stream so the sets overlap on
+//     the device): every iteration admits new ciphertexts, advances every active slot of the set
+//     by `chunk_steps` blind-rotation steps in one launch, key-switches the slots that finished,
+//     copies their results out and frees the slots.  Requests that arrive while others are on
+//     the device join at the next iteration.
 //
 // A ciphertext's result does not depend on its slot or on the iteration it was admitted in:
 // every slot runs the same steps on its own data and the accumulator is parked in HBM between
@@ -219,11 +219,12 @@ struct tp_server {
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::map<uint64_t, std::unique_ptr<Req>> reqs;
-    std::thread worker;
+    std::vector<std::thread> workers;
     std::atomic<bool> stop{false};
     int failed = 0;
     std::string fail_msg;
-    uint64_t launches = 0, iterations = 0, ciphertexts = 0;
+    std::atomic<uint64_t> launches{0};  // advanced outside the lock by the set workers
+    uint64_t iterations = 0, ciphertexts = 0;
 };
 
 namespace {
@@ -343,42 +344,32 @@ void fail_all(tp_server* S, int code, const std::string& msg) {
     S->cv_done.notify_all();
 }
 
-void server_main(tp_server* S) {
-    if (tp::bind_device(S->sets[0].ctx)) {
+// One worker per parameter set: each advances its own slots on its own stream, so a set with
+// short steps is not held back by a slower one; the sets share the device and the HBM budget.
+void server_main(tp_server* S, uint32_t si) {
+    tp_server::Set& st = S->sets[si];
+    if (tp::bind_device(st.ctx)) {
         fail_all(S, TP_E_DEVICE, "server thread could not select the device");
         return;
     }
     while (!S->stop.load()) {
-        bool worked = false;
         int rc = TP_OK;
         {
             std::lock_guard<std::mutex> lk(S->mu);
-            for (uint32_t si = 0; si < S->sets.size() && !rc; ++si) rc = admit(S, si);
+            if (S->failed) return;
+            rc = admit(S, si);
         }
-        for (uint32_t si = 0; si < S->sets.size() && !rc; ++si) {
-            rc = advance(S, S->sets[si]);
-            worked |= !S->sets[si].active.empty();
-        }
-        for (tp_server::Set& st : S->sets) {
-            if (rc) break;
-            if (!st.active.empty() && hipStreamSynchronize(tp::context_stream(st.ctx)) != hipSuccess)
-                rc = tp::set_error(TP_E_DEVICE, "server stream failed");
-        }
+        if (!rc) rc = advance(S, st);
+        if (!rc && !st.active.empty() && hipStreamSynchronize(tp::context_stream(st.ctx)) != hipSuccess)
+            rc = tp::set_error(TP_E_DEVICE, "server stream failed");
         if (rc) {
             fail_all(S, rc, tp_last_error());
             return;
         }
-        {
-            std::lock_guard<std::mutex> lk(S->mu);
-            bool completed = false;
-            for (uint32_t si = 0; si < S->sets.size(); ++si) completed |= deliver(S, si);
-            S->iterations++;
-            if (completed) S->cv_done.notify_all();
-        }
-        if (!worked) {
-            std::unique_lock<std::mutex> lk(S->mu);
-            S->cv_work.wait_for(lk, std::chrono::milliseconds(1));
-        }
+        std::unique_lock<std::mutex> lk(S->mu);
+        if (deliver(S, si)) S->cv_done.notify_all();
+        S->iterations++;
+        if (st.active.empty()) S->cv_work.wait_for(lk, std::chrono::milliseconds(1));
     }
 }
 
@@ -467,7 +458,7 @@ int tp_server_create(tp_server** out, tp_context* const* ctxs, uint32_t nsets, c
         return rc;
     }
     tp_server* raw = S.release();
-    raw->worker = std::thread(server_main, raw);
+    for (uint32_t si = 0; si < nsets; ++si) raw->workers.emplace_back(server_main, raw, si);
     *out = raw;
     return TP_OK;
 }
@@ -476,7 +467,8 @@ int tp_server_destroy(tp_server* S) {
     if (!S) return TP_OK;
     S->stop.store(true);
     S->cv_work.notify_all();
-    if (S->worker.joinable()) S->worker.join();
+    for (std::thread& w : S->workers)
+        if (w.joinable()) w.join();
     destroy_sets(S);
     tp_sched_destroy(S->sched);
     delete S;
@@ -579,7 +571,7 @@ int tp_server_stats(tp_server* S, uint64_t* iterations, uint64_t* launches, uint
     if (!S) return tp::set_error(TP_E_INVALID, "null argument");
     std::lock_guard<std::mutex> lk(S->mu);
     if (iterations) *iterations = S->iterations;
-    if (launches) *launches = S->launches;
+    if (launches) *launches = S->launches.load();
     if (ciphertexts) *ciphertexts = S->ciphertexts;
     return TP_OK;
 }
