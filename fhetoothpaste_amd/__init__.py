@@ -7,4 +7,4 @@ operator API.  Build with `python -m fhetoothpaste_amd.build`.
 from ._lib import Q, ToothpasteError, device_count, lib  # noqa: F401
 from .operators import (  # noqa: F401
     BINARY, TERNARY, CGGIBlindRotationContext, CGGIBlindRotator, Context, DeviceBuffer, GenericKey,
-    LWEtoLWEConverter, SampleExtract, keygen_auto_keys, keygen_brk, keygen_lwe_ksk, keygen_secret)
+    LWEtoLWEConverter, SampleExtract, keygen_auto_keys, keygen_brk, keygen_lwe_ksk, keygen_rlwe_ksk, keygen_secret)

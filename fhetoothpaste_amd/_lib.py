@@ -14,6 +14,8 @@ LIB_PATH = os.path.join(HERE, "libtoothpaste_amd.so")
 Q = 0xFFFFFFFF00000001
 TP_OK, TP_E_INVALID, TP_E_DEVICE, TP_E_STATE, TP_E_NOMEM, TP_E_TIMEOUT = 0, -1, -2, -3, -4, -5
 METHOD_CGGI_BINARY, METHOD_CGGI_TERNARY, METHOD_BMMP = 0, 1, 2
+TP_RKS_CONVERT, TP_RKS_AUTOMORPHISM, TP_RKS_SWITCH_ONLY, TP_RKS_ACCUMULATE = 0, 1, 2, 4
+TP_ELT_ADD, TP_ELT_SUB, TP_ELT_MUL, TP_ELT_FMA, TP_ELT_MULS, TP_ELT_REDUCE = 0, 1, 2, 3, 4, 5
 
 u64p = ctypes.POINTER(ctypes.c_uint64)
 vp = ctypes.c_void_p
@@ -65,6 +67,11 @@ SIGNATURES = {
     "tp_automorphism_device": [vp, SZ, U32, vp, vp],
     "tp_trace_device": [vp, SZ, vp, vp],
     "tp_pack_device": [vp, SZ, vp, vp, vp],
+    "tp_import_rlwe_ksk": [vp, u64p, SZ],
+    "tp_rlwe_keyswitch_device": [vp, SZ, U32, INT, INT, vp, vp],
+    "tp_pack_consecutive_device": [vp, SZ, vp, vp],
+    "tp_eltwise_device": [vp, INT, vp, vp, vp, vp, U64, SZ],
+    "tp_keygen_rlwe_ksk": [u64p, u64p, u64p, U32, U32, U32, ctypes.c_double, U64, U64],
     "tp_sched_create": [ctypes.POINTER(vp), U32, ctypes.POINTER(SZ), SZ, ctypes.POINTER(U32)],
     "tp_sched_destroy": [vp],
     "tp_sched_submit": [vp, U32, SZ, ctypes.POINTER(U64)],

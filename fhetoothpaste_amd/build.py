@@ -21,6 +21,7 @@ HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall"]
 SOURCES = [
     ("kernels.hip", DEVICE_FLAGS),
     ("kernels_pack.hip", DEVICE_FLAGS),
+    ("kernels_eltwise.hip", DEVICE_FLAGS),
     ("capi.cpp", HOST_FLAGS),
     ("keygen.cpp", HOST_FLAGS),
     ("serve.cpp", HOST_FLAGS),

@@ -62,6 +62,7 @@ struct tp_context {
     size_t nodes_words = 0;
     uint32_t* d_perm = nullptr;
     size_t perm_len = 0;
+    uint64_t* d_rksk = nullptr;  // RLWEtoRLWEConverter key, d x 2N, key-row order
     hipEvent_t ev[8] = {};
 };
 
@@ -253,7 +254,7 @@ int tp_context_destroy(tp_context* c) {
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
     for (uint64_t* p : {c->d_tw, c->d_itw, c->d_brk, c->d_ksk, c->d_corr, c->d_scratch, c->d_io, c->d_mono,
-                        c->d_auto, c->d_nodes})
+                        c->d_auto, c->d_nodes, c->d_rksk})
         if (p) hipFree(p);
     for (auto& ev : c->ev)
         if (ev) hipEventDestroy(ev);
@@ -387,10 +388,8 @@ int tp_trace_device(tp_context* c, size_t count, const uint64_t* in, uint64_t* o
     return TP_OK;
 }
 
-int tp_pack_device(tp_context* c, size_t len, const uint64_t* lwes, const uint32_t* perm, uint64_t* out) {
-    if (!c || !lwes || !out) return fail(TP_E_INVALID, "null argument");
-    if (!c->d_auto) return fail(TP_E_STATE, "automorphism keys not imported");
-    if (len < 2 || len > c->N || (len & (len - 1))) return fail(TP_E_INVALID, "len must be a power of two in [2, N]");
+namespace {
+int run_pack(tp_context* c, size_t len, size_t n_inputs, const uint64_t* lwes, const uint32_t* perm, uint64_t* out) {
     int rc = set_device(c);
     if (rc) return rc;
     const size_t words = len * 2 * c->N;
@@ -401,9 +400,78 @@ int tp_pack_device(tp_context* c, size_t len, const uint64_t* lwes, const uint32
         if (hipMalloc(&c->d_nodes, words * 8) != hipSuccess) return fail(TP_E_NOMEM, "packing buffer allocation failed");
         c->nodes_words = words;
     }
-    hipError_t e = tp::launch_pack(c->logN, auto_args(c), lwes, perm, (int)len, c->d_nodes, c->d_auto, c->stream);
+    hipError_t e = tp::launch_pack(c->logN, auto_args(c), lwes, perm, (int)len, (int)n_inputs, c->d_nodes, c->d_auto,
+                                   c->stream);
     if (e != hipSuccess) return fail(TP_E_DEVICE, std::string("packing launch: ") + hipGetErrorString(e));
     TP_HIP(hipMemcpyAsync(out, c->d_nodes, 2 * c->N * 8, hipMemcpyDeviceToDevice, c->stream));
+    return TP_OK;
+}
+}  // namespace
+
+int tp_pack_device(tp_context* c, size_t len, const uint64_t* lwes, const uint32_t* perm, uint64_t* out) {
+    if (!c || !lwes || !out) return fail(TP_E_INVALID, "null argument");
+    if (!c->d_auto) return fail(TP_E_STATE, "automorphism keys not imported");
+    if (len < 2 || len > c->N || (len & (len - 1))) return fail(TP_E_INVALID, "len must be a power of two in [2, N]");
+    return run_pack(c, len, len, lwes, perm, out);
+}
+
+int tp_pack_consecutive_device(tp_context* c, size_t len, const uint64_t* lwes, uint64_t* out) {
+    if (!c || !lwes || !out) return fail(TP_E_INVALID, "null argument");
+    if (!c->d_auto) return fail(TP_E_STATE, "automorphism keys not imported");
+    if (len < 1 || len > c->N) return fail(TP_E_INVALID, "len must lie in [1, N]");
+    return run_pack(c, c->N, len, lwes, nullptr, out);
+}
+
+int tp_import_rlwe_ksk(tp_context* c, const uint64_t* key, size_t words) {
+    if (!c || !key) return fail(TP_E_INVALID, "null argument");
+    if (!c->p.rlwe_ks_digits) return fail(TP_E_INVALID, "context has no RLWE gadget (rlwe_ks_digits = 0)");
+    const size_t want = (size_t)c->p.rlwe_ks_digits * 2 * c->N;
+    if (words != want) return fail(TP_E_INVALID, "RLWE key size mismatch: expected " + std::to_string(want));
+    int rc = set_device(c);
+    if (rc) return rc;
+    std::vector<uint64_t> dev = to_device_rows(key, words, c->N);
+    if (c->d_rksk) { hipFree(c->d_rksk); c->d_rksk = nullptr; }
+    if (hipMalloc(&c->d_rksk, words * 8) != hipSuccess) return fail(TP_E_NOMEM, "RLWE key allocation failed");
+    TP_HIP(hipMemcpy(c->d_rksk, dev.data(), words * 8, hipMemcpyHostToDevice));
+    return TP_OK;
+}
+
+int tp_rlwe_keyswitch_device(tp_context* c, size_t count, uint32_t t, int mode, int key, const uint64_t* in,
+                             uint64_t* out) {
+    if (!c || !in || !out) return fail(TP_E_INVALID, "null argument");
+    const int base_mode = mode & 3;
+    if ((mode & ~(3 | TP_RKS_ACCUMULATE)) || base_mode > TP_RKS_SWITCH_ONLY) return fail(TP_E_INVALID, "bad mode");
+    if (!(t & 1) || t >= 2 * c->N) return fail(TP_E_INVALID, "automorphism index must be odd and below 2N");
+    if (base_mode == TP_RKS_CONVERT && t != 1) return fail(TP_E_INVALID, "plain key switch takes t = 1");
+    const uint64_t* ksk = nullptr;
+    if (key == 0) {
+        if (!c->d_rksk) return fail(TP_E_STATE, "RLWE key not imported");
+        ksk = c->d_rksk;
+    } else {
+        if (key < 1 || key > c->logN) return fail(TP_E_INVALID, "automorphism key index must lie in [1, log2 N]");
+        if (!c->d_auto) return fail(TP_E_STATE, "automorphism keys not imported");
+        ksk = c->d_auto + (size_t)(key - 1) * c->p.rlwe_ks_digits * 2 * c->N;
+    }
+    if (count == 0) return TP_OK;
+    if (count > 0x7fffffff) return fail(TP_E_INVALID, "count too large");
+    int rc = set_device(c);
+    if (rc) return rc;
+    hipError_t e = tp::launch_rlwe_keyswitch(c->logN, auto_args(c), in, out, (int)count, t, base_mode,
+                                             (mode & TP_RKS_ACCUMULATE) ? 1 : 0, ksk, c->stream);
+    if (e != hipSuccess) return fail(TP_E_DEVICE, std::string("RLWE key switch launch: ") + hipGetErrorString(e));
+    return TP_OK;
+}
+
+int tp_eltwise_device(tp_context* c, int op, uint64_t* dst, const uint64_t* a, const uint64_t* b, const uint64_t* cc,
+                      uint64_t scalar, size_t words) {
+    if (!c || !dst || !a) return fail(TP_E_INVALID, "null argument");
+    if (op < TP_ELT_ADD || op > TP_ELT_REDUCE) return fail(TP_E_INVALID, "unknown elementwise op");
+    if ((op <= TP_ELT_FMA && !b) || (op == TP_ELT_FMA && !cc)) return fail(TP_E_INVALID, "missing operand");
+    if (op == TP_ELT_MULS && scalar >= gl::kQ) return fail(TP_E_INVALID, "scalar must be canonical");
+    int rc = set_device(c);
+    if (rc) return rc;
+    hipError_t e = tp::launch_eltwise(op, dst, a, b, cc, scalar, words, c->stream);
+    if (e != hipSuccess) return fail(TP_E_DEVICE, std::string("elementwise launch: ") + hipGetErrorString(e));
     return TP_OK;
 }
 

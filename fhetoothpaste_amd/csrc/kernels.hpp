@@ -60,8 +60,15 @@ hipError_t launch_automorphism(int logN, const KeySwitchArgs& A, const uint64_t*
                                uint32_t t, const uint64_t* ksk, hipStream_t s);
 hipError_t launch_trace(int logN, const KeySwitchArgs& A, const uint64_t* in, uint64_t* out, int count,
                         const uint64_t* keys, hipStream_t s);
+// Packing tree over len leaves; leaves whose input index is >= n_inputs are zero.
 hipError_t launch_pack(int logN, const KeySwitchArgs& A, const uint64_t* lwes, const uint32_t* perm, int len,
-                       uint64_t* nodes, const uint64_t* keys, hipStream_t s);
+                       int n_inputs, uint64_t* nodes, const uint64_t* keys, hipStream_t s);
+// mode 0 convert ([a coef | b NTT]), 1 automorphism ([a|b] coef), 2 automorphism of a only.
+hipError_t launch_rlwe_keyswitch(int logN, const KeySwitchArgs& A, const uint64_t* in, uint64_t* out, int count,
+                                uint32_t t, int mode, int accumulate, const uint64_t* ksk, hipStream_t s);
+// Elementwise mod-Q arithmetic (op codes TP_ELT_* of the C-ABI).
+hipError_t launch_eltwise(int op, uint64_t* dst, const uint64_t* a, const uint64_t* b, const uint64_t* c,
+                          uint64_t scalar, size_t words, hipStream_t s);
 
 hipError_t launch_ntt_batch(int logN, uint64_t* out, const uint64_t* in, size_t count, const uint64_t* tw,
                             bool inverse, hipStream_t s);

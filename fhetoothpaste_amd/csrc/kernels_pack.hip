@@ -134,6 +134,60 @@ __global__ void __launch_bounds__(Shape<LOGN>::P) automorphism_kernel(KeySwitchA
     }
 }
 
+// RLWE key switch with optional automorphism, one ciphertext per work-group:
+//   mode 0 (RLWEtoRLWEConverter::Convert, rlwe_conversion.cpp:224-249): in = [a coef | b NTT],
+//          r = (-sum NTT(sdigit_j(a)) K_j.a, b - sum NTT(sdigit_j(a)) K_j.b);
+//   mode 1 (AutomorphismEvaluator::Eval, automorphism_evaluation.cpp:150-159): in = [a|b] coef,
+//          a' = tau_t(a), b' = NTT(tau_t(b)), then as mode 0;
+//   mode 2 (AutomorphismEvaluator::EvalSwitchOnly, :171-176): a' = tau_t(a), b' = 0, then as mode 0.
+// out (NTT domain) = r, or out + r when accumulating (the reference subtracts into its output).
+template <int LOGN>
+__global__ void __launch_bounds__(Shape<LOGN>::P) rlwe_keyswitch_kernel(KeySwitchArgs A, const uint64_t* in,
+                                                                        uint64_t* out, int count, uint32_t t,
+                                                                        int mode, int accumulate,
+                                                                        const uint64_t* ksk) {
+    using S = Shape<LOGN>;
+    constexpr int N = S::N;
+    __shared__ uint64_t lds[S::LDS_WORDS];
+    const int L = threadIdx.x;
+    for (int ct = blockIdx.x; ct < count; ct += gridDim.x) {
+        const uint64_t* src = in + (size_t)ct * 2 * N;
+        uint64_t a[kE], b[kE];
+#pragma unroll
+        for (int k = 0; k < kE; ++k) a[k] = src[idx1<LOGN>(L, k)];
+        if (mode == 0) {
+#pragma unroll
+            for (int k = 0; k < kE; ++k) b[k] = src[N + idx3(L, k)];
+        } else {
+            automorphism_regs<LOGN>(a, lds, t, L);
+            if (mode == 1) {
+#pragma unroll
+                for (int k = 0; k < kE; ++k) b[k] = src[N + idx1<LOGN>(L, k)];
+                automorphism_regs<LOGN>(b, lds, t, L);
+                ntt_forward<LOGN>(b, lds, A.tw, L);
+            } else {
+#pragma unroll
+                for (int k = 0; k < kE; ++k) b[k] = 0;
+            }
+        }
+        uint64_t oa[kE], ob[kE];
+        rlwe_ks_core<LOGN>(oa, ob, a, lds, A, ksk, L);
+        uint64_t* dst = out + (size_t)ct * 2 * N;
+#pragma unroll
+        for (int k = 0; k < kE; ++k) {
+            uint64_t ra = gl::neg(gl::canon(oa[k]));
+            uint64_t rb = gl::sub(gl::canon(b[k]), gl::canon(ob[k]));
+            if (accumulate) {
+                ra = gl::add(dst[idx3(L, k)], ra);
+                rb = gl::add(dst[N + idx3(L, k)], rb);
+            }
+            dst[idx3(L, k)] = ra;
+            dst[N + idx3(L, k)] = rb;
+        }
+        poly_sync<S::WAVES>();
+    }
+}
+
 // Batched trace (trace_evaluation.cpp:135-162): o = in / N; for t = N+1, N/2+1, ..., 3:
 // o += INTT(Auto_t(o)); out = NTT(o).  in coefficients, out NTT domain.  keys: log2 N key blocks,
 // block j-1 for t = 2^j + 1.
@@ -186,9 +240,13 @@ __global__ void __launch_bounds__(Shape<LOGN>::P) trace_kernel(KeySwitchArgs A, 
 
 // Packing leaves (lwe_to_rlwe_packing.cpp:238-251): node k = L0Setup(lwes[perm((len-1) ^ brv(k))]):
 // a_0 = lwe_0, a_i = -lwe_(N-i), b_0 = lwe_N, all scaled by N^-1, other b coefficients 0.
+// Leaves whose input index is >= n_inputs are zero (PackConsecutively's "fake" nodes,
+// lwe_to_rlwe_packing.cpp:260-396: merging zeros yields zeros exactly, so the full tree over
+// them equals the reference's pruned stack walk).
 template <int LOGN>
 __global__ void __launch_bounds__(Shape<LOGN>::P) pack_leaf_kernel(const uint64_t* lwes, const uint32_t* perm,
-                                                                   uint64_t* nodes, int len, int log_len) {
+                                                                   uint64_t* nodes, int len, int log_len,
+                                                                   int n_inputs) {
     using S = Shape<LOGN>;
     constexpr int N = S::N;
     const int L = threadIdx.x;
@@ -197,9 +255,15 @@ __global__ void __launch_bounds__(Shape<LOGN>::P) pack_leaf_kernel(const uint64_
         uint32_t r = 0, x = (uint32_t)leaf;
         for (int b = 0; b < log_len; ++b) { r = (r << 1) | (x & 1); x >>= 1; }
         uint32_t src = (uint32_t)(len - 1) ^ r;
+        uint64_t* dst = nodes + (size_t)leaf * 2 * N;
+        if (src >= (uint32_t)n_inputs) {
+            // leaf past the inputs (consecutive packing): a zero RLWE, which every merge keeps exact
+#pragma unroll
+            for (int k = 0; k < kE; ++k) dst[idx1<LOGN>(L, k)] = dst[N + idx1<LOGN>(L, k)] = 0;
+            continue;
+        }
         if (perm) src = perm[src];
         const uint64_t* lwe = lwes + (size_t)src * (N + 1);
-        uint64_t* dst = nodes + (size_t)leaf * 2 * N;
 #pragma unroll
         for (int k = 0; k < kE; ++k) {
             const int i = idx1<LOGN>(L, k);
@@ -307,6 +371,13 @@ hipError_t launch_automorphism(int logN, const KeySwitchArgs& A, const uint64_t*
     return hipGetLastError();
 }
 
+hipError_t launch_rlwe_keyswitch(int logN, const KeySwitchArgs& A, const uint64_t* in, uint64_t* out, int count,
+                                uint32_t t, int mode, int accumulate, const uint64_t* ksk, hipStream_t s) {
+    TP_LOGN_SWITCH(logN, hipLaunchKernelGGL(rlwe_keyswitch_kernel<LG>, dim3(count), dim3(Shape<LG>::P), 0, s, A, in,
+                                            out, count, t, mode, accumulate, ksk));
+    return hipGetLastError();
+}
+
 hipError_t launch_trace(int logN, const KeySwitchArgs& A, const uint64_t* in, uint64_t* out, int count,
                         const uint64_t* keys, hipStream_t s) {
     TP_LOGN_SWITCH(logN, hipLaunchKernelGGL(trace_kernel<LG>, dim3(count), dim3(Shape<LG>::P), 0, s, A, in, out,
@@ -315,12 +386,13 @@ hipError_t launch_trace(int logN, const KeySwitchArgs& A, const uint64_t* in, ui
 }
 
 hipError_t launch_pack(int logN, const KeySwitchArgs& A, const uint64_t* lwes, const uint32_t* perm, int len,
-                       uint64_t* nodes, const uint64_t* keys, hipStream_t s) {
+                       int n_inputs, uint64_t* nodes, const uint64_t* keys, hipStream_t s) {
     int log_len = 0;
     while ((1 << log_len) < len) ++log_len;
     const size_t key_words = (size_t)A.digits * 2 * ((size_t)1 << logN);
     TP_LOGN_SWITCH(logN, {
-        hipLaunchKernelGGL(pack_leaf_kernel<LG>, dim3(len), dim3(Shape<LG>::P), 0, s, lwes, perm, nodes, len, log_len);
+        hipLaunchKernelGGL(pack_leaf_kernel<LG>, dim3(len), dim3(Shape<LG>::P), 0, s, lwes, perm, nodes, len, log_len,
+                           n_inputs);
         for (int lvl = 1; lvl <= log_len; ++lvl)
             hipLaunchKernelGGL(pack_merge_kernel<LG>, dim3(len >> lvl), dim3(Shape<LG>::P), 0, s, A, nodes,
                                len >> lvl, lvl, keys + (size_t)(lvl - 1) * key_words);

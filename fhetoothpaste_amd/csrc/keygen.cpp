@@ -200,3 +200,22 @@ extern "C" int tp_keygen_auto_keys(uint64_t* keys, const uint64_t* s, uint32_t N
     }
     return TP_OK;
 }
+
+extern "C" int tp_keygen_rlwe_ksk(uint64_t* key, const uint64_t* src, const uint64_t* dst, uint32_t N, uint32_t b,
+                                  uint32_t d, double sigma, uint64_t seed, uint64_t tag) {
+    if (!key || !src || !dst || b == 0 || b > 32 || d == 0 || N < 16 || (N & (N - 1))) return TP_E_INVALID;
+    tp::HostNtt T(N);
+    Rng g(seed, tag);
+    Gauss G(sigma);
+    std::vector<uint64_t> t_ntt(dst, dst + N), msg(N), e(N);
+    T.forward(t_ntt.data());
+    uint64_t gij = 1ULL << (64 - b);
+    for (uint32_t i = 0; i < d; ++i) {
+        for (uint32_t k = 0; k < N; ++k) msg[k] = gl::mul(src[k], gij);
+        T.forward(msg.data());
+        make_rlwe(T, key + (size_t)i * 2 * N, msg.data(), t_ntt.data(), g, G, e.data());
+        gij >>= b;
+        if (gij == 0) gij = 1;
+    }
+    return TP_OK;
+}

@@ -171,6 +171,39 @@ class Context:
         check(lib().tp_pack_device(self.h, x.shape[0], d_in.p, d_perm.p if d_perm else None, d_out.p))
         return d_out.to_host()
 
+    def import_rlwe_ksk(self, key):
+        key = np.ascontiguousarray(key, dtype=np.uint64)
+        check(lib().tp_import_rlwe_ksk(self.h, ptr(key), key.size))
+
+    def rlwe_keyswitch(self, rlwes, t=1, mode=0, key=0, out=None):
+        """[count][2N] -> [count][2N] NTT domain (mode and key as tp_rlwe_keyswitch_device); with
+        out given, the result is accumulated into a copy of it (the reference's semantics)."""
+        x = np.ascontiguousarray(rlwes, dtype=np.uint64).reshape(-1, 2 * self.N)
+        d_in = DeviceBuffer.from_host(self, x)
+        if out is not None:
+            d_out = DeviceBuffer.from_host(self, np.ascontiguousarray(out, dtype=np.uint64).reshape(x.shape))
+            mode |= T.TP_RKS_ACCUMULATE
+        else:
+            d_out = DeviceBuffer(self, x.size)
+        check(lib().tp_rlwe_keyswitch_device(self.h, x.shape[0], t, mode, key, d_in.p, d_out.p))
+        return d_out.to_host().reshape(x.shape)
+
+    def pack_consecutive(self, lwes):
+        x = np.ascontiguousarray(lwes, dtype=np.uint64).reshape(-1, self.N + 1)
+        d_in = DeviceBuffer.from_host(self, x)
+        d_out = DeviceBuffer(self, 2 * self.N)
+        check(lib().tp_pack_consecutive_device(self.h, x.shape[0], d_in.p, d_out.p))
+        return d_out.to_host()
+
+    def eltwise(self, op, a, b=None, c=None, scalar=0):
+        a = np.ascontiguousarray(a, dtype=np.uint64)
+        bufs = [DeviceBuffer.from_host(self, np.ascontiguousarray(v, dtype=np.uint64)) if v is not None else None
+                for v in (a, b, c)]
+        d_out = DeviceBuffer(self, a.size)
+        check(lib().tp_eltwise_device(self.h, op, d_out.p, bufs[0].p, bufs[1].p if bufs[1] else None,
+                                      bufs[2].p if bufs[2] else None, scalar, a.size))
+        return d_out.to_host().reshape(a.shape)
+
     def bootstrap_batch(self, lwes, tv):
         lwes = np.ascontiguousarray(lwes, dtype=np.uint64).reshape(-1, self.n + 1)
         tv = np.ascontiguousarray(tv, dtype=np.uint64)
@@ -206,6 +239,13 @@ def keygen_auto_keys(s, N, base_log2, digits, sigma, seed):
     out = np.empty(logN * digits * 2 * N, dtype=np.uint64)
     check(lib().tp_keygen_auto_keys(ptr(out), ptr(np.ascontiguousarray(s, np.uint64)), N, base_log2, digits, sigma,
                                     seed))
+    return out
+
+
+def keygen_rlwe_ksk(src, dst, N, base_log2, digits, sigma, seed, tag=6):
+    out = np.empty(digits * 2 * N, dtype=np.uint64)
+    check(lib().tp_keygen_rlwe_ksk(ptr(out), ptr(np.ascontiguousarray(src, np.uint64)),
+                                   ptr(np.ascontiguousarray(dst, np.uint64)), N, base_log2, digits, sigma, seed, tag))
     return out
 
 

@@ -105,6 +105,49 @@ int tp_trace_device(tp_context* ctx, size_t count, const uint64_t* in, uint64_t*
  * context scratch; the tree levels are launched in order on the context stream. */
 int tp_pack_device(tp_context* ctx, size_t len, const uint64_t* lwes, const uint32_t* perm, uint64_t* out);
 
+/* RLWE->RLWE key switching with an imported key, optionally after the automorphism X -> X^t.
+ * tp_import_rlwe_ksk: RLWEtoRLWEConverter key [d][2N], NTT domain, d = rlwe_ks_digits, built by
+ * RLWEtoRLWEConverter::KeyGen (rlwe_conversion.cpp:187-218) or tp_keygen_rlwe_ksk.
+ * tp_rlwe_keyswitch_device, count ciphertexts [count][2N] -> [count][2N] NTT domain:
+ *   TP_RKS_CONVERT      in = [a coef | b NTT] (t must be 1): RLWEtoRLWEConverter::Convert
+ *                       (rlwe_conversion.cpp:224-249)
+ *   TP_RKS_AUTOMORPHISM in = [a|b] coefficients: AutomorphismEvaluator::Eval
+ *                       (automorphism_evaluation.cpp:150-159)
+ *   TP_RKS_SWITCH_ONLY  in = [a|.] coefficients, b ignored: AutomorphismEvaluator::EvalSwitchOnly
+ *                       (automorphism_evaluation.cpp:171-176)
+ *   | TP_RKS_ACCUMULATE out += result, the reference's subtract-into-output semantics;
+ *                       without it out = result.
+ * key: 0 = the imported RLWE key, j in [1, log2 N] = automorphism key j (t = 2^j + 1) of
+ * tp_import_auto_keys.  t odd, 1 <= t < 2N. */
+#define TP_RKS_CONVERT 0
+#define TP_RKS_AUTOMORPHISM 1
+#define TP_RKS_SWITCH_ONLY 2
+#define TP_RKS_ACCUMULATE 4
+int tp_import_rlwe_ksk(tp_context* ctx, const uint64_t* key, size_t words);
+int tp_rlwe_keyswitch_device(tp_context* ctx, size_t count, uint32_t t, int mode, int key, const uint64_t* in,
+                             uint64_t* out);
+
+/* LWEtoRLWEPacker::PackConsecutively (lwe_to_rlwe_packing.cpp:260-396): len LWEs (dimension N,
+ * 1 <= len <= N, [len][N+1]) -> one RLWE in coefficients whose coefficient i holds message i for
+ * i < len.  The reference walks the full N-leaf tree with a stack and prunes subtrees without
+ * inputs; here the tree runs level by level with zero leaves for them (merging zeros is exact).
+ * Uses N*2N words of context scratch. */
+int tp_pack_consecutive_device(tp_context* ctx, size_t len, const uint64_t* lwes, uint64_t* out);
+
+/* Elementwise mod-Q arithmetic over `words` words (device pointers, inputs canonical, outputs
+ * canonical, dst may alias an input): MathWorker::{AddModElw, SubModElw, MulModElw, FMAModElw,
+ * MulSModElw, ReduceModElw} and their Eq forms (lib/include/backend/backend.h:35-55).
+ *   TP_ELT_ADD dst = a + b    TP_ELT_SUB dst = a - b     TP_ELT_MUL dst = a * b
+ *   TP_ELT_FMA dst = a*b + c  TP_ELT_MULS dst = a * scalar   TP_ELT_REDUCE dst = a mod Q (any a) */
+#define TP_ELT_ADD 0
+#define TP_ELT_SUB 1
+#define TP_ELT_MUL 2
+#define TP_ELT_FMA 3
+#define TP_ELT_MULS 4
+#define TP_ELT_REDUCE 5
+int tp_eltwise_device(tp_context* ctx, int op, uint64_t* dst, const uint64_t* a, const uint64_t* b,
+                      const uint64_t* c, uint64_t scalar, size_t words);
+
 /* Negacyclic NTT over Z_Q[X]/(X^N+1) of `count` polynomials (device pointers).
  * Replace SolinasWorker::ForwardNTT / BackwardNTT (lib/src/backend/solinas.cpp:228-313);
  * dst may equal src. */
@@ -217,6 +260,10 @@ int tp_keygen_lwe_ksk(uint64_t* ksk, const uint64_t* src, const uint64_t* dst, u
  * keys [log2 N][d][2N]. */
 int tp_keygen_auto_keys(uint64_t* keys, const uint64_t* s, uint32_t N, uint32_t base_log2, uint32_t digits,
                         double sigma, uint64_t seed);
+/* RLWEtoRLWEConverter key from src to dst (coefficients, N each): row i = RLWE_dst(g_i src),
+ * g_i = 2^(64 - b(i+1)) (kept >= 1), out [d][2N] NTT domain; stream (seed, tag). */
+int tp_keygen_rlwe_ksk(uint64_t* key, const uint64_t* src, const uint64_t* dst, uint32_t N, uint32_t base_log2,
+                       uint32_t digits, double sigma, uint64_t seed, uint64_t tag);
 
 #ifdef __cplusplus
 }

@@ -903,6 +903,65 @@ OR_EXPORT int or_pack(const or_ring* R, uint64_t* out, const uint64_t* lwes, uin
     return 0;
 }
 
+/* LWEtoRLWEPacker::PackConsecutively (lwe_to_rlwe_packing.cpp:260-396): the N-leaf tree walked
+ * with a stack, leaf k reading input (N-1) ^ brv(k) when that index is below len and being a
+ * "fake" node otherwise; two fake nodes merge into a fake node without work, any other pair
+ * merges as in Pack (earlier node = left/neck, later = right/head).  Output: stack bottom once it
+ * reaches level log2 N, coefficients. */
+OR_EXPORT int or_pack_consecutive(const or_ring* R, uint64_t* out, const uint64_t* lwes, uint32_t len,
+                                  const uint64_t* keys, uint32_t b, uint32_t d) {
+    const uint32_t N = R->N, logN = R->logN;
+    if (len < 1 || len > N) return -1;
+    const uint32_t depth = logN + 2;
+    uint64_t* st = (uint64_t*)calloc((size_t)depth * 2 * N, sizeof(uint64_t));
+    uint64_t* rot = (uint64_t*)malloc(sizeof(uint64_t) * 4 * N);
+    uint64_t* aut = rot + 2 * N;
+    uint32_t lvl[64], fake[64];
+    int top = -1;
+    uint32_t next = 0;
+    for (;;) {
+        if (top == 0 && lvl[0] == logN) break;
+        if (top >= 1 && lvl[top] == lvl[top - 1]) {
+            const uint32_t o = lvl[top] + 1;
+            if (fake[top] && fake[top - 1]) {
+                --top;
+                lvl[top] = o;
+                continue;
+            }
+            uint64_t* neck = st + (size_t)(top - 1) * 2 * N;
+            uint64_t* head = st + (size_t)top * 2 * N;
+            rot_poly2(rot, neck, N >> o, N);
+            vadd(neck, rot, head, 2 * N);   /* head + X^s neck */
+            vsub(head, head, rot, 2 * N);   /* head - X^s neck */
+            or_automorphism(R, aut, head, (1ULL << o) + 1, keys + (size_t)(o - 1) * d * 2 * N, b, d);
+            ntt_inv(R, aut);
+            ntt_inv(R, aut + N);
+            vadd(neck, neck, aut, 2 * N);
+            --top;
+            lvl[top] = o;
+            fake[top] = 0;
+            continue;
+        }
+        /* push the next leaf */
+        ++top;
+        uint64_t* slot = st + (size_t)top * 2 * N;
+        const uint32_t src = (N - 1) ^ brv(next, logN);
+        if (src < len) {
+            pack_l0(R, slot, lwes + (size_t)src * (N + 1));
+            fake[top] = 0;
+        } else {
+            memset(slot, 0, sizeof(uint64_t) * 2 * N);
+            fake[top] = 1;
+        }
+        lvl[top] = 0;
+        ++next;
+    }
+    memcpy(out, st, sizeof(uint64_t) * 2 * N);
+    free(st);
+    free(rot);
+    return 0;
+}
+
 /* ------------------------------------------------------------------------ */
 /* Batched bootstrap for the CPU baseline: EvalFunc (negacyclic_function_    */
 /* evaluator.cpp:92-101) + LWE key switch, one rotator per thread.          */

@@ -351,3 +351,37 @@ def test_pack_noise_free(oracle, length):
     for j in range(N):
         want = int(msgs[j // stride]) if j % stride == 0 else 0
         assert ph[j] == want, j
+
+
+def _noise_free_lwes(oracle, N, s, msgs, seed):
+    rng = np.random.default_rng(seed)
+    lwes = np.zeros((len(msgs), N + 1), dtype=np.uint64)
+    for i, m in enumerate(msgs):
+        a = rng.integers(0, Q, size=N, dtype=np.uint64)
+        lwes[i, :N] = a
+        lwes[i, N] = (int(m) + sum(int(x) * int(y) for x, y in zip(a, s))) % Q
+    return lwes
+
+
+@pytest.mark.parametrize("length", [1, 3, 16, 77, 128])
+def test_pack_consecutive_noise_free(oracle, length):
+    """PackConsecutively: coefficient i holds message i for i < len and 0 beyond (packing_tests.cpp:99-104)."""
+    N, b, d = 128, 16, 4
+    s = oracle.secret(N, 0, 13, 2)
+    keys = oracle.auto_keys(N, s, b, d, 0.0, 13)
+    msgs = np.random.default_rng(length).integers(0, Q, size=length, dtype=np.uint64)
+    lwes = _noise_free_lwes(oracle, N, s, msgs, length + 100)
+    out = oracle.pack_consecutive(N, lwes, length, keys, b, d)
+    R = oracle.Ring(N)
+    ph = _phase_coef(oracle, N, np.concatenate([R.ntt(out[:N]), R.ntt(out[N:])]), s)
+    for j in range(N):
+        assert ph[j] == (int(msgs[j]) if j < length else 0), j
+
+
+def test_pack_consecutive_full_equals_pack(oracle):
+    """With len = N nothing is pruned: the stack walk equals Pack's level order, bit for bit (with noise)."""
+    N, b, d = 128, 16, 4
+    s = oracle.secret(N, 0, 14, 2)
+    keys = oracle.auto_keys(N, s, b, d, 3.2, 14)
+    lwes = np.random.default_rng(9).integers(0, Q, size=(N, N + 1), dtype=np.uint64)
+    assert np.array_equal(oracle.pack_consecutive(N, lwes, N, keys, b, d), oracle.pack(N, lwes, N, keys, b, d))
