@@ -44,6 +44,7 @@ SIGNATURES = {
     "tp_context_create": [ctypes.POINTER(vp), ctypes.POINTER(Params), INT],
     "tp_context_destroy": [vp],
     "tp_context_sync": [vp],
+    "tp_context_params": [vp, ctypes.POINTER(Params)],
     "tp_import_brk": [vp, u64p, SZ],
     "tp_import_lwe_ksk": [vp, u64p, SZ],
     "tp_ntt_forward_device": [vp, vp, vp, SZ],

@@ -9,6 +9,7 @@ from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
+ROOT = os.path.dirname(HERE)
 OUT = os.path.join(HERE, "libtoothpaste_amd.so")
 BUILD_DIR = os.path.join(HERE, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -44,6 +45,10 @@ def build(verbose=True):
     tmp = OUT + ".tmp"
     subprocess.check_call([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs)
     os.replace(tmp, OUT)
+    # C++ program that drives the drop-in classes for the parity tests (tests/test_gpu_dropin.py)
+    parity = os.path.join(ROOT, "tests", "cpp", "dropin_parity")
+    subprocess.check_call([HIPCC, "-O2", "-std=c++17", "-o", parity, os.path.join(ROOT, "tests", "cpp", "dropin_parity.cpp"),
+                           "-L" + HERE, "-ltoothpaste_amd", "-Wl,-rpath,$ORIGIN/../../fhetoothpaste_amd"])
     probe = os.path.join(HERE, "issue_rate_probe")
     subprocess.check_call([HIPCC, "--offload-arch=gfx950", "-O3", "-o", probe,
                            os.path.join(CSRC, "issue_rate_probe.hip")])

@@ -211,9 +211,13 @@ int tp_context_create(tp_context** out, const tp_params* p, int device) {
         tp_context_destroy(c);
         return fail(TP_E_NOMEM, "table allocation failed");
     }
-    hipMemcpy(c->d_tw, T.psi_rev.data(), N * 8, hipMemcpyHostToDevice);
-    hipMemcpy(c->d_itw, T.psi_inv_rev.data(), N * 8, hipMemcpyHostToDevice);
-    for (auto& ev : c->ev) hipEventCreate(&ev);
+    bool ok = hipMemcpy(c->d_tw, T.psi_rev.data(), N * 8, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(c->d_itw, T.psi_inv_rev.data(), N * 8, hipMemcpyHostToDevice) == hipSuccess;
+    for (auto& ev : c->ev) ok = ok && hipEventCreate(&ev) == hipSuccess;
+    if (!ok) {
+        tp_context_destroy(c);
+        return fail(TP_E_DEVICE, "context set-up failed");
+    }
     {
         // signed-digit corrector (gadget_decomp.h:16-28)
         const uint32_t b = p->br_base_log2;
@@ -243,7 +247,10 @@ int tp_context_create(tp_context** out, const tp_params* p, int device) {
             tp_context_destroy(c);
             return fail(TP_E_NOMEM, "monomial table allocation failed");
         }
-        hipMemcpy(c->d_mono, mono.data(), mono.size() * 8, hipMemcpyHostToDevice);
+        if (hipMemcpy(c->d_mono, mono.data(), mono.size() * 8, hipMemcpyHostToDevice) != hipSuccess) {
+            tp_context_destroy(c);
+            return fail(TP_E_DEVICE, "monomial table upload failed");
+        }
     }
     *out = c;
     return TP_OK;
@@ -251,15 +258,16 @@ int tp_context_create(tp_context** out, const tp_params* p, int device) {
 
 int tp_context_destroy(tp_context* c) {
     if (!c) return TP_OK;
-    hipSetDevice(c->device);
-    if (c->stream) hipStreamSynchronize(c->stream);
+    // teardown is best effort: errors here have no caller to report to
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (uint64_t* p : {c->d_tw, c->d_itw, c->d_brk, c->d_ksk, c->d_corr, c->d_scratch, c->d_io, c->d_mono,
                         c->d_auto, c->d_nodes, c->d_rksk})
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);
     for (auto& ev : c->ev)
-        if (ev) hipEventDestroy(ev);
-    if (c->d_perm) hipFree(c->d_perm);
-    if (c->stream) hipStreamDestroy(c->stream);
+        if (ev) (void)hipEventDestroy(ev);
+    if (c->d_perm) (void)hipFree(c->d_perm);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return TP_OK;
 }
@@ -267,6 +275,12 @@ int tp_context_destroy(tp_context* c) {
 int tp_context_sync(tp_context* c) {
     if (!c) return fail(TP_E_INVALID, "null context");
     TP_HIP(hipStreamSynchronize(c->stream));
+    return TP_OK;
+}
+
+int tp_context_params(tp_context* c, tp_params* out) {
+    if (!c || !out) return fail(TP_E_INVALID, "null argument");
+    *out = c->p;
     return TP_OK;
 }
 
@@ -286,7 +300,7 @@ int tp_import_brk(tp_context* c, const uint64_t* brk, size_t words) {
         for (uint32_t L = 0; L < P; ++L)
             for (uint32_t k = 0; k < 16; ++k) dst[(k >> 1) * 2 * P + 2 * L + (k & 1)] = src[16 * L + k];
     }
-    if (c->d_brk) { hipFree(c->d_brk); c->d_brk = nullptr; }
+    if (c->d_brk) { TP_HIP(hipFree(c->d_brk)); c->d_brk = nullptr; }
     if (hipMalloc(&c->d_brk, words * 8) != hipSuccess) return fail(TP_E_NOMEM, "brk allocation failed");
     TP_HIP(hipMemcpy(c->d_brk, dev.data(), words * 8, hipMemcpyHostToDevice));
     c->brk_words = words;
@@ -320,8 +334,8 @@ int tp_import_lwe_ksk(tp_context* c, const uint64_t* ksk, size_t words) {
     while (shift > 0) { C += 1ULL << (shift + b - 1); shift -= b; }
     C += 1ULL << (shift + b - 1);
     c->ks_corrector = C % gl::kQ;
-    if (c->d_ksk) { hipFree(c->d_ksk); c->d_ksk = nullptr; }
-    if (c->d_corr) { hipFree(c->d_corr); c->d_corr = nullptr; }
+    if (c->d_ksk) { TP_HIP(hipFree(c->d_ksk)); c->d_ksk = nullptr; }
+    if (c->d_corr) { TP_HIP(hipFree(c->d_corr)); c->d_corr = nullptr; }
     if (hipMalloc(&c->d_ksk, (words + tp::kKsPadWords) * 8) != hipSuccess ||
         hipMalloc(&c->d_corr, W * 8) != hipSuccess)
         return fail(TP_E_NOMEM, "ksk allocation failed");
@@ -353,7 +367,7 @@ int tp_import_auto_keys(tp_context* c, const uint64_t* keys, size_t words) {
     int rc = set_device(c);
     if (rc) return rc;
     std::vector<uint64_t> dev = to_device_rows(keys, words, c->N);
-    if (c->d_auto) { hipFree(c->d_auto); c->d_auto = nullptr; }
+    if (c->d_auto) { TP_HIP(hipFree(c->d_auto)); c->d_auto = nullptr; }
     if (hipMalloc(&c->d_auto, words * 8) != hipSuccess) return fail(TP_E_NOMEM, "automorphism key allocation failed");
     TP_HIP(hipMemcpy(c->d_auto, dev.data(), words * 8, hipMemcpyHostToDevice));
     return TP_OK;
@@ -430,7 +444,7 @@ int tp_import_rlwe_ksk(tp_context* c, const uint64_t* key, size_t words) {
     int rc = set_device(c);
     if (rc) return rc;
     std::vector<uint64_t> dev = to_device_rows(key, words, c->N);
-    if (c->d_rksk) { hipFree(c->d_rksk); c->d_rksk = nullptr; }
+    if (c->d_rksk) { TP_HIP(hipFree(c->d_rksk)); c->d_rksk = nullptr; }
     if (hipMalloc(&c->d_rksk, words * 8) != hipSuccess) return fail(TP_E_NOMEM, "RLWE key allocation failed");
     TP_HIP(hipMemcpy(c->d_rksk, dev.data(), words * 8, hipMemcpyHostToDevice));
     return TP_OK;

@@ -377,12 +377,12 @@ void destroy_sets(tp_server* S) {
     for (tp_server::Set& st : S->sets) {
         if (!st.ctx) continue;
         tp::bind_device(st.ctx);
-        hipStreamSynchronize(tp::context_stream(st.ctx));
+        (void)hipStreamSynchronize(tp::context_stream(st.ctx));  // teardown: best effort
         for (void* p : {(void*)st.d_lwe, (void*)st.d_state, (void*)st.d_ext, (void*)st.d_out, (void*)st.d_tv,
                         (void*)st.d_ctrl})
-            if (p) hipFree(p);
+            if (p) (void)hipFree(p);
         for (void* p : {(void*)st.h_ctrl, (void*)st.h_in, (void*)st.h_out})
-            if (p) hipHostFree(p);
+            if (p) (void)hipHostFree(p);
     }
 }
 

@@ -73,6 +73,8 @@ int tp_device_count(int* count);
 int tp_context_create(tp_context** out, const tp_params* params, int device);
 int tp_context_destroy(tp_context* ctx);
 int tp_context_sync(tp_context* ctx);
+/* Parameters a context was created with. */
+int tp_context_params(tp_context* ctx, tp_params* out);
 
 /* Import the blind-rotation key in the reference byte layout (host memory).  Replaces the
  * key material built by CGGIBlindRotator::KeyGen (cggi_blind_rotation.cpp:215-311) and
@@ -114,7 +116,7 @@ int tp_pack_device(tp_context* ctx, size_t len, const uint64_t* lwes, const uint
  *   TP_RKS_AUTOMORPHISM in = [a|b] coefficients: AutomorphismEvaluator::Eval
  *                       (automorphism_evaluation.cpp:150-159)
  *   TP_RKS_SWITCH_ONLY  in = [a|.] coefficients, b ignored: AutomorphismEvaluator::EvalSwitchOnly
- *                       (automorphism_evaluation.cpp:171-176)
+ *                       (automorphism_evaluation.cpp:172-177)
  *   | TP_RKS_ACCUMULATE out += result, the reference's subtract-into-output semantics;
  *                       without it out = result.
  * key: 0 = the imported RLWE key, j in [1, log2 N] = automorphism key j (t = 2^j + 1) of
