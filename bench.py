@@ -83,7 +83,9 @@ def serving_leg(args, tp, ctx_bin):
     ctx_ter = tp.Context(TER["N"], TER["n"], TER["b"], TER["l"], TER["method"], TER["n_out"], TER["kb"], TER["kd"])
     ctx_ter.import_brk(tp.keygen_brk(lk3, rk3, TER["N"], TER["n"], TER["b"], TER["l"], TER["method"], 1, 3.2, seed))
     ctx_ter.import_lwe_ksk(tp.keygen_lwe_ksk(rk3, lk3, TER["N"], TER["n_out"], TER["kb"], TER["kd"], 3.2, seed))
-    log("serving keys ready (%.1f s)" % (time.time() - t0))
+    tune = ctx_ter.autotune(4096)
+    log("serving keys ready (%.1f s); ternary set variant %d wave(s)/SIMD (two: %.1f ms, one: %.1f ms)" % (
+        time.time() - t0, tune["waves"], tune["ms_two_waves"], tune["ms_one_wave"]))
 
     rng = np.random.default_rng(SEED + 5)
     dims = [(N, n), (TER["N"], TER["n"])]
@@ -133,6 +135,7 @@ def serving_leg(args, tp, ctx_bin):
         "throughput_bootstraps_per_s": round(cts / span, 1),
         "p50_ms": round(1000 * float(np.percentile(lat, 50)), 2),
         "p99_ms": round(1000 * float(np.percentile(lat, 99)), 2),
+        "ternary_br_waves_per_simd": tune["waves"],
         "chunk_steps": args.serve_chunk, "launches": st["launches"], "iterations": st["iterations"],
         "counts": "lognormal median 32 sigma_ln 1.0 clipped to [1, 512]",
         "lut": "random 3-bit, f(p) = v_floor(8p/N) * floor(q/16)",
@@ -182,6 +185,9 @@ def main():
     ctx = tp.Context(N, n, BB, L, 0, KS_OUT, KS_B, KS_D)
     ctx.import_brk(brk)
     ctx.import_lwe_ksk(ksk)
+    tune = ctx.autotune(args.batch)
+    log("blind-rotation variant: %d wave(s)/SIMD (two: %.1f ms, one: %.1f ms)" % (
+        tune["waves"], tune["ms_two_waves"], tune["ms_one_wave"]))
 
     B = args.batch
     rng = np.random.default_rng(SEED)
@@ -286,6 +292,8 @@ def main():
                      "kernel": "br_kernel<10,3,binary>", "work_per_launch_mulmod": w_br * B,
                      "hbm_key_stream_GBps": round(key_bytes / (br_avg_s + 1e-3 * sum(ks_ms) / len(ks_ms)) / 1e9, 2)},
         "keygen_s": round(keygen_s, 2),
+        "tuning": {"br_waves_per_simd": tune["waves"], "br_ms_two_waves": round(tune["ms_two_waves"], 2),
+                   "br_ms_one_wave": round(tune["ms_one_wave"], 2)},
     }
 
     if args.serve_seconds > 0:

@@ -15,6 +15,7 @@ Q = 0xFFFFFFFF00000001
 TP_OK, TP_E_INVALID, TP_E_DEVICE, TP_E_STATE, TP_E_NOMEM, TP_E_TIMEOUT = 0, -1, -2, -3, -4, -5
 METHOD_CGGI_BINARY, METHOD_CGGI_TERNARY, METHOD_BMMP = 0, 1, 2
 TP_RKS_CONVERT, TP_RKS_AUTOMORPHISM, TP_RKS_SWITCH_ONLY, TP_RKS_ACCUMULATE = 0, 1, 2, 4
+TP_OPT_BR_WAVES = 1
 TP_ELT_ADD, TP_ELT_SUB, TP_ELT_MUL, TP_ELT_FMA, TP_ELT_MULS, TP_ELT_REDUCE = 0, 1, 2, 3, 4, 5
 
 u64p = ctypes.POINTER(ctypes.c_uint64)
@@ -69,6 +70,9 @@ SIGNATURES = {
     "tp_trace_device": [vp, SZ, vp, vp],
     "tp_pack_device": [vp, SZ, vp, vp, vp],
     "tp_import_rlwe_ksk": [vp, u64p, SZ],
+    "tp_set_option": [vp, INT, INT],
+    "tp_get_option": [vp, INT, ctypes.POINTER(INT)],
+    "tp_autotune": [vp, SZ, ctypes.POINTER(ctypes.c_float)],
     "tp_rlwe_keyswitch_device": [vp, SZ, U32, INT, INT, vp, vp],
     "tp_pack_consecutive_device": [vp, SZ, vp, vp],
     "tp_eltwise_device": [vp, INT, vp, vp, vp, vp, U64, SZ],

@@ -21,6 +21,7 @@ HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall"]
 
 SOURCES = [
     ("kernels.hip", DEVICE_FLAGS),
+    ("kernels_br1.hip", DEVICE_FLAGS),
     ("kernels_pack.hip", DEVICE_FLAGS),
     ("kernels_eltwise.hip", DEVICE_FLAGS),
     ("capi.cpp", HOST_FLAGS),
@@ -40,7 +41,7 @@ def _compile(src, flags):
 def build(verbose=True):
     os.makedirs(BUILD_DIR, exist_ok=True)
     srcs = [(s, f) for s, f in SOURCES if os.path.exists(os.path.join(CSRC, s))]
-    with ThreadPoolExecutor(max_workers=4) as ex:
+    with ThreadPoolExecutor(max_workers=6) as ex:
         objs = list(ex.map(lambda sf: _compile(*sf), srcs))
     tmp = OUT + ".tmp"
     subprocess.check_call([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs)

@@ -63,6 +63,7 @@ struct tp_context {
     uint32_t* d_perm = nullptr;
     size_t perm_len = 0;
     uint64_t* d_rksk = nullptr;  // RLWEtoRLWEConverter key, d x 2N, key-row order
+    int br_waves = 2;            // blind-rotation variant: waves per SIMD (tp_set_option / tp_autotune)
     hipEvent_t ev[8] = {};
 };
 
@@ -145,7 +146,8 @@ int launch_eval(tp_context* c, size_t count, const uint64_t* lwes, const uint64_
     tp::BrArgs A{};
     fill_br_args(c, A, lwes, tv, tv_stride, out_lwe, out_rlwe, rlwe_ntt, count);
     if (grid <= 0) grid = (int)count;
-    hipError_t e = tp::launch_blind_rotate(c->logN, (int)c->p.method, (int)c->p.br_digits, A, grid, c->stream);
+    hipError_t e =
+        tp::launch_blind_rotate(c->logN, (int)c->p.method, (int)c->p.br_digits, A, grid, c->br_waves, c->stream);
     if (e != hipSuccess) return fail(TP_E_DEVICE, std::string("blind rotation launch: ") + hipGetErrorString(e));
     return TP_OK;
 }
@@ -585,7 +587,8 @@ int launch_chunk(tp_context* c, size_t count, const uint32_t* active, uint32_t* 
     A.state = state;
     A.tv_index = tv_index;
     A.chunk_steps = (int)chunk_steps;
-    hipError_t e = tp::launch_blind_rotate(c->logN, (int)c->p.method, (int)c->p.br_digits, A, (int)count, c->stream);
+    hipError_t e = tp::launch_blind_rotate(c->logN, (int)c->p.method, (int)c->p.br_digits, A, (int)count, c->br_waves,
+                                           c->stream);
     if (e != hipSuccess) return fail(TP_E_DEVICE, std::string("blind rotation launch: ") + hipGetErrorString(e));
     return TP_OK;
 }
@@ -611,6 +614,82 @@ int tp_bootstrap_batch_device(tp_context* c, size_t count, const uint64_t* lwes,
     rc = launch_eval(c, count, lwes, tv, tv_stride, c->d_scratch, nullptr, 0, 0);
     if (rc) return rc;
     return tp_lwe_keyswitch_batch_device(c, count, c->d_scratch, c->N + 1, out);
+}
+
+int tp_set_option(tp_context* c, int option, int value) {
+    if (!c) return fail(TP_E_INVALID, "null context");
+    if (option != TP_OPT_BR_WAVES) return fail(TP_E_INVALID, "unknown option");
+    if (value != 1 && value != 2) return fail(TP_E_INVALID, "waves per SIMD must be 1 or 2");
+    if (value == 1 && c->p.br_digits != 3 && c->p.br_digits != 4)
+        return fail(TP_E_INVALID, "the one-wave variant is built for 3 and 4 digits");
+    c->br_waves = value;
+    return TP_OK;
+}
+
+int tp_get_option(tp_context* c, int option, int* value) {
+    if (!c || !value) return fail(TP_E_INVALID, "null argument");
+    if (option != TP_OPT_BR_WAVES) return fail(TP_E_INVALID, "unknown option");
+    *value = c->br_waves;
+    return TP_OK;
+}
+
+int tp_autotune(tp_context* c, size_t count, float* ms) {
+    if (!c) return fail(TP_E_INVALID, "null context");
+    if (!c->d_brk) return fail(TP_E_STATE, "blind-rotation key not imported");
+    if (count == 0 || count > (1u << 20)) return fail(TP_E_INVALID, "count must lie in [1, 2^20]");
+    int rc = set_device(c);
+    if (rc) return rc;
+    const uint32_t N = c->N, n = c->p.lwe_dim;
+    // synthetic inputs: LWE entries in [0, 2N) from a fixed SplitMix64 stream, zero test vector
+    std::vector<uint64_t> lwes(count * (n + 1));
+    uint64_t st = 0x5DEECE66DULL;
+    for (auto& x : lwes) {
+        uint64_t z = (st += 0x9E3779B97F4A7C15ULL);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+        x = (z ^ (z >> 31)) % (2 * (uint64_t)N);
+    }
+    uint64_t *d_lwe = nullptr, *d_tv = nullptr, *d_out = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    float best = -1.0f;
+    int best_w = c->br_waves;
+    const int saved = c->br_waves;
+    rc = TP_OK;
+    if (hipMalloc(&d_lwe, lwes.size() * 8) != hipSuccess || hipMalloc(&d_tv, 2 * (size_t)N * 8) != hipSuccess ||
+        hipMalloc(&d_out, count * (N + 1) * 8) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+        hipEventCreate(&e1) != hipSuccess) {
+        rc = fail(TP_E_NOMEM, "autotune buffers");
+    }
+    if (!rc && (hipMemcpyAsync(d_lwe, lwes.data(), lwes.size() * 8, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+                hipMemsetAsync(d_tv, 0, 2 * (size_t)N * 8, c->stream) != hipSuccess))
+        rc = fail(TP_E_DEVICE, "autotune upload");
+    for (int w = 2; w >= 1 && !rc; --w) {
+        float t = -1.0f;
+        if (w == 1 && c->p.br_digits != 3 && c->p.br_digits != 4) {
+            if (ms) ms[2 - w] = t;
+            continue;
+        }
+        c->br_waves = w;
+        rc = launch_eval(c, count, d_lwe, d_tv, 0, d_out, nullptr, 0, 0);  // warm-up
+        if (!rc && (hipEventRecord(e0, c->stream) != hipSuccess))
+            rc = fail(TP_E_DEVICE, "event record");
+        if (!rc) rc = launch_eval(c, count, d_lwe, d_tv, 0, d_out, nullptr, 0, 0);
+        if (!rc && (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                    hipEventElapsedTime(&t, e0, e1) != hipSuccess))
+            rc = fail(TP_E_DEVICE, "autotune timing");
+        if (ms) ms[2 - w] = t;
+        if (!rc && (best < 0 || t < best)) {
+            best = t;
+            best_w = w;
+        }
+    }
+    c->br_waves = rc ? saved : best_w;
+    (void)hipStreamSynchronize(c->stream);
+    for (void* p : {(void*)d_lwe, (void*)d_tv, (void*)d_out})
+        if (p) (void)hipFree(p);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    return rc;
 }
 
 int tp_malloc(tp_context* c, uint64_t** ptr, size_t words) {

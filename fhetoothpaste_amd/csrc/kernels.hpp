@@ -72,7 +72,10 @@ hipError_t launch_eltwise(int op, uint64_t* dst, const uint64_t* a, const uint64
 
 hipError_t launch_ntt_batch(int logN, uint64_t* out, const uint64_t* in, size_t count, const uint64_t* tw,
                             bool inverse, hipStream_t s);
-hipError_t launch_blind_rotate(int logN, int method, int digits, const BrArgs& A, int grid, hipStream_t s);
+// waves: 2 (256-register variants) or 1 (512-register variants, digits 3 and 4 only).
+hipError_t launch_blind_rotate(int logN, int method, int digits, const BrArgs& A, int grid, int waves,
+                               hipStream_t s);
+hipError_t launch_blind_rotate_w1(int logN, int method, int digits, const BrArgs& A, int grid, hipStream_t s);
 hipError_t launch_lwe_keyswitch(const KsArgs& A, hipStream_t s);
 constexpr int kKsPadWords = 16;  // zero words after the device key-switching key
 

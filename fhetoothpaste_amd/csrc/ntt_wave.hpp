@@ -14,9 +14,23 @@
 // A level with m blocks uses twiddle psi_rev[m + block] (psi_inv_rev for the inverse).
 #pragma once
 #include <hip/hip_runtime.h>
+#include <utility>
 #include "gl64.hpp"
 
 namespace tp {
+
+// f(integral_constant<int, 0>) ... f(integral_constant<int, N-1>): an unrolled loop whose index
+// is a compile-time constant in every copy, whatever the unroller decides.  Register arrays
+// indexed by it stay in registers (a loop the unroller gives up on would index them at run time
+// and move them to scratch memory).
+template <class F, int... I>
+__device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, I...>) {
+    (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+    static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
 
 constexpr int kE = 16;  // coefficients per lane
 

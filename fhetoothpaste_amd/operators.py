@@ -171,6 +171,18 @@ class Context:
         check(lib().tp_pack_device(self.h, x.shape[0], d_in.p, d_perm.p if d_perm else None, d_out.p))
         return d_out.to_host()
 
+    def autotune(self, count):
+        """Time the blind-rotation variants on `count` synthetic ciphertexts and keep the faster;
+        returns {"waves": chosen, "ms_two_waves": .., "ms_one_wave": .. (-1 if not built)}."""
+        ms = (ctypes.c_float * 2)()
+        check(lib().tp_autotune(self.h, count, ms))
+        w = ctypes.c_int()
+        check(lib().tp_get_option(self.h, T.TP_OPT_BR_WAVES, ctypes.byref(w)))
+        return {"waves": w.value, "ms_two_waves": ms[0], "ms_one_wave": ms[1]}
+
+    def set_br_waves(self, waves):
+        check(lib().tp_set_option(self.h, T.TP_OPT_BR_WAVES, waves))
+
     def import_rlwe_ksk(self, key):
         key = np.ascontiguousarray(key, dtype=np.uint64)
         check(lib().tp_import_rlwe_ksk(self.h, ptr(key), key.size))

@@ -184,6 +184,17 @@ int tp_lwe_keyswitch_batch_device(tp_context* ctx, size_t count, const uint64_t*
 int tp_bootstrap_batch_device(tp_context* ctx, size_t count, const uint64_t* lwes, const uint64_t* tv,
                               size_t tv_stride, uint64_t* out);
 
+/* Tuning.  TP_OPT_BR_WAVES selects the blind-rotation kernel variant: 2 (default; 256 registers
+ * per lane, two ciphertexts' waves share a SIMD) or 1 (512 registers, no spills in the ternary
+ * and BMMP key combination; built for 3 and 4 digits).  No counterpart in the reference.
+ * tp_autotune times both variants on `count` synthetic ciphertexts with the imported key, keeps
+ * the faster one for this context and reports ms[0] (two waves) and ms[1] (one wave, -1 if not
+ * built); the context's later launches, including the server's, use it. */
+#define TP_OPT_BR_WAVES 1
+int tp_set_option(tp_context* ctx, int option, int value);
+int tp_get_option(tp_context* ctx, int option, int* value);
+int tp_autotune(tp_context* ctx, size_t count, float* ms);
+
 /* Device memory helpers (so that callers need no HIP headers). */
 int tp_malloc(tp_context* ctx, uint64_t** ptr, size_t words);
 int tp_free(tp_context* ctx, uint64_t* ptr);

@@ -200,3 +200,45 @@ def test_eval_func_ternary_bmmp_full_size(tp, oracle, method, N, n, b, l, count)
         err = (ph - want) % Q
         err = err if err < Q // 2 else err - Q
         assert abs(err) < Q // 16, (c, p)
+
+
+@pytest.mark.parametrize("method,N,n,b,l", [(0, 1024, 24, 12, 3), (1, 2048, 12, 16, 4), (2, 1024, 12, 14, 3),
+                                            (2, 2048, 8, 14, 3), (0, 4096, 6, 16, 4)])
+def test_one_wave_variant_matches_oracle(tp, oracle, method, N, n, b, l):
+    """The 512-register (one wave per SIMD) build of the blind rotation is bit-identical."""
+    say("one-wave variant", method, N, n, b, l)
+    dist = 1 if method == oracle.TERNARY else 0
+    lk, rk, brk = _keys(oracle, N, n, b, l, method, dist, 3.2, 60 + n)
+    ctx = tp.Context(N, n, b, l, method)
+    ctx.import_brk(brk)
+    lwes = oracle.random_mod_pow2(3 * (n + 1), 2 * N, 61, 62 + method).reshape(3, n + 1)
+    tv = np.random.default_rng(n).integers(0, Q, size=2 * N, dtype=np.uint64)
+    ctx.set_br_waves(1)
+    got1, rl1 = ctx.eval_func_batch(lwes, tv, want_rlwe=True)
+    ctx.set_br_waves(2)
+    got2, rl2 = ctx.eval_func_batch(lwes, tv, want_rlwe=True)
+    assert np.array_equal(rl1, rl2) and np.array_equal(got1, got2)
+    for c in range(3):
+        assert np.array_equal(rl1[c], oracle.blind_rotate(N, lwes[c], tv, brk, n, b, l, method, True)), c
+
+
+def test_autotune_picks_a_variant(tp, oracle):
+    N, n, b, l = 1024, 16, 16, 4
+    say("autotune")
+    lk, rk, brk = _keys(oracle, N, n, b, l, oracle.TERNARY, 1, 3.2, 70)
+    ctx = tp.Context(N, n, b, l, oracle.TERNARY)
+    ctx.import_brk(brk)
+    r = ctx.autotune(512)
+    assert r["waves"] in (1, 2) and r["ms_two_waves"] > 0 and r["ms_one_wave"] > 0
+    chosen = r["ms_one_wave"] if r["waves"] == 1 else r["ms_two_waves"]
+    assert chosen <= min(r["ms_one_wave"], r["ms_two_waves"])
+    # the chosen variant still computes the same rotation
+    lwes = oracle.random_mod_pow2(2 * (n + 1), 2 * N, 71, 72).reshape(2, n + 1)
+    tv = np.random.default_rng(5).integers(0, Q, size=2 * N, dtype=np.uint64)
+    _, rl = ctx.eval_func_batch(lwes, tv, want_rlwe=True)
+    for c in range(2):
+        assert np.array_equal(rl[c], oracle.blind_rotate(N, lwes[c], tv, brk, n, b, l, oracle.TERNARY, True)), c
+    from fhetoothpaste_amd import ToothpasteError
+    ctx2 = tp.Context(N, n, 12, 2, 0)
+    with pytest.raises(ToothpasteError):
+        ctx2.set_br_waves(1)  # not built for 2 digits
