@@ -148,6 +148,88 @@ def serving_leg(args, tp, ctx_bin):
     return out
 
 
+def timed(ctx, lib, chk, fn, reps):
+    """Average ms of fn() on the context stream (HIP events), after one warm-up call."""
+    import ctypes
+    fn()
+    ctx.sync()
+    chk(lib.tp_timer_mark(ctx.h, 3))
+    for _ in range(reps):
+        fn()
+    chk(lib.tp_timer_mark(ctx.h, 4))
+    ms = ctypes.c_float()
+    chk(lib.tp_timer_elapsed(ctx.h, 3, 4, ctypes.byref(ms)))
+    return ms.value / reps
+
+
+def extra_workloads(tp, reps=2):
+    """The other bootstrapping workloads: uneven ternary batches (n=800, N=2048) in one launch with a
+    random 3-bit LUT per batch, and BMMP (n=1024, N=2048) followed by packing all outputs."""
+    from fhetoothpaste_amd import _lib
+    from fhetoothpaste_amd.operators import DeviceBuffer
+    from fhetoothpaste_amd.workload import lognormal_counts, random_3bit_lut
+    lib, chk = _lib.lib(), _lib.check
+    out = {}
+    rng = np.random.default_rng(SEED + 9)
+
+    # ternary, 16 lognormal batches (median 256, sigma_ln 1.0, clipped to [1, 4096]) concatenated
+    P = TER
+    seed = SEED + 1
+    lk = tp.keygen_secret(P["n"], 1, seed, 1)
+    rk = tp.keygen_secret(P["N"], 0, seed, 2)
+    ctx = tp.Context(P["N"], P["n"], P["b"], P["l"], P["method"], P["n_out"], P["kb"], P["kd"])
+    ctx.import_brk(tp.keygen_brk(lk, rk, P["N"], P["n"], P["b"], P["l"], P["method"], 1, 3.2, seed))
+    ctx.import_lwe_ksk(tp.keygen_lwe_ksk(rk, lk, P["N"], P["n_out"], P["kb"], P["kd"], 3.2, seed))
+    sizes = lognormal_counts(rng, 16, median=256, sigma_ln=1.0, lo=1, hi=4096)
+    total = int(sizes.sum())
+    tvs = np.concatenate([np.tile(random_3bit_lut(P["N"], rng)[0], (int(c), 1)) for c in sizes])
+    lwes = rng.integers(0, 2 * P["N"], size=(total, P["n"] + 1), dtype=np.uint64)
+    tune = ctx.autotune(total)
+    d_lwe, d_tv = DeviceBuffer.from_host(ctx, lwes), DeviceBuffer.from_host(ctx, tvs)
+    d_out = DeviceBuffer(ctx, total * (P["n_out"] + 1))
+    ms = timed(ctx, lib, chk, lambda: chk(lib.tp_bootstrap_batch_device(ctx.h, total, d_lwe.p, d_tv.p, 2 * P["N"],
+                                                                         d_out.p)), reps)
+    out["ternary_uneven_batches"] = {
+        "workload": "16 lognormal batches (median 256, max 4096) in one launch, ternary n=800 N=2048 b=2^16 l=4, "
+                    "random 3-bit LUT per batch, KS 2048->800 b=2^5 d=12",
+        "ciphertexts": total, "batch_sizes": [int(c) for c in sizes], "ms": round(ms, 2),
+        "bootstraps_per_s": round(total / ms * 1000, 1), "br_waves_per_simd": tune["waves"]}
+    log("ternary uneven batches: %d ciphertexts, %.1f ms" % (total, ms))
+    for b in (d_lwe, d_tv, d_out):
+        b.free()
+    ctx.close()
+
+    # BMMP n=1024 N=2048 b=2^14 l=3, batch 2048, then packing the 2048 extracted LWEs (permuted)
+    N4, n4, B4 = 2048, 1024, 2048
+    seed = SEED + 2
+    lk = tp.keygen_secret(n4, 0, seed, 1)
+    rk = tp.keygen_secret(N4, 0, seed, 2)
+    ctx = tp.Context(N4, n4, 14, 3, 2, 0, 0, 0, 0, 16, 4)
+    ctx.import_brk(tp.keygen_brk(lk, rk, N4, n4, 14, 3, 2, 1, 3.2, seed))
+    ctx.import_auto_keys(tp.keygen_auto_keys(rk, N4, 16, 4, 3.2, seed))
+    tune = ctx.autotune(B4)
+    lwes = rng.integers(0, 2 * N4, size=(B4, n4 + 1), dtype=np.uint64)
+    tv = random_3bit_lut(N4, rng)[0]
+    perm = rng.permutation(B4).astype(np.uint32)
+    d_lwe, d_tv = DeviceBuffer.from_host(ctx, lwes), DeviceBuffer.from_host(ctx, tv)
+    d_ext = DeviceBuffer(ctx, B4 * (N4 + 1))
+    d_perm = DeviceBuffer.from_host(ctx, np.frombuffer(perm.tobytes(), dtype=np.uint64))
+    d_pack = DeviceBuffer(ctx, 2 * N4)
+    ms_br = timed(ctx, lib, chk, lambda: chk(lib.tp_eval_func_batch_device(ctx.h, B4, d_lwe.p, d_tv.p, 0, d_ext.p,
+                                                                            None, 0)), reps)
+    ms_pack = timed(ctx, lib, chk, lambda: chk(lib.tp_pack_device(ctx.h, B4, d_ext.p, d_perm.p, d_pack.p)), reps)
+    out["bmmp_and_packing"] = {
+        "workload": "BMMP n=1024 N=2048 b=2^14 l=3 batch 2048 (blind rotation + sample extract), then packing the "
+                    "2048 outputs under a random permutation (automorphism keys b=2^16 d=4)",
+        "br_ms": round(ms_br, 2), "pack_ms": round(ms_pack, 3),
+        "bootstraps_per_s": round(B4 / (ms_br + ms_pack) * 1000, 1), "br_waves_per_simd": tune["waves"]}
+    log("BMMP batch %.1f ms, packing %.2f ms" % (ms_br, ms_pack))
+    for b in (d_lwe, d_tv, d_ext, d_perm, d_pack):
+        b.free()
+    ctx.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -159,6 +241,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=60.0,
                     help="length of the Poisson request stream of the serving leg (0 skips it)")
+    ap.add_argument("--no-extra", action="store_true", help="skip the ternary and BMMP+packing workloads")
     ap.add_argument("--serve-chunk", type=int, default=64, help="blind-rotation steps per serving launch")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="issue one device-to-device copy of this size first (profiler byte-counter scale)")
@@ -295,6 +378,9 @@ def main():
         "tuning": {"br_waves_per_simd": tune["waves"], "br_ms_two_waves": round(tune["ms_two_waves"], 2),
                    "br_ms_one_wave": round(tune["ms_one_wave"], 2)},
     }
+
+    if not args.no_extra:
+        result["other_workloads"] = extra_workloads(tp)
 
     if args.serve_seconds > 0:
         result["serving"] = serving_leg(args, tp, ctx)
