@@ -76,6 +76,7 @@ SIGNATURES = {
     "tp_rlwe_keyswitch_device": [vp, SZ, U32, INT, INT, vp, vp],
     "tp_pack_consecutive_device": [vp, SZ, vp, vp],
     "tp_eltwise_device": [vp, INT, vp, vp, vp, vp, U64, SZ],
+    "tp_modswitch_device": [vp, vp, vp, SZ, U64, U64, INT, U64],
     "tp_keygen_rlwe_ksk": [u64p, u64p, u64p, U32, U32, U32, ctypes.c_double, U64, U64],
     "tp_sched_create": [ctypes.POINTER(vp), U32, ctypes.POINTER(SZ), SZ, ctypes.POINTER(U32)],
     "tp_sched_destroy": [vp],

@@ -478,6 +478,21 @@ int tp_rlwe_keyswitch_device(tp_context* c, size_t count, uint32_t t, int mode, 
     return TP_OK;
 }
 
+int tp_modswitch_device(tp_context* c, uint64_t* dst, const uint64_t* src, size_t words, uint64_t source_modulus,
+                        uint64_t target_modulus, int type, uint64_t seed) {
+    if (!c || !dst || !src) return fail(TP_E_INVALID, "null argument");
+    if (source_modulus < (1ULL << 63)) return fail(TP_E_INVALID, "source modulus must be >= 2^63");
+    if (target_modulus < 2 || target_modulus > (1ULL << 62) || (target_modulus & (target_modulus - 1)))
+        return fail(TP_E_INVALID, "target modulus must be a power of two in [2, 2^62]");
+    if (type < TP_MS_ROUND || type > TP_MS_RANDOM) return fail(TP_E_INVALID, "unknown modulus-switch type");
+    int rc = set_device(c);
+    if (rc) return rc;
+    const int t = __builtin_ctzll(target_modulus);
+    hipError_t e = tp::launch_modswitch(dst, src, words, source_modulus, t, type, seed, c->stream);
+    if (e != hipSuccess) return fail(TP_E_DEVICE, std::string("modulus switch launch: ") + hipGetErrorString(e));
+    return TP_OK;
+}
+
 int tp_eltwise_device(tp_context* c, int op, uint64_t* dst, const uint64_t* a, const uint64_t* b, const uint64_t* cc,
                       uint64_t scalar, size_t words) {
     if (!c || !dst || !a) return fail(TP_E_INVALID, "null argument");

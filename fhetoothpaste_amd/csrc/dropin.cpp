@@ -223,6 +223,36 @@ std::shared_ptr<MathWorker> SelectWorkerArbitraryDimension(uint64_t modulus) {
     return w;
 }
 
+// ------------------------------------------------------------------ modulus switching
+
+namespace {
+std::mutex g_ms_mu;
+DeviceContext g_ms_dev;
+uint64_t g_ms_calls = 0;
+}  // namespace
+
+void ModulusSwitch(uint64_t* vec, uint64_t n, uint64_t source_modulus, uint64_t target_modulus,
+                   ModulusSwitchType type) {
+    if (n == 0) return;
+    std::lock_guard<std::mutex> lk(g_ms_mu);
+    if (!g_ms_dev) g_ms_dev = make_device_context(base_params(1024));
+    tp_context* x = g_ms_dev.get();
+    DevBuf d(x, n);
+    d.put(vec, n);
+    // RANDOM: a fresh seed per call, as the reference seeds its coin per call from random_device
+    const uint64_t seed = type == ModulusSwitchType::RANDOM ? toothpaste_amd::fresh_seed() ^ ++g_ms_calls : 0;
+    check(tp_modswitch_device(x, d.p, d.p, n, source_modulus, target_modulus, (int)type, seed), "ModulusSwitch");
+    d.get(vec, n);
+}
+
+long double EstimateModulusSwitchingVariance(long double input_variance, uint64_t source_modulus,
+                                             uint64_t target_modulus, KeyDistribution kd, uint64_t expected_l0) {
+    // modulus_switching.cpp:12-34: (Qs^2 var) / Qt^2 + E[hamming weight] / 12, binary/ternary keys
+    require(kd == BINARY || kd == TERNARY, "modulus-switching variance is defined for binary and ternary keys");
+    const long double qs = (long double)source_modulus, qt = (long double)target_modulus;
+    return (qs * qs * input_variance) / (qt * qt) + (long double)expected_l0 / 12.0L;
+}
+
 // ------------------------------------------------------------------ BlindRotator default batch
 
 void BlindRotator::BlindRotateBatch(uint64_t* results, const uint64_t* lwes, uint64_t count,

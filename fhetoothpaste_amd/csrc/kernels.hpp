@@ -66,6 +66,9 @@ hipError_t launch_pack(int logN, const KeySwitchArgs& A, const uint64_t* lwes, c
 // mode 0 convert ([a coef | b NTT]), 1 automorphism ([a|b] coef), 2 automorphism of a only.
 hipError_t launch_rlwe_keyswitch(int logN, const KeySwitchArgs& A, const uint64_t* in, uint64_t* out, int count,
                                 uint32_t t, int mode, int accumulate, const uint64_t* ksk, hipStream_t s);
+// Modulus switch to 2^t (kind: 0 round, 1 floor, 2 ceil, 3 random with a seeded coin).
+hipError_t launch_modswitch(uint64_t* dst, const uint64_t* src, size_t words, uint64_t sM, int t, int kind,
+                            uint64_t seed, hipStream_t s);
 // Elementwise mod-Q arithmetic (op codes TP_ELT_* of the C-ABI).
 hipError_t launch_eltwise(int op, uint64_t* dst, const uint64_t* a, const uint64_t* b, const uint64_t* c,
                           uint64_t scalar, size_t words, hipStream_t s);

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include "gl64.hpp"
 #include "kernels.hpp"
+#include "modswitch.hpp"
 #include "../../include/toothpaste_amd.h"
 
 namespace tp {
@@ -31,7 +32,36 @@ __global__ void __launch_bounds__(256) eltwise_kernel(uint64_t* dst, const uint6
     }
 }
 
+// ModulusSwitch (modulus_switching.cpp:36-93) of `words` values: canonical inputs (< sM; larger
+// ones are reduced by one subtraction first), outputs in [0, 2^t).  kind 3 (RANDOM) picks floor or
+// ceil by a coin from a counter-based stream (seed, index), where the reference draws an unseeded one.
+__global__ void __launch_bounds__(256) modswitch_kernel(uint64_t* dst, const uint64_t* src, size_t words, uint64_t sM,
+                                                        int t, int kind, uint64_t seed) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += stride) {
+        uint64_t v = src[i];
+        if (v >= sM) v -= sM;
+        int k = kind;
+        if (kind == 3) {
+            uint64_t z = seed + (i + 1) * 0x9E3779B97F4A7C15ULL;
+            z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+            z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+            k = ((z ^ (z >> 31)) >> 63) ? kMsFloor : kMsCeil;
+        }
+        dst[i] = ms_switch(v, sM, t, k);
+    }
+}
+
 }  // namespace
+
+hipError_t launch_modswitch(uint64_t* dst, const uint64_t* src, size_t words, uint64_t sM, int t, int kind,
+                            uint64_t seed, hipStream_t s) {
+    if (words == 0) return hipSuccess;
+    size_t groups = (words + 255) / 256;
+    if (groups > 8192) groups = 8192;
+    hipLaunchKernelGGL(modswitch_kernel, dim3((unsigned)groups), dim3(256), 0, s, dst, src, words, sM, t, kind, seed);
+    return hipGetLastError();
+}
 
 hipError_t launch_eltwise(int op, uint64_t* dst, const uint64_t* a, const uint64_t* b, const uint64_t* c,
                           uint64_t scalar, size_t words, hipStream_t s) {

@@ -207,6 +207,13 @@ class Context:
         check(lib().tp_pack_consecutive_device(self.h, x.shape[0], d_in.p, d_out.p))
         return d_out.to_host()
 
+    def modswitch(self, vec, source_modulus, target_modulus, kind=0, seed=0):
+        """ModulusSwitch on the GPU (kind 0 ROUND, 1 FLOOR, 2 CEIL, 3 RANDOM with a seeded coin)."""
+        v = np.ascontiguousarray(vec, dtype=np.uint64)
+        d = DeviceBuffer.from_host(self, v)
+        check(lib().tp_modswitch_device(self.h, d.p, d.p, v.size, source_modulus, target_modulus, kind, seed))
+        return d.to_host().reshape(v.shape)
+
     def eltwise(self, op, a, b=None, c=None, scalar=0):
         a = np.ascontiguousarray(a, dtype=np.uint64)
         bufs = [DeviceBuffer.from_host(self, np.ascontiguousarray(v, dtype=np.uint64)) if v is not None else None

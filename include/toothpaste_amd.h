@@ -150,6 +150,19 @@ int tp_pack_consecutive_device(tp_context* ctx, size_t len, const uint64_t* lwes
 int tp_eltwise_device(tp_context* ctx, int op, uint64_t* dst, const uint64_t* a, const uint64_t* b,
                       const uint64_t* c, uint64_t scalar, size_t words);
 
+/* ModulusSwitch (lib/src/static/modulus_switching.cpp:36-93): dst[i] = R(tM src[i] / sM), then
+ * minus tM if >= tM, for `words` values (device pointers, dst may equal src).  The quotient is
+ * rounded to a 64-bit significand exactly as the reference's x87 long double does, so results
+ * are bit-identical to it, near ties included.  sM >= 2^63 (e.g. the Goldilocks Q), tM = 2^t with
+ * 1 <= t <= 62, inputs < sM.  type: TP_MS_ROUND (roundl), TP_MS_FLOOR, TP_MS_CEIL, TP_MS_RANDOM
+ * (floor or ceil by a fair coin from the stream (seed, index); the reference's coin is unseeded). */
+#define TP_MS_ROUND 0
+#define TP_MS_FLOOR 1
+#define TP_MS_CEIL 2
+#define TP_MS_RANDOM 3
+int tp_modswitch_device(tp_context* ctx, uint64_t* dst, const uint64_t* src, size_t words, uint64_t source_modulus,
+                        uint64_t target_modulus, int type, uint64_t seed);
+
 /* Negacyclic NTT over Z_Q[X]/(X^N+1) of `count` polynomials (device pointers).
  * Replace SolinasWorker::ForwardNTT / BackwardNTT (lib/src/backend/solinas.cpp:228-313);
  * dst may equal src. */

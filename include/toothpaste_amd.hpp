@@ -149,6 +149,15 @@ void SampleExtract(std::vector<IntType, Allocator>& out, std::vector<IntType, Al
     }
 }
 
+// Modulus switching (lib/include/static/modulus_switching.h): vec[i] = R(target * vec[i] / source),
+// computed on the GPU with the reference's x87 extended-precision rounding reproduced exactly.
+// source >= 2^63, target a power of two; RANDOM draws its coin from a seeded stream.
+enum struct ModulusSwitchType { ROUND, FLOOR, CEIL, RANDOM };
+void ModulusSwitch(uint64_t* vec, uint64_t n, uint64_t source_modulus, uint64_t target_modulus,
+                   ModulusSwitchType type);
+long double EstimateModulusSwitchingVariance(long double input_variance, uint64_t source_modulus,
+                                             uint64_t target_modulus, KeyDistribution kd, uint64_t expected_l0);
+
 // ------------------------------------------------------------------ backend
 
 struct MathWorker {

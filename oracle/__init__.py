@@ -73,6 +73,7 @@ def lib():
             "or_automorphism": (None, [ctypes.c_void_p, u64p, u64p, U64, u64p, U32, U32]),
             "or_trace": (None, [ctypes.c_void_p, u64p, u64p, u64p, U32, U32]),
             "or_pack": (ctypes.c_int, [ctypes.c_void_p, u64p, u64p, U32, u64p, U32, U32]),
+            "or_modswitch": (None, [u64p, ctypes.c_uint64, U64, U64, ctypes.c_int]),
             "or_pack_consecutive": (ctypes.c_int, [ctypes.c_void_p, u64p, u64p, U32, u64p, U32, U32]),
             "or_bootstrap_batch": (ctypes.c_double, [ctypes.c_void_p, u64p, u64p, ctypes.c_size_t, u64p, u64p,
                                                      U32, U32, U32, ctypes.c_int, u64p, U32, U32, U32,
@@ -272,4 +273,11 @@ def pack_consecutive(N, lwes, length, keys, b, d):
     out = arr(2 * N)
     rc = lib().or_pack_consecutive(Ring(N).h, P(out), P(np.ascontiguousarray(lwes, np.uint64)), length, P(keys), b, d)
     assert rc == 0
+    return out
+
+
+def modswitch(vec, source_modulus, target_modulus, kind):
+    """ModulusSwitch in x87 long double (kind 0 ROUND, 1 FLOOR, 2 CEIL)."""
+    out = np.ascontiguousarray(vec, dtype=np.uint64).copy()
+    lib().or_modswitch(P(out), out.size, source_modulus, target_modulus, kind)
     return out

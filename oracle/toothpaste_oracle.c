@@ -962,6 +962,22 @@ OR_EXPORT int or_pack_consecutive(const or_ring* R, uint64_t* out, const uint64_
     return 0;
 }
 
+/* ModulusSwitch (static/modulus_switching.cpp:36-93) for ROUND (0), FLOOR (1) and CEIL (2) in
+ * the reference's own arithmetic: x87 long double (64-bit significand) on x86-64, as gcc compiles
+ * the reference there.  RANDOM is unseeded in the reference and not restated. */
+OR_EXPORT void or_modswitch(uint64_t* vec, uint64_t n, uint64_t source_modulus, uint64_t target_modulus, int type) {
+    const long double sM = (long double)source_modulus, tM = (long double)target_modulus;
+    for (uint64_t i = 0; i < n; ++i) {
+        const long double x = (tM * (long double)vec[i]) / sM;
+        uint64_t r;
+        if (type == 0) r = (uint64_t)roundl(x);
+        else if (type == 1) r = (uint64_t)floorl(x);
+        else r = (uint64_t)ceill(x);
+        if (r >= target_modulus) r -= target_modulus;
+        vec[i] = r;
+    }
+}
+
 /* ------------------------------------------------------------------------ */
 /* Batched bootstrap for the CPU baseline: EvalFunc (negacyclic_function_    */
 /* evaluator.cpp:92-101) + LWE key switch, one rotator per thread.          */

@@ -269,6 +269,26 @@ int main(int argc, char** argv) {
             if (!threw) throw std::runtime_error("SelectWorker accepted a foreign modulus");
         }
 
+        step("modulus switch");
+        {
+            auto v = g.modq(4096);
+            // near-integer quotients k Q / 2048 (where the reference's long double ceiling differs
+            // from the exact one) and near-half-integers
+            for (uint64_t k = 1; k < 200; ++k) {
+                v[k] = (uint64_t)(((unsigned __int128)(2047 - k) * Q + 2047) / 2048);
+                v[200 + k] = (uint64_t)(((unsigned __int128)(2 * k + 1) * Q) / 4096);
+            }
+            dump("ms_in", v);
+            for (int kind = 0; kind < 3; ++kind) {
+                auto w = v;
+                ModulusSwitch(w.data(), w.size(), Q, 2048, (ModulusSwitchType)kind);
+                dump("ms_out" + std::to_string(kind), w);
+            }
+            auto r = v;
+            ModulusSwitch(r.data(), r.size(), Q, 2048, ModulusSwitchType::RANDOM);
+            dump("ms_random", r);
+        }
+
         step("sample extract");
         {
             auto in = g.modq(2 * N);

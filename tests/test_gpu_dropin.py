@@ -171,3 +171,11 @@ def test_sample_extract_template(run):
                 want[i] = Q - int(x[N - i])
         want[N] = x[N + idx]
         assert np.array_equal(run(name), want), idx
+
+
+def test_modulus_switch(run, oracle):
+    x = run("ms_in")
+    for kind in range(3):
+        assert np.array_equal(run("ms_out%d" % kind), oracle.modswitch(x, Q, 2048, kind)), kind
+    fl, ce, rnd = oracle.modswitch(x, Q, 2048, 1), oracle.modswitch(x, Q, 2048, 2), run("ms_random")
+    assert np.all((rnd == fl) | (rnd == ce))

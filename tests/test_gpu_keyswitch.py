@@ -166,3 +166,24 @@ def test_eltwise_ops_exact(tp):
     raw[:3] = [Q, Q + 5, 2 ** 64 - 1]
     got = ctx.eltwise(5, raw)
     assert np.array_equal(got.astype(object), raw.astype(object) % Q)
+
+
+@pytest.mark.parametrize("tM", [2048, 4096, 1 << 20])
+def test_modswitch_vs_long_double(tp, oracle, tM):
+    """ModulusSwitch on the GPU == the reference's x87 long double arithmetic (oracle), near ties included."""
+    say("modswitch", tM)
+    ctx = _ctx(tp, 1024, 16, 4)
+    rng = np.random.default_rng(tM)
+    v = list(rng.integers(0, Q, size=20000, dtype=np.uint64))
+    for k in rng.integers(1, tM, size=3000):
+        k = int(k)
+        for d in (-1, 0, 1):
+            v.append((((2 * k + 1) * Q) // (2 * tM) + d) % Q)  # near half-integers
+            v.append(((k * Q + tM - 1) // tM + d) % Q)          # near integers
+    v += [0, 1, Q - 1, Q - 2, (1 << 63), (1 << 63) - 1]
+    v = np.array(v, dtype=np.uint64)
+    for kind in range(3):
+        assert np.array_equal(ctx.modswitch(v, Q, tM, kind), oracle.modswitch(v, Q, tM, kind)), kind
+    r = ctx.modswitch(v, Q, tM, 3, seed=5)
+    assert np.all((r == oracle.modswitch(v, Q, tM, 1)) | (r == oracle.modswitch(v, Q, tM, 2)))
+    assert np.array_equal(r, ctx.modswitch(v, Q, tM, 3, seed=5))  # the seeded coin is reproducible
