@@ -78,6 +78,13 @@ SIGNATURES = {
     "tp_eltwise_device": [vp, INT, vp, vp, vp, vp, U64, SZ],
     "tp_modswitch_device": [vp, vp, vp, SZ, U64, U64, INT, U64],
     "tp_keygen_rlwe_ksk": [u64p, u64p, u64p, U32, U32, U32, ctypes.c_double, U64, U64],
+    "tp_wire_bytes": [SZ, ctypes.POINTER(SZ)],
+    "tp_wire_encode": [ctypes.c_char_p, SZ, INT, U32, ctypes.POINTER(Params), u64p, SZ],
+    "tp_wire_decode": [ctypes.c_char_p, SZ, ctypes.POINTER(INT), ctypes.POINTER(U32), ctypes.POINTER(Params), u64p,
+                       SZ, ctypes.POINTER(SZ)],
+    "tp_wire_save": [ctypes.c_char_p, INT, U32, ctypes.POINTER(Params), u64p, SZ],
+    "tp_wire_load": [ctypes.c_char_p, ctypes.POINTER(INT), ctypes.POINTER(U32), ctypes.POINTER(Params), u64p, SZ,
+                     ctypes.POINTER(SZ)],
     "tp_sched_create": [ctypes.POINTER(vp), U32, ctypes.POINTER(SZ), SZ, ctypes.POINTER(U32)],
     "tp_sched_destroy": [vp],
     "tp_sched_submit": [vp, U32, SZ, ctypes.POINTER(U64)],

@@ -27,6 +27,7 @@ SOURCES = [
     ("capi.cpp", HOST_FLAGS),
     ("keygen.cpp", HOST_FLAGS),
     ("serve.cpp", HOST_FLAGS),
+    ("wire.cpp", HOST_FLAGS),
     ("dropin.cpp", HOST_FLAGS),
 ]
 

@@ -270,6 +270,31 @@ int tp_server_release(tp_server* srv, uint64_t id);
 int tp_server_times(tp_server* srv, uint64_t id, double* submitted, double* completed);
 int tp_server_stats(tp_server* srv, uint64_t* iterations, uint64_t* launches, uint64_t* ciphertexts);
 
+/* ------------------------------------------------------------------ wire format
+ * No counterpart in the reference (it never serialises keys): a versioned record around a key or
+ * ciphertext array in the reference layout, for resident serving keys and shared test vectors.
+ * Header: "TPW1", version 1, kind, flags (bit 0: NTT domain), the tp_params it belongs to, the
+ * payload word count and an FNV-1a 64 checksum; payload little-endian uint64.  Key kinds are
+ * checked against their parameters (BRK n*4Nl / 2n*4Nl / (n/2)*12Nl words, LWE KSK N*d*(n_out+1),
+ * automorphism keys log2(N)*d*2N, RLWE KSK d*2N); decoding rejects a bad magic, version, size or
+ * checksum with TP_E_INVALID.  tp_wire_load with data NULL returns the word count only. */
+#define TP_WIRE_RAW 0
+#define TP_WIRE_BRK 1
+#define TP_WIRE_LWE_KSK 2
+#define TP_WIRE_AUTO_KEYS 3
+#define TP_WIRE_RLWE_KSK 4
+#define TP_WIRE_LWE_BATCH 5
+#define TP_WIRE_RLWE_BATCH 6
+int tp_wire_bytes(size_t words, size_t* bytes);
+int tp_wire_encode(uint8_t* buf, size_t buf_bytes, int kind, uint32_t flags, const tp_params* params,
+                   const uint64_t* data, size_t words);
+int tp_wire_decode(const uint8_t* buf, size_t buf_bytes, int* kind, uint32_t* flags, tp_params* params,
+                   uint64_t* data, size_t cap_words, size_t* words);
+int tp_wire_save(const char* path, int kind, uint32_t flags, const tp_params* params, const uint64_t* data,
+                 size_t words);
+int tp_wire_load(const char* path, int* kind, uint32_t* flags, tp_params* params, uint64_t* data, size_t cap_words,
+                 size_t* words);
+
 /* ------------------------------------------------------------------ seeded key generation
  * Host-side key generation with a seeded sampler (the reference samples with OpenFHE, which
  * has no seed API: lib/src/base_crypto.cpp:34-38,139-142).  Output bytes use the reference
