@@ -989,3 +989,125 @@ void NegacyclicFunctionEvaluator::EvalFuncBatch(uint64_t* results, const uint64_
     check(tp_eval_func_batch_device(dev, count, d_lwe.p, d_lut.p, lut_stride, d_out.p, nullptr, 0), "EvalFunc");
     d_out.get(results, count * (N + 1));
 }
+
+// ------------------------------------------------------------------ LWE -> RGSW
+
+SchemeSwitchingContext::SchemeSwitchingContext(std::shared_ptr<OperatorContext<BlindRotator>> blind_rotation_context,
+                                               std::shared_ptr<TraceEvaluationContext> trace_params,
+                                               std::shared_ptr<RLWEConversionContext> squaring_params,
+                                               uint64_t output_digits, uint64_t basis)
+    : m_rotation_context(std::move(blind_rotation_context)), m_trace_params(std::move(trace_params)),
+      m_squaring_params(std::move(squaring_params)), m_output_digits(output_digits), m_output_basis(basis),
+      m_output_basis_log2(int_log2(basis)) {}
+
+void SchemeSwitchingContext::SetOutputBasis(uint64_t basis) {
+    m_output_basis = basis;
+    m_output_basis_log2 = int_log2(basis);
+}
+
+long double SchemeSwitchingContext::ComputeOutputVariance(long double input_variance) const {
+    // lwe_to_rgsw_conversion.cpp:45-51: rotation, then trace, then squaring switch
+    const long double o = m_rotation_context->ComputeOutputVariance(input_variance);
+    return m_squaring_params->ComputeOutputVariance(m_trace_params->ComputeOutputVariance(o));
+}
+
+Container SchemeSwitchingContext::GetInputContainer() const {
+    auto br = std::dynamic_pointer_cast<TupleContainerImpl>(m_rotation_context->GetInputContainer());
+    require(br != nullptr, "rotation context has no tuple input container");
+    return br->GetElem(0);
+}
+
+Container SchemeSwitchingContext::GetOutputContainer(Container) const {
+    return std::make_shared<RGSWContainerImpl>(m_squaring_params->GetDimension(), m_squaring_params->GetModulus(),
+                                               m_output_digits, m_output_basis, ComputeOutputVariance());
+}
+
+std::unique_ptr<LWEtoRGSWConverter> SchemeSwitchingContext::ConstructOperator(const std::vector<GenericKey>& keys) const {
+    // keys = {LWE secret, RLWE secret, ...}; the trace takes keys[1..], the squaring switch s^2 -> s
+    require(keys.size() >= 2, "scheme-switching bundle is {LWE secret, RLWE secret}");
+    std::vector<GenericKey> trace_key(keys.begin() + 1, keys.end());
+    auto rotator = m_rotation_context->ConstructOperator(keys);
+    auto tracer = m_trace_params->ConstructOperator(trace_key);
+    const uint64_t N = m_trace_params->GetDimension();
+    require(trace_key[0].GetKey().size() >= N, "RLWE secret too short");
+    auto ntt = m_trace_params->GetNTT();
+    std::vector<uint64_t> sq(N);
+    ntt->ForwardNTT(sq.data(), trace_key[0].GetKeyPtr());
+    ntt->MulEqModElw(sq.data(), sq.data(), N);  // the reference squares in place (EltwiseMultMod with aliases)
+    ntt->BackwardNTT(sq.data(), sq.data());
+    std::vector<GenericKey> square_keys = {{"SQUARE", sq.data(), N}, {"DEFAULT", keys[1].GetKeyPtr(), N}};
+    auto squarer = m_squaring_params->ConstructOperator(square_keys);
+    return std::make_unique<LWEtoRGSWConverter>(shared_from_this(), std::move(rotator), std::move(tracer),
+                                                std::move(squarer));
+}
+
+LWEtoRGSWConverter::LWEtoRGSWConverter(std::shared_ptr<const SchemeSwitchingContext> context,
+                                       std::unique_ptr<BlindRotator> rotator, std::unique_ptr<TraceEvaluator> trace_eval,
+                                       std::unique_ptr<RLWEtoRLWEConverter> squaring_eval)
+    : m_params(std::move(context)), m_rotator(std::move(rotator)), m_trace_eval(std::move(trace_eval)),
+      m_square_eval(std::move(squaring_eval)) {
+    // accumulator of lwe_to_rgsw_conversion.cpp:116-161: block-wise staircase of the output gadget,
+    // centred on 0, halved; acc = [0 | -t/2], extra = t/2
+    const uint64_t N = m_params->GetTraceContext()->GetDimension();
+    const uint64_t digits = m_params->GetOutputDigits(), bb = m_params->GetOutputBasisLog2();
+    require(digits >= 1 && N % digits == 0, "output digits must divide N");
+    const uint64_t block = N / digits;
+    const uint64_t max_digits = 64 / bb + (64 % bb ? 1 : 0);  // ceil(log2 Q / bb) for the Goldilocks Q
+    require(digits <= max_digits, "more output digits than the modulus holds");
+    m_acc.assign(3 * N, 0);
+    m_extraction_buffer.assign(4 * N, 0);
+    uint64_t scale = 1ULL << (bb * (max_digits - digits));
+    for (uint64_t i = 0; i < block / 2; ++i) {
+        m_acc[N + i] = scale;
+        m_acc[N + N - i - 1] = kQ - scale;
+    }
+    for (uint64_t d = 1; d < digits; ++d) {
+        scale <<= bb;
+        for (uint64_t j = 0; j < block; ++j) m_acc[N + d * block + j - block / 2] = scale;
+    }
+    auto ntt = m_params->GetSquaringContext()->GetNTT();
+    ntt->MulSEqModElw(m_acc.data() + N, gl::inv(2), N);
+    std::copy(m_acc.begin() + N, m_acc.begin() + 2 * N, m_acc.begin() + 2 * N);
+    ntt->MulSModElw(m_acc.data() + N, m_acc.data() + 2 * N, kQ - 1, N);
+}
+
+void LWEtoRGSWConverter::Convert(uint64_t* output, const uint64_t* const input) {
+    auto sq = m_square_eval->GetContext();
+    const uint64_t N = sq->GetDimension(), digits = m_params->GetOutputDigits(), block = N / digits;
+    auto ntt = sq->GetNTT();
+    uint64_t* buffer = m_extraction_buffer.data();
+    uint64_t* const lev_left = output;
+    uint64_t* const lev_right = output + 2 * N * digits;
+    std::vector<uint64_t> acc(m_acc.begin(), m_acc.begin() + 2 * N);
+    m_rotator->BlindRotate(lev_right, input, acc.data(), true);
+    ntt->AddEqModElw(lev_right + N, m_acc.data() + 2 * N, N);
+    // buffer = [A, -A, B, -B]: row d is rows 0 rotated by X^(-d block)
+    std::fill(m_extraction_buffer.begin(), m_extraction_buffer.end(), 0);
+    std::copy(lev_right, lev_right + N, buffer);
+    std::copy(lev_right + N, lev_right + 2 * N, buffer + 2 * N);
+    ntt->SubEqModElw(buffer + N, buffer, N);
+    ntt->SubEqModElw(buffer + 3 * N, buffer + 2 * N, N);
+    for (uint64_t d = 1; d < digits; ++d) {
+        uint64_t* row = lev_right + d * 2 * N;
+        std::copy(buffer + d * block, buffer + d * block + N, row);
+        std::copy(buffer + 2 * N + d * block, buffer + 2 * N + d * block + N, row + N);
+    }
+    for (uint64_t d = 0; d < digits; ++d) {
+        uint64_t* row = lev_right + d * 2 * N;
+        m_trace_eval->Eval(row, row);
+    }
+    // RLWE(-s m) from RLWE(m): the switch from s^2 to s of (a, 0), then (a' + b, b') of its negation
+    std::fill(m_extraction_buffer.begin(), m_extraction_buffer.end(), 0);
+    for (uint64_t i = 0; i < digits; ++i) {
+        uint64_t* m = lev_right + 2 * N * i;
+        uint64_t* sm = lev_left + 2 * N * i;
+        ntt->BackwardNTT(buffer, m);
+        m_square_eval->Convert(sm, buffer);
+        ntt->SubModElw(sm, buffer + 2 * N, sm, 2 * N);
+        ntt->AddModElw(sm, m + N, sm, N);
+    }
+}
+
+void LWEtoRGSWConverter::Convert(std::vector<uint64_t>& output, const std::vector<uint64_t>& input) {
+    Convert(output.data(), input.data());
+}

@@ -90,6 +90,21 @@ private:
     long double m_var;
 };
 
+struct RGSWContainerImpl : public ContainerImpl {
+    RGSWContainerImpl(uint64_t N, uint64_t Q, uint64_t digits, uint64_t basis, long double var)
+        : m_Q(Q), m_N(N), m_digits(digits), m_basis(basis), m_var(var) {}
+    ContainerLabel GetLabel() override { return RGSW; }
+    uint64_t GetQ() const { return m_Q; }
+    uint64_t GetN() const { return m_N; }
+    uint64_t GetDigits() const { return m_digits; }
+    uint64_t GetBasis() const { return m_basis; }
+    long double GetVariance() const { return m_var; }
+
+private:
+    uint64_t m_Q, m_N, m_digits, m_basis;
+    long double m_var;
+};
+
 struct TupleContainerImpl : public ContainerImpl {
     explicit TupleContainerImpl(std::vector<Container> elems) : m_elems(std::move(elems)) {}
     ContainerLabel GetLabel() override { return Tuple; }
@@ -622,6 +637,60 @@ private:
     std::shared_ptr<const LWEtoRLWEPackingContext> m_context;
     toothpaste_amd::DeviceContext m_dev;
     std::vector<uint64_t> m_keys;
+};
+
+// ------------------------------------------------------------------ LWE -> RGSW (scheme switching)
+
+struct LWEtoRGSWConverter;
+struct SchemeSwitchingContext : OperatorContext<LWEtoRGSWConverter>,
+                                public std::enable_shared_from_this<SchemeSwitchingContext> {
+    SchemeSwitchingContext(std::shared_ptr<OperatorContext<BlindRotator>> blind_rotation_context,
+                           std::shared_ptr<TraceEvaluationContext> trace_params,
+                           std::shared_ptr<RLWEConversionContext> squaring_params, uint64_t output_digits,
+                           uint64_t basis);
+    long double ComputeOutputVariance(long double input_variance = 0) const override;
+    [[nodiscard]] const std::shared_ptr<OperatorContext<BlindRotator>> GetBlindRotationContext() const {
+        return m_rotation_context;
+    }
+    [[nodiscard]] const std::shared_ptr<TraceEvaluationContext> GetTraceContext() const { return m_trace_params; }
+    [[nodiscard]] const std::shared_ptr<RLWEConversionContext> GetSquaringContext() const { return m_squaring_params; }
+    [[nodiscard]] uint64_t GetOutputDigits() const { return m_output_digits; }
+    [[nodiscard]] uint64_t GetOutputBasis() const { return m_output_basis; }
+    [[nodiscard]] uint64_t GetOutputBasisLog2() const { return m_output_basis_log2; }
+    void SetBlindRotationContext(std::shared_ptr<OperatorContext<BlindRotator>> c) { m_rotation_context = std::move(c); }
+    void SetTraceContext(std::shared_ptr<TraceEvaluationContext> c) { m_trace_params = std::move(c); }
+    void SetSquaringParameters(std::shared_ptr<RLWEConversionContext> c) { m_squaring_params = std::move(c); }
+    void SetOutputDigits(uint64_t d) { m_output_digits = d; }
+    void SetOutputBasis(uint64_t basis);
+    [[nodiscard]] Container GetInputContainer() const override;
+    [[nodiscard]] Container GetOutputContainer(Container container) const override;
+    std::unique_ptr<LWEtoRGSWConverter> ConstructOperator(const std::vector<GenericKey>& keys) const override;
+    [[nodiscard]] OperatorID GetOperatorID() const override { return CONV_LWE_RGSW; }
+
+private:
+    std::shared_ptr<OperatorContext<BlindRotator>> m_rotation_context;
+    std::shared_ptr<TraceEvaluationContext> m_trace_params;
+    std::shared_ptr<RLWEConversionContext> m_squaring_params;
+    uint64_t m_output_digits, m_output_basis, m_output_basis_log2;
+};
+
+struct LWEtoRGSWConverter : SchemeConverter<SchemeSwitchingContext> {
+    friend struct SchemeSwitchingContext;
+    LWEtoRGSWConverter(std::shared_ptr<const SchemeSwitchingContext> context, std::unique_ptr<BlindRotator> rotator,
+                       std::unique_ptr<TraceEvaluator> trace_eval, std::unique_ptr<RLWEtoRLWEConverter> squaring_eval);
+    [[nodiscard]] const std::shared_ptr<const SchemeSwitchingContext> GetContext() const override { return m_params; }
+    // input LWE (n + 1 in Z_2N) -> output [RLWE(-s m L^i) rows | RLWE(m L^i) rows], digits x 2N each, NTT
+    // domain (lwe_to_rgsw_conversion.cpp:163-231); the first half is subtracted into, as the reference
+    // does, so pass it zeroed.
+    void Convert(uint64_t* output, const uint64_t* const input) override;
+    void Convert(std::vector<uint64_t>& output, const std::vector<uint64_t>& input) override;
+
+private:
+    std::shared_ptr<const SchemeSwitchingContext> m_params;
+    std::unique_ptr<BlindRotator> m_rotator;
+    std::unique_ptr<TraceEvaluator> m_trace_eval;
+    std::unique_ptr<RLWEtoRLWEConverter> m_square_eval;
+    std::vector<uint64_t> m_acc, m_extraction_buffer;
 };
 
 // ------------------------------------------------------------------ function evaluation

@@ -73,6 +73,8 @@ def lib():
             "or_automorphism": (None, [ctypes.c_void_p, u64p, u64p, U64, u64p, U32, U32]),
             "or_trace": (None, [ctypes.c_void_p, u64p, u64p, u64p, U32, U32]),
             "or_pack": (ctypes.c_int, [ctypes.c_void_p, u64p, u64p, U32, u64p, U32, U32]),
+            "or_lwe_to_rgsw": (ctypes.c_int, [ctypes.c_void_p, u64p, u64p, u64p, U32, U32, U32, ctypes.c_int, u64p,
+                                               U32, U32, u64p, U32, U32, U32, U32]),
             "or_modswitch": (None, [u64p, ctypes.c_uint64, U64, U64, ctypes.c_int]),
             "or_pack_consecutive": (ctypes.c_int, [ctypes.c_void_p, u64p, u64p, U32, u64p, U32, U32]),
             "or_bootstrap_batch": (ctypes.c_double, [ctypes.c_void_p, u64p, u64p, ctypes.c_size_t, u64p, u64p,
@@ -280,4 +282,13 @@ def modswitch(vec, source_modulus, target_modulus, kind):
     """ModulusSwitch in x87 long double (kind 0 ROUND, 1 FLOOR, 2 CEIL)."""
     out = np.ascontiguousarray(vec, dtype=np.uint64).copy()
     lib().or_modswitch(P(out), out.size, source_modulus, target_modulus, kind)
+    return out
+
+
+def lwe_to_rgsw(N, lwe, brk, n, bl, ll, method, trace_keys, bt, dt, sq_key, bs, ds, digits, basebits):
+    """LWEtoRGSWConverter::Convert: [lev_left | lev_right], digits x 2N each, NTT domain."""
+    out = arr(4 * N * digits)
+    rc = lib().or_lwe_to_rgsw(Ring(N).h, P(out), P(np.ascontiguousarray(lwe, np.uint64)), P(brk), n, bl, ll, method,
+                              P(trace_keys), bt, dt, P(sq_key), bs, ds, digits, basebits)
+    assert rc == 0, rc
     return out

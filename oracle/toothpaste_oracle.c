@@ -978,6 +978,78 @@ OR_EXPORT void or_modswitch(uint64_t* vec, uint64_t n, uint64_t source_modulus, 
     }
 }
 
+/* LWEtoRGSWConverter (lwe_to_rgsw_conversion.cpp:116-231).  Accumulator set-up of the
+ * constructor (:116-161): with block = N / digits and scale = 2^(b (D - digits)), D = ceil(64 / b),
+ * t[i] = scale and t[N-1-i] = -scale for i < block/2, t[d block + j - block/2] = scale 2^(b d) for
+ * d >= 1, j < block; acc = [0 | -t/2], extra = t/2.  Convert (:163-231): BR (coefficient output)
+ * of acc by the input, b += extra; rows d >= 1 are X^(-d block) times row 0; trace of each row
+ * (NTT out); then lev_left row i = (KS_{s^2->s}(INTT(a_i)) + (b_i, 0)) with KS in the subtract-
+ * into-zero form negated, i.e. (sum_a + b_i, sum_b).  out = [lev_left: digits x 2N | lev_right:
+ * digits x 2N], NTT domain; brk/method/bl/ll: the rotation key, trace keys (log2 N x dt x 2N) with
+ * gadget bt, square key (ds x 2N) with gadget bs. */
+OR_EXPORT int or_lwe_to_rgsw(const or_ring* R, uint64_t* out, const uint64_t* lwe, const uint64_t* brk, uint32_t n,
+                             uint32_t bl, uint32_t ll, int method, const uint64_t* trace_keys, uint32_t bt, uint32_t dt,
+                             const uint64_t* sq_key, uint32_t bs, uint32_t ds, uint32_t digits, uint32_t basebits) {
+    const uint32_t N = R->N;
+    if (digits == 0 || N % digits) return -1;
+    const uint32_t block = N / digits;
+    const uint32_t D = 64 / basebits + (64 % basebits ? 1 : 0);
+    uint64_t* t = (uint64_t*)calloc((size_t)N, sizeof(uint64_t));
+    uint64_t* acc = (uint64_t*)calloc((size_t)3 * N, sizeof(uint64_t));
+    uint64_t* buf = (uint64_t*)calloc((size_t)4 * N, sizeof(uint64_t));
+    uint64_t scale = 1ULL << (basebits * (D - digits));
+    for (uint32_t i = 0; i < block / 2; ++i) {
+        t[i] = scale;
+        t[N - i - 1] = OR_Q - scale;
+    }
+    for (uint32_t d = 1; d < digits; ++d) {
+        scale <<= basebits;
+        for (uint32_t j = 0; j < block; ++j) t[d * block + j - block / 2] = scale;
+    }
+    const uint64_t inv2 = or_inv(2);
+    for (uint32_t i = 0; i < N; ++i) {
+        acc[2 * N + i] = or_mul(t[i], inv2);
+        acc[N + i] = or_mul(acc[2 * N + i], OR_Q - 1);
+    }
+    uint64_t* left = out;
+    uint64_t* right = out + (size_t)2 * N * digits;
+    memset(out, 0, sizeof(uint64_t) * 4 * N * digits);
+    int rc = or_blind_rotate(R, right, lwe, acc, brk, n, bl, ll, method, 1);
+    if (rc) goto done;
+    vadd(right + N, right + N, acc + 2 * N, N);
+    memcpy(buf, right, sizeof(uint64_t) * N);
+    memcpy(buf + 2 * N, right + N, sizeof(uint64_t) * N);
+    for (uint32_t i = 0; i < N; ++i) {
+        buf[N + i] = neg(buf[i]);
+        buf[3 * N + i] = neg(buf[2 * N + i]);
+    }
+    for (uint32_t d = 1; d < digits; ++d) {
+        uint64_t* row = right + (size_t)d * 2 * N;
+        memcpy(row, buf + d * block, sizeof(uint64_t) * N);
+        memcpy(row + N, buf + 2 * N + d * block, sizeof(uint64_t) * N);
+    }
+    for (uint32_t d = 0; d < digits; ++d) {
+        uint64_t* row = right + (size_t)d * 2 * N;
+        memcpy(buf, row, sizeof(uint64_t) * 2 * N);
+        or_trace(R, row, buf, trace_keys, bt, dt);
+    }
+    memset(buf, 0, sizeof(uint64_t) * 4 * N);
+    for (uint32_t i = 0; i < digits; ++i) {
+        uint64_t* m = right + (size_t)2 * N * i;
+        uint64_t* sm = left + (size_t)2 * N * i;
+        memcpy(buf, m, sizeof(uint64_t) * N);
+        ntt_inv(R, buf);
+        or_rlwe_keyswitch(R, sm, buf, sq_key, bs, ds);      /* sm = -sum (sm starts at 0) */
+        vsub(sm, buf + 2 * N, sm, 2 * N);                     /* sm = sum */
+        vadd(sm, m + N, sm, N);                               /* a += b_m */
+    }
+done:
+    free(t);
+    free(acc);
+    free(buf);
+    return rc;
+}
+
 /* ------------------------------------------------------------------------ */
 /* Batched bootstrap for the CPU baseline: EvalFunc (negacyclic_function_    */
 /* evaluator.cpp:92-101) + LWE key switch, one rotator per thread.          */

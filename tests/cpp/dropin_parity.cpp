@@ -269,6 +269,27 @@ int main(int argc, char** argv) {
             if (!threw) throw std::runtime_error("SelectWorker accepted a foreign modulus");
         }
 
+        step("lwe to rgsw");
+        {
+            const uint64_t n = 16;
+            auto lk = secret(n, 0, 109, 1), rk = secret(N, 0, 109, 2);
+            auto br = std::make_shared<CGGIBlindRotationContext>(BINARY, Q, N, n, 1 << 16, 4, 3.2);
+            br->SetSeed(109);
+            auto ntt = br->GetNTT();
+            auto trace = std::make_shared<TraceEvaluationContext>(BINARY, ntt, 1 << 16, 4, 3.2);
+            trace->SetSeed(110);
+            auto square = std::make_shared<RLWEConversionContext>(BINARY, ntt, 1 << 16, 4, 3.2);
+            square->SetSeed(111);
+            auto rot = std::reinterpret_pointer_cast<OperatorContext<BlindRotator>>(br);
+            auto ss = std::make_shared<SchemeSwitchingContext>(rot, trace, square, 4, 1 << 10);
+            auto conv = ss->ConstructOperator({GenericKey("LWE", lk), GenericKey("RLWE", rk)});
+            auto lwe = g.below(n + 1, 2 * N);
+            std::vector<uint64_t> out(4 * N * 4, 0);
+            conv->Convert(out, lwe);
+            dump("rgsw_lwe", lwe);
+            dump("rgsw_out", out);
+        }
+
         step("modulus switch");
         {
             auto v = g.modq(4096);

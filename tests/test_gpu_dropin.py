@@ -179,3 +179,16 @@ def test_modulus_switch(run, oracle):
         assert np.array_equal(run("ms_out%d" % kind), oracle.modswitch(x, Q, 2048, kind)), kind
     fl, ce, rnd = oracle.modswitch(x, Q, 2048, 1), oracle.modswitch(x, Q, 2048, 2), run("ms_random")
     assert np.all((rnd == fl) | (rnd == ce))
+
+
+def test_lwe_to_rgsw(run, oracle):
+    n = 16
+    lk = oracle.secret(n, 0, 109, 1)
+    s = oracle.secret(N, 0, 109, 2)
+    brk = oracle.brk(N, lk, s, n, 16, 4, oracle.BINARY, 0, 3.2, 109)
+    tkeys = oracle.auto_keys(N, s, 16, 4, 3.2, 110)
+    R = oracle.Ring(N)
+    s2 = R.intt(np.array([int(x) * int(x) % Q for x in R.ntt(s)], dtype=np.uint64))
+    sq = oracle.rlwe_ksk(N, s2, s, 16, 4, 3.2, 111)
+    ref = oracle.lwe_to_rgsw(N, run("rgsw_lwe"), brk, n, 16, 4, oracle.BINARY, tkeys, 16, 4, sq, 16, 4, 4, 10)
+    assert np.array_equal(run("rgsw_out"), ref)

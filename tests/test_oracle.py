@@ -385,3 +385,38 @@ def test_pack_consecutive_full_equals_pack(oracle):
     keys = oracle.auto_keys(N, s, b, d, 3.2, 14)
     lwes = np.random.default_rng(9).integers(0, Q, size=(N, N + 1), dtype=np.uint64)
     assert np.array_equal(oracle.pack_consecutive(N, lwes, N, keys, b, d), oracle.pack(N, lwes, N, keys, b, d))
+
+
+@pytest.mark.parametrize("basebits,digits", [(16, 4), (10, 4)])
+def test_lwe_to_rgsw_noise_free(oracle, basebits, digits):
+    """LWE(m q/2) -> RGSW(m): with noise-free keys, lev_right row i has phase m B^(i + D - digits) and
+    lev_left row i phase -s m B^(i + D - digits), D = ceil(64 / log2 B) (scheme_switch_tests.cpp:100-119,
+    170-188)."""
+    N, n, b, l = 1024, 16, 16, 4
+    lk = oracle.secret(n, 0, 21, 1)
+    s = oracle.secret(N, 0, 21, 2)
+    brk = oracle.brk(N, lk, s, n, b, l, oracle.BINARY, 0, 0.0, 21)
+    tkeys = oracle.auto_keys(N, s, 16, 4, 0.0, 21)
+    R = oracle.Ring(N)
+    s_ntt = R.ntt(s)
+    s2 = R.intt(np.array([int(x) * int(x) % Q for x in s_ntt], dtype=np.uint64))
+    sq = oracle.rlwe_ksk(N, s2, s, 16, 4, 0.0, 21)
+    D = -(-64 // basebits)
+    rng = np.random.default_rng(basebits)
+    for m in (0, 1):
+        a = rng.integers(0, 2 * N, size=n)
+        bb = (int(sum(int(x) * int(y) for x, y in zip(a, lk))) + m * N) % (2 * N)
+        lwe = np.array(list(a) + [bb], dtype=np.uint64)
+        out = oracle.lwe_to_rgsw(N, lwe, brk, n, b, l, oracle.BINARY, tkeys, 16, 4, sq, 16, 4, digits, basebits)
+        for i in range(2 * digits):
+            row = out[i * 2 * N:(i + 1) * 2 * N]
+            ph = oracle.phase_rlwe(N, row, s)
+            scal = (m * pow(2, basebits * (i % digits + D - digits), Q)) % Q
+            if i >= digits:
+                want = [scal] + [0] * (N - 1)
+            else:
+                want = [(Q - scal * int(x)) % Q for x in s]
+            for j in range(N):
+                err = (int(ph[j]) - want[j]) % Q
+                err = err if err < Q // 2 else err - Q
+                assert abs(err) <= 1, (m, i, j, err)
