@@ -77,6 +77,9 @@ SIGNATURES = {
     "tp_pack_consecutive_device": [vp, SZ, vp, vp],
     "tp_eltwise_device": [vp, INT, vp, vp, vp, vp, U64, SZ],
     "tp_modswitch_device": [vp, vp, vp, SZ, U64, U64, INT, U64],
+    "tp_sap_decompose_device": [vp, SZ, U32, U32, U64, vp, vp, U32, vp],
+    "tp_sap_homtrunc_device": [vp, SZ, U32, vp, vp],
+    "tp_sap_reset_device": [vp, SZ, U64, vp],
     "tp_keygen_rlwe_ksk": [u64p, u64p, u64p, U32, U32, U32, ctypes.c_double, U64, U64],
     "tp_wire_bytes": [SZ, ctypes.POINTER(SZ)],
     "tp_wire_encode": [ctypes.c_char_p, SZ, INT, U32, ctypes.POINTER(Params), u64p, SZ],

@@ -24,6 +24,7 @@ SOURCES = [
     ("kernels_br1.hip", DEVICE_FLAGS),
     ("kernels_pack.hip", DEVICE_FLAGS),
     ("kernels_eltwise.hip", DEVICE_FLAGS),
+    ("kernels_sap.hip", DEVICE_FLAGS),
     ("capi.cpp", HOST_FLAGS),
     ("keygen.cpp", HOST_FLAGS),
     ("serve.cpp", HOST_FLAGS),

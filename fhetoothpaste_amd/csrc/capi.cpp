@@ -64,6 +64,10 @@ struct tp_context {
     size_t perm_len = 0;
     uint64_t* d_rksk = nullptr;  // RLWEtoRLWEConverter key, d x 2N, key-row order
     int br_waves = 2;            // blind-rotation variant: waves per SIMD (tp_set_option / tp_autotune)
+    uint64_t* d_sapw = nullptr;  // SAP decomposition workspace (per-input state, accumulators, tree slots)
+    size_t sapw_words = 0;
+    uint64_t* d_sap_polys = nullptr;  // SAP NTT-domain polynomials for sap_radix
+    uint32_t sap_radix = 0;
     hipEvent_t ev[8] = {};
 };
 
@@ -264,7 +268,7 @@ int tp_context_destroy(tp_context* c) {
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (uint64_t* p : {c->d_tw, c->d_itw, c->d_brk, c->d_ksk, c->d_corr, c->d_scratch, c->d_io, c->d_mono,
-                        c->d_auto, c->d_nodes, c->d_rksk})
+                        c->d_auto, c->d_nodes, c->d_rksk, c->d_sapw, c->d_sap_polys})
         if (p) (void)hipFree(p);
     for (auto& ev : c->ev)
         if (ev) (void)hipEventDestroy(ev);
@@ -760,6 +764,287 @@ int tp_timer_elapsed(tp_context* c, int a, int b, float* ms) {
     if (!c || !ms || a < 0 || a >= 8 || b < 0 || b >= 8) return fail(TP_E_INVALID, "bad timer slot");
     TP_HIP(hipEventSynchronize(c->ev[b]));
     TP_HIP(hipEventElapsedTime(ms, c->ev[a], c->ev[b]));
+    return TP_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ SAP digit decomposition
+//
+// SAPDecomposer (lib/src/operators/sap_decomposer.cpp:190-494) over a batch: every step of the
+// reference runs for all inputs at once (one blind-rotation launch per digit, one automorphism
+// key-switch launch per node of the HomTrunc tree).  RLWEs are [a|b]; a batch of them is 2 count
+// rows of N words, so the per-row polynomial products broadcast one N-word polynomial.
+
+namespace {
+
+uint32_t int_log2_ceil(uint64_t x) {  // IntLog2 (utils/math_utils.cpp:11-25)
+    if (x <= 1) return 0;
+    uint32_t r = 0;
+    for (x -= 1; x; x >>= 1) ++r;
+    return r;
+}
+
+uint32_t bit_reverse(uint32_t x, uint32_t bits) {
+    uint32_t r = 0;
+    for (uint32_t b = 0; b < bits; ++b) { r = (r << 1) | (x & 1); x >>= 1; }
+    return r;
+}
+
+// Polynomials that depend on (N, radix), NTT domain, in this order:
+//   xp      x -> x mod radix extraction, reversed: xp[N - i] = -(i & (radix - 1))   (:412-416)
+//   trunc   N^-1 sum_{i < radix} X^-i                                               (:220-233)
+//   shift   [logN + 1] level l: -X^(N - N / 2^l)                                    (:280-285)
+//   leaf    [N / radix] exponent e: X^brv(e) over log2(N / radix) bits              (:266-268)
+size_t sap_poly_words(uint32_t N, uint32_t logN, uint32_t radix) { return (size_t)N * (3 + logN + N / radix); }
+
+int sap_prepare(tp_context* c, uint32_t radix) {
+    if (c->sap_radix == radix && c->d_sap_polys) return TP_OK;
+    const uint32_t N = c->N, logN = (uint32_t)c->logN, lr = int_log2_ceil(radix);
+    const size_t words = sap_poly_words(N, logN, radix);
+    std::vector<uint64_t> h(words, 0);
+    tp::HostNtt T(N);
+    uint64_t* xp = h.data();
+    uint64_t* trunc = xp + N;
+    uint64_t* shift = trunc + N;
+    uint64_t* leaf = shift + (size_t)(logN + 1) * N;
+    for (uint32_t i = 1; i < N; ++i) xp[N - i] = gl::neg(i & (radix - 1));
+    T.forward(xp);
+    const uint64_t n_inv = gl::inv(N);
+    trunc[0] = n_inv;
+    for (uint32_t i = N - (radix - 1); i < N; ++i) trunc[i] = gl::neg(n_inv);
+    T.forward(trunc);
+    for (uint32_t l = 1; l <= logN; ++l) {
+        uint64_t* p = shift + (size_t)l * N;
+        p[N - (N >> l)] = gl::kQ - 1;
+        T.forward(p);
+    }
+    const uint32_t rev_bits = logN - lr;
+    for (uint32_t e = 0; e < (N >> lr); ++e) {
+        uint64_t* p = leaf + (size_t)e * N;
+        p[rev_bits ? bit_reverse(e, rev_bits) : 0] = 1;
+        T.forward(p);
+    }
+    if (c->d_sap_polys) TP_HIP(hipFree(c->d_sap_polys));
+    c->d_sap_polys = nullptr;
+    c->sap_radix = 0;
+    if (hipMalloc(&c->d_sap_polys, words * 8) != hipSuccess) return fail(TP_E_NOMEM, "SAP polynomial allocation failed");
+    TP_HIP(hipMemcpy(c->d_sap_polys, h.data(), words * 8, hipMemcpyHostToDevice));
+    c->sap_radix = radix;
+    return TP_OK;
+}
+
+// Workspace carve-up for `count` inputs.
+struct SapWork {
+    uint64_t *cp, *sub, *cm, *acc, *tv, *ext, *slots, *rext, *rlwe_n, *rpoly, *fresh;
+    uint32_t* digits;
+};
+
+int sap_workspace(tp_context* c, size_t count, uint32_t lr, SapWork& w) {
+    const size_t N = c->N, n1 = (size_t)c->p.lwe_dim + 1, k1 = (size_t)c->p.ks_out_dim + 1;
+    const size_t slots = (size_t)c->logN - lr + 3;
+    const size_t rl = 2 * N * count;
+    const size_t words = 2 * n1 * count + 2 * count + 3 * rl + slots * rl + (N + 1) * count + k1 * count + 3 * N;
+    if (c->sapw_words < words) {
+        if (c->d_sapw) TP_HIP(hipFree(c->d_sapw));
+        c->d_sapw = nullptr;
+        c->sapw_words = 0;
+        if (hipMalloc(&c->d_sapw, words * 8) != hipSuccess) return fail(TP_E_NOMEM, "SAP workspace allocation failed");
+        c->sapw_words = words;
+    }
+    uint64_t* p = c->d_sapw;
+    w.cp = p; p += n1 * count;
+    w.sub = p; p += n1 * count;
+    w.cm = p; p += count;
+    w.digits = reinterpret_cast<uint32_t*>(p); p += count;
+    w.acc = p; p += rl;
+    w.tv = p; p += rl;
+    w.ext = p; p += rl;
+    w.slots = p; p += slots * rl;
+    w.rext = p; p += (N + 1) * count;
+    w.rlwe_n = p; p += k1 * count;
+    w.rpoly = p; p += N;
+    w.fresh = p;
+    return TP_OK;
+}
+
+// Q/(2N) X^0 as a coefficient-domain accumulator [0 | Q/(2N), 0, ...] (:405-407, :368).
+int sap_upload_fresh(tp_context* c, const SapWork& w) {
+    std::vector<uint64_t> f(2 * (size_t)c->N, 0);
+    f[c->N] = gl::kQ / (2 * (uint64_t)c->N);
+    TP_HIP(hipMemcpyAsync(w.fresh, f.data(), f.size() * 8, hipMemcpyHostToDevice, c->stream));
+    TP_HIP(hipStreamSynchronize(c->stream));  // f is a stack buffer
+    return TP_OK;
+}
+
+#define TP_LAUNCH(what, call)                                                                        \
+    do {                                                                                             \
+        hipError_t e_ = (call);                                                                      \
+        if (e_ != hipSuccess) return fail(TP_E_DEVICE, std::string(what ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+// HomTrunc (:190-312) of count RLWEs `in` (NTT) into `out` (NTT); out may equal in.
+int sap_homtrunc(tp_context* c, const SapWork& w, size_t count, uint32_t radix, const uint64_t* in, uint64_t* out) {
+    const uint32_t N = c->N, logN = (uint32_t)c->logN, lr = int_log2_ceil(radix);
+    const size_t rows = 2 * count, rl = 2 * (size_t)N * count;
+    const uint64_t* trunc = c->d_sap_polys + N;
+    const uint64_t* shift = trunc + N;
+    const uint64_t* leaf = shift + (size_t)(logN + 1) * N;
+    const size_t key_words = (size_t)c->p.rlwe_ks_digits * 2 * N;
+    const tp::KeySwitchArgs A = auto_args(c);
+    auto slot = [&](uint32_t h) { return w.slots + (size_t)h * rl; };
+    TP_LAUNCH("SAP truncation product", tp::launch_sap_poly_mul(slot(0), in, trunc, rows, N, c->stream));
+    // partial trace over t = N + 1, N/2 + 1, ..., N/2^(lr-1) + 1 (:236-250)
+    for (uint32_t i = 0; i < lr; ++i) {
+        const uint32_t j = logN - i;
+        TP_LAUNCH("SAP inverse NTT", tp::launch_ntt_batch(c->logN, slot(1), slot(0), rows, c->d_itw, true, c->stream));
+        TP_LAUNCH("SAP automorphism", tp::launch_rlwe_keyswitch(c->logN, A, slot(1), slot(0), (int)count, (1u << j) + 1,
+                                                                TP_RKS_AUTOMORPHISM, 1,
+                                                                c->d_auto + (size_t)(j - 1) * key_words, c->stream));
+    }
+    // the clean-up tree (:266-311), in the reference's stack order and slot assignment
+    uint32_t lvl[64];
+    uint32_t ex[64];
+    uint32_t head = 1;
+    lvl[0] = logN - lr;
+    ex[0] = 0;
+    bool first_leaf = true;
+    while (head != 0) {
+        --head;
+        const uint32_t level = lvl[head], e = ex[head];
+        if (level == 0) {
+            TP_LAUNCH("SAP leaf", tp::launch_sap_leaf(out, slot(head), leaf + (size_t)e * N, rows, N,
+                                                      first_leaf ? 0 : 1, c->stream));
+            first_leaf = false;
+            continue;
+        }
+        TP_LAUNCH("SAP inverse NTT",
+                  tp::launch_ntt_batch(c->logN, slot(head + 1), slot(head), rows, c->d_itw, true, c->stream));
+        TP_LAUNCH("SAP automorphism",
+                  tp::launch_rlwe_keyswitch(c->logN, A, slot(head + 1), slot(head + 2), (int)count, (1u << level) + 1,
+                                            TP_RKS_AUTOMORPHISM, 0, c->d_auto + (size_t)(level - 1) * key_words,
+                                            c->stream));
+        TP_LAUNCH("SAP tree node", tp::launch_sap_node(slot(head), slot(head + 2), slot(head + 1),
+                                                       shift + (size_t)level * N, rows, N, c->stream));
+        lvl[head] = level - 1;
+        ex[head] = 2 * e;
+        lvl[head + 1] = level - 1;
+        ex[head + 1] = 2 * e + 1;
+        head += 2;
+    }
+    return TP_OK;
+}
+
+// ResetAccumulator (:314-373) of count accumulators (NTT, in place).  The key switch is mod Q,
+// so the reference's optional ROUND switch to the key-switching modulus (:358-360) is the identity.
+int sap_reset(tp_context* c, const SapWork& w, size_t count, uint64_t radix, uint64_t* acc) {
+    const uint32_t N = c->N, rlog = int_log2_ceil(radix);
+    const size_t rows = 2 * count;
+    std::vector<uint64_t> poly(N, 0);
+    for (uint32_t i = 1; i < N; ++i) poly[i] = gl::neg((uint64_t)(((N - i) >> rlog) << rlog) % gl::kQ);
+    tp::HostNtt(N).forward(poly.data());
+    TP_HIP(hipMemcpyAsync(w.rpoly, poly.data(), N * 8, hipMemcpyHostToDevice, c->stream));
+    TP_HIP(hipStreamSynchronize(c->stream));
+    TP_LAUNCH("SAP reset product", tp::launch_sap_poly_mul(w.tv, acc, w.rpoly, rows, N, c->stream));
+    TP_LAUNCH("SAP inverse NTT", tp::launch_ntt_batch(c->logN, w.ext, w.tv, rows, c->d_itw, true, c->stream));
+    TP_LAUNCH("SAP extraction", tp::launch_sap_extract(w.rext, w.ext, nullptr, 0, (size_t)N + 1, (int)count, N,
+                                                       c->stream));
+    int rc = tp::launch_ks_rows(c, count, w.rext, (size_t)N + 1, nullptr, w.rlwe_n);
+    if (rc) return rc;
+    const size_t k1 = (size_t)c->p.ks_out_dim + 1;
+    TP_LAUNCH("SAP modulus switch", tp::launch_modswitch(w.rlwe_n, w.rlwe_n, k1 * count, gl::kQ, c->logN + 1,
+                                                         TP_MS_FLOOR, 0, c->stream));
+    return launch_eval(c, count, w.rlwe_n, w.fresh, 0, nullptr, acc, 1, 0);
+}
+
+int sap_check(tp_context* c, uint32_t radix, bool need_brk, bool need_auto, bool need_ksk) {
+    if (radix < 2 || radix > c->N || (radix & (radix - 1))) return fail(TP_E_INVALID, "radix must be a power of two in [2, N]");
+    if (need_brk && !c->d_brk) return fail(TP_E_STATE, "blind-rotation key not imported");
+    if (need_auto && !c->d_auto) return fail(TP_E_STATE, "automorphism keys not imported");
+    if (need_ksk) {
+        if (!c->d_ksk) return fail(TP_E_STATE, "key-switching key not imported");
+        if (c->p.ks_out_dim != c->p.lwe_dim) return fail(TP_E_INVALID, "reset needs ks_out_dim == lwe_dim");
+    }
+    return TP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tp_sap_homtrunc_device(tp_context* c, size_t count, uint32_t radix, const uint64_t* in, uint64_t* out) {
+    if (!c || !in || !out) return fail(TP_E_INVALID, "null argument");
+    int rc = sap_check(c, radix, false, true, false);
+    if (rc) return rc;
+    if (count == 0) return TP_OK;
+    if (count > 0x7fffffff) return fail(TP_E_INVALID, "count too large");
+    if ((rc = set_device(c)) || (rc = sap_prepare(c, radix))) return rc;
+    SapWork w{};
+    if ((rc = sap_workspace(c, count, int_log2_ceil(radix), w))) return rc;
+    return sap_homtrunc(c, w, count, radix, in, out);
+}
+
+int tp_sap_reset_device(tp_context* c, size_t count, uint64_t radix, uint64_t* acc) {
+    if (!c || !acc) return fail(TP_E_INVALID, "null argument");
+    int rc = sap_check(c, 2, true, false, true);
+    if (rc) return rc;
+    if (count == 0) return TP_OK;
+    if (count > 0x7fffffff) return fail(TP_E_INVALID, "count too large");
+    if ((rc = set_device(c))) return rc;
+    SapWork w{};
+    if ((rc = sap_workspace(c, count, 1, w)) || (rc = sap_upload_fresh(c, w))) return rc;
+    return sap_reset(c, w, count, radix, acc);
+}
+
+int tp_sap_decompose_device(tp_context* c, size_t count, uint32_t radix, uint32_t beta, uint64_t restart,
+                            const uint64_t* lwes, uint64_t* out, uint32_t max_digits, uint32_t* digits) {
+    if (!c || !lwes || !out) return fail(TP_E_INVALID, "null argument");
+    int rc = sap_check(c, radix, true, true, restart != UINT64_MAX && restart != 0);
+    if (rc) return rc;
+    if (restart == 0) return fail(TP_E_INVALID, "restart period must be positive (UINT64_MAX: never)");
+    if (beta > c->p.lwe_dim) return fail(TP_E_INVALID, "beta (key Hamming weight) exceeds the LWE dimension");
+    if (count == 0) return TP_OK;
+    if (count > 0x7fffffff) return fail(TP_E_INVALID, "count too large");
+    if ((rc = set_device(c)) || (rc = sap_prepare(c, radix))) return rc;
+    const uint32_t N = c->N, n = c->p.lwe_dim, lr = int_log2_ceil(radix);
+    SapWork w{};
+    if ((rc = sap_workspace(c, count, lr, w)) || (rc = sap_upload_fresh(c, w))) return rc;
+    const size_t rows = 2 * count;
+    TP_HIP(hipMemcpyAsync(w.cp, lwes, (size_t)(n + 1) * count * 8, hipMemcpyDeviceToDevice, c->stream));
+    TP_LAUNCH("SAP init", tp::launch_sap_init(lwes, w.cm, w.digits, (int)n, (int)count, (int)lr, c->stream));
+    std::vector<uint32_t> dig(count);
+    TP_HIP(hipMemcpyAsync(dig.data(), w.digits, count * 4, hipMemcpyDeviceToHost, c->stream));
+    TP_HIP(hipStreamSynchronize(c->stream));
+    uint32_t D = 0;
+    for (uint32_t d : dig) D = d > D ? d : D;
+    if (D > max_digits)
+        return fail(TP_E_INVALID, "an input needs " + std::to_string(D) + " digits, more than max_digits");
+    if (digits) TP_HIP(hipMemcpyAsync(digits, w.digits, count * 4, hipMemcpyDeviceToDevice, c->stream));
+    const size_t out_stride = (size_t)max_digits * (N + 1);
+    const uint64_t* xp = c->d_sap_polys;
+    for (uint32_t k = 0; k < D; ++k) {
+        TP_LAUNCH("SAP digit", tp::launch_sap_digit(w.cp, w.cm, w.sub, (int)n, (int)count, (int)lr, radix - 1,
+                                                    (uint64_t)radix * beta, beta, 2 * (uint64_t)N, c->stream));
+        // the accumulator enters the rotation in coefficients; the first one is Q/(2N) X^0 itself
+        const uint64_t* tv = w.fresh;
+        size_t tv_stride = 0;
+        if (k > 0) {
+            TP_LAUNCH("SAP inverse NTT", tp::launch_ntt_batch(c->logN, w.tv, w.acc, rows, c->d_itw, true, c->stream));
+            tv = w.tv;
+            tv_stride = 2 * (size_t)N;
+        }
+        if ((rc = launch_eval(c, count, w.sub, tv, tv_stride, nullptr, w.acc, 1, 0))) return rc;
+        TP_LAUNCH("SAP extraction product", tp::launch_sap_poly_mul(w.ext, w.acc, xp, rows, N, c->stream));
+        TP_LAUNCH("SAP inverse NTT", tp::launch_ntt_batch(c->logN, w.tv, w.ext, rows, c->d_itw, true, c->stream));
+        TP_LAUNCH("SAP extraction", tp::launch_sap_extract(out, w.tv, w.digits, k, out_stride, (int)count, N,
+                                                           c->stream));
+        if (k + 1 == D) break;  // the accumulator is not read again
+        if (restart != UINT64_MAX && (k + 1) % restart == 0) {
+            // :480-482 resets with radix 0 (inputs whose modulus reached 0 have no further digits)
+            if ((rc = sap_reset(c, w, count, 0, w.acc))) return rc;
+        }
+        if ((rc = sap_homtrunc(c, w, count, radix, w.acc, w.acc))) return rc;
+    }
     return TP_OK;
 }
 

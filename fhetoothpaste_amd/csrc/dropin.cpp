@@ -4,7 +4,9 @@
 // material (seeded key generation, the automorphism of a secret key).
 #include "../../include/toothpaste_amd.hpp"
 
+#include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <map>
 #include <mutex>
 #include <random>
@@ -31,10 +33,12 @@ void require(bool ok, const std::string& msg) {
     if (!ok) throw std::invalid_argument(msg);
 }
 
-// floor(log2 x), as the reference's IntLog2
+// ceil(log2 x) (0 for x <= 1), the reference's IntLog2 (utils/math_utils.cpp:11-26): exact for the
+// power-of-two bases and dimensions, and 64 for the Goldilocks modulus in the noise estimates.
 uint64_t int_log2(uint64_t x) {
+    if (x <= 1) return 0;
     uint64_t r = 0;
-    while (x >>= 1) ++r;
+    for (x -= 1; x; x >>= 1) ++r;
     return r;
 }
 
@@ -1110,4 +1114,211 @@ void LWEtoRGSWConverter::Convert(uint64_t* output, const uint64_t* const input) 
 
 void LWEtoRGSWConverter::Convert(std::vector<uint64_t>& output, const std::vector<uint64_t>& input) {
     Convert(output.data(), input.data());
+}
+
+// ------------------------------------------------------------------ SAP digit decomposition
+
+long double EstimateFailureProbability(long double variance, uint64_t modulus, uint64_t plaintext_modulus) {
+    const long double lQ = (long double)modulus, lP = (long double)plaintext_modulus;
+    return std::erfc(lQ / (2 * lP * std::sqrt(variance * 2)));
+}
+
+SAPDecompositionContext::SAPDecompositionContext(std::shared_ptr<OperatorContext<BlindRotator>> blind_rotation_context,
+                                                 std::shared_ptr<TraceEvaluationContext> trace_context,
+                                                 std::shared_ptr<LWEConversionContext> lwe_conversion_context,
+                                                 uint64_t default_radix)
+    : m_rotation_context(std::move(blind_rotation_context)), m_packing_context(std::move(trace_context)),
+      m_conversion_context(std::move(lwe_conversion_context)), m_default_radix(default_radix) {
+    m_restart_iter = ComputeRestartIterationForRadix(default_radix, IMPLICIT_SK_L0);
+}
+
+void SAPDecompositionContext::SetDefaultRadix(uint64_t new_radix) {
+    m_default_radix = new_radix;
+    m_restart_iter = ComputeRestartIterationForRadix(new_radix, IMPLICIT_SK_L0);
+}
+
+Container SAPDecompositionContext::GetInputContainer() const {
+    // the conversion's output dimension, modulus unknown until the values are seen (:36-45)
+    auto out = std::dynamic_pointer_cast<LWEContainerImpl>(
+        m_conversion_context->GetOutputContainer(m_conversion_context->GetInputContainer()));
+    require(out != nullptr, "conversion context has no LWE output container");
+    return std::make_shared<LWEContainerImpl>(out->GetN(), UINT64_MAX, 0.0);
+}
+
+long double SAPDecompositionContext::ComputeOutputVariance(long double) const {
+    // :47-60: per restart period, rotation -> cluster of N radix^2 / 12 -> key switch
+    const long double N = (long double)m_packing_context->GetDimension();
+    long double out_var = 0.0L, cluster_var = 0.0L;
+    for (uint64_t i = 0; i < m_restart_iter; ++i) {
+        const long double tmp_var = m_rotation_context->ComputeOutputVariance(out_var);
+        const long double next = tmp_var * N * std::pow((long double)m_default_radix, 2.0L) / 12.0L;
+        const long double next_out = m_conversion_context->ComputeOutputVariance(next);
+        const bool fixed = i > 0 && next == cluster_var && next_out == out_var;
+        cluster_var = next;
+        out_var = next_out;
+        if (fixed) break;
+    }
+    return cluster_var;
+}
+
+Container SAPDecompositionContext::GetOutputContainer(Container) const {
+    auto out = std::dynamic_pointer_cast<RLWEContainerImpl>(
+        m_rotation_context->GetOutputContainer(m_rotation_context->GetInputContainer()));
+    require(out != nullptr, "rotation context has no RLWE output container");
+    return std::make_shared<LWEContainerImpl>(out->GetN(), out->GetQ(), ComputeOutputVariance(0.0));
+}
+
+const uint64_t SAPDecompositionContext::ComputeRestartIterationForRadix(uint64_t radix, uint64_t sk_hamming) const {
+    // :81-152, step for step
+    const long double log_max_p_fail = -60;
+    long double last_state_extraction_var = 0, current_var = 0;
+    const uint64_t Q = m_packing_context->GetModulus(), N = m_packing_context->GetDimension();
+    int iter = 0;
+    const uint64_t worst_trunc_factor = radix, state_elem_bound = N / radix;
+    while (true) {
+        const long double blind_rotation_var = m_rotation_context->ComputeOutputVariance(current_var);
+        const long double trunc_addition_var = worst_trunc_factor * blind_rotation_var;
+        current_var = m_packing_context->ComputeOutputVariance(trunc_addition_var);
+        if (current_var == 0.0L) return UINT64_MAX;
+        const uint64_t factor = std::max(radix, state_elem_bound);
+        const long double extraction_var = (blind_rotation_var * N * factor * factor) / 12.0L;
+        const long double p_fail = EstimateFailureProbability(extraction_var, Q, factor);
+        // the reference loops for as long as the estimate stays below 2^-60; a cap stands in for
+        // its non-terminating case (an estimate that never grows)
+        if (iter >= (1 << 20)) return UINT64_MAX;
+        if (p_fail <= 0.0L) {
+            iter++;
+            continue;
+        }
+        if (std::log2(p_fail) <= log_max_p_fail) {
+            iter++;
+            last_state_extraction_var = extraction_var;
+            continue;
+        }
+        break;
+    }
+    const long double lwe_conversion_var = m_conversion_context->ComputeOutputVariance(last_state_extraction_var);
+    const long double ms_var = EstimateModulusSwitchingVariance(lwe_conversion_var, Q, 2 * N, BINARY, sk_hamming);
+    const long double lwe_p_fail = EstimateFailureProbability(ms_var, 2 * N, state_elem_bound);
+    if (lwe_p_fail <= 0.0L) return (uint64_t)iter;
+    if (std::log2(lwe_p_fail) <= log_max_p_fail) return (uint64_t)iter;
+    return (uint64_t)(iter - 1);  // the reference returns iter - 1 as uint64 (UINT64_MAX for iter 0)
+}
+
+std::unique_ptr<SAPDecomposer> SAPDecompositionContext::ConstructOperator(const std::vector<GenericKey>& keys) const {
+    require(keys.size() >= 2, "SAP key bundle is {LWE secret, RLWE secret}");
+    std::vector<GenericKey> conv_keys = {GenericKey(keys[1].GetKey()), GenericKey(keys[0].GetKey())};
+    std::vector<GenericKey> pack_keys = {GenericKey(keys[1].GetKey())};
+    uint64_t lwe_l0 = 0;
+    for (uint64_t v : keys[0].GetKey()) lwe_l0 += v != 0;
+    auto rotator = m_rotation_context->ConstructOperator(keys);
+    auto packer = m_packing_context->ConstructOperator(pack_keys);
+    auto conv = m_conversion_context->ConstructOperator(conv_keys);
+    return std::unique_ptr<SAPDecomposer>(new SAPDecomposer(shared_from_this(), std::move(rotator), std::move(packer),
+                                                            std::move(conv), m_default_radix, lwe_l0, m_restart_iter));
+}
+
+SAPDecomposer::SAPDecomposer(std::shared_ptr<const SAPDecompositionContext> ctx, std::unique_ptr<BlindRotator> rotator,
+                             std::unique_ptr<TraceEvaluator> packer, std::unique_ptr<LWEtoLWEConverter> conv,
+                             uint64_t max_radix, uint64_t lwe_sk_hamming_weight, uint64_t reset_period)
+    : m_context(std::move(ctx)), m_rotator(std::move(rotator)), m_packer(std::move(packer)),
+      m_lwe_converter(std::move(conv)), m_beta(lwe_sk_hamming_weight), m_max_radix(max_radix),
+      m_restart_iteration(reset_period) {
+    // one device context with the rotation key, the automorphism keys and the reset's switching key
+    tp_context* rot = m_rotator->DeviceContext();
+    require(rot != nullptr, "SAP decomposition needs a GPU blind rotator of this library");
+    const std::vector<uint64_t>* brk = nullptr;
+    if (auto* c = dynamic_cast<CGGIBlindRotator*>(m_rotator.get())) brk = &c->GetBlindRotationKey();
+    else if (auto* b = dynamic_cast<BMMPBlindRotator*>(m_rotator.get())) brk = &b->GetBlindRotationKey();
+    require(brk != nullptr, "SAP decomposition needs a CGGI or BMMP rotator");
+    tp_params p{};
+    check(tp_context_params(rot, &p), "tp_context_params");
+    auto trace = m_context->GetTraceContext();
+    auto conv_ctx = m_lwe_converter->GetContext();
+    require(trace->GetDimension() == p.ring_dim, "trace and rotation ring dimensions differ");
+    require(conv_ctx->GetSourceDimension() == p.ring_dim, "reset key switch must start from dimension N");
+    p.rlwe_ks_base_log2 = (uint32_t)trace->GetGadgetBasisLog2();
+    p.rlwe_ks_digits = (uint32_t)trace->GetGadgetDigits();
+    p.ks_out_dim = (uint32_t)conv_ctx->GetTargetDimension();
+    p.ks_base_log2 = (uint32_t)conv_ctx->GetGadgetBasisLog2();
+    p.ks_digits = (uint32_t)conv_ctx->GetGadgetDigits();
+    m_dev = make_device_context(p);
+    check(tp_import_brk(m_dev.get(), brk->data(), brk->size()), "tp_import_brk");
+    const auto& ak = m_packer->GetAutomorphismKeys();
+    check(tp_import_auto_keys(m_dev.get(), ak.data(), ak.size()), "tp_import_auto_keys");
+    const auto& ksk = m_lwe_converter->GetSwitchingKey();
+    check(tp_import_lwe_ksk(m_dev.get(), ksk.data(), ksk.size()), "tp_import_lwe_ksk");
+}
+
+uint64_t SAPDecomposer::RestartFor(uint64_t radix) const {
+    // :437-442
+    if (radix == m_max_radix && m_beta <= SAPDecompositionContext::IMPLICIT_SK_L0) return m_restart_iteration;
+    return m_context->ComputeRestartIterationForRadix(radix, m_beta);
+}
+
+void SAPDecomposer::DecomposeBatch(uint64_t* output, const uint64_t* inputs, uint64_t count, uint64_t radix,
+                                   uint64_t max_digits, uint32_t* digit_counts) {
+    tp_params p{};
+    check(tp_context_params(m_dev.get(), &p), "tp_context_params");
+    const uint64_t N = p.ring_dim, n1 = (uint64_t)p.lwe_dim + 1;
+    require(max_digits >= 1 && max_digits <= 64, "max_digits must lie in [1, 64]");
+    tp_context* x = m_dev.get();
+    DevBuf d_in(x, count * n1), d_out(x, count * max_digits * (N + 1)), d_cnt(x, (count + 1) / 2);
+    d_in.put(inputs, count * n1);
+    d_out.put(output, count * max_digits * (N + 1));  // rows an input does not reach keep their words
+    check(tp_sap_decompose_device(x, count, (uint32_t)radix, (uint32_t)m_beta, RestartFor(radix), d_in.p, d_out.p,
+                                  (uint32_t)max_digits, reinterpret_cast<uint32_t*>(d_cnt.p)),
+          "SAP Decompose");
+    d_out.get(output, count * max_digits * (N + 1));
+    if (digit_counts) {
+        std::vector<uint64_t> c((count + 1) / 2);
+        d_cnt.get(c.data(), c.size());
+        std::memcpy(digit_counts, c.data(), count * sizeof(uint32_t));
+    }
+}
+
+void SAPDecomposer::Decompose(uint64_t* output, const uint64_t* const input, uint64_t radix) {
+    tp_params p{};
+    check(tp_context_params(m_dev.get(), &p), "tp_context_params");
+    const uint64_t N = p.ring_dim, W = N + 1;
+    std::vector<uint64_t> digits(64 * W, 0);
+    uint32_t cnt[2] = {0, 0};
+    DecomposeBatch(digits.data(), input, 1, radix, 64, cnt);
+    // SampleExtract leaves out[i] alone where the coefficient it negates is 0 (sample_extraction.h:29-33);
+    // such words are the only zeros among out[1..N) that it would write
+    for (uint32_t k = 0; k < cnt[0]; ++k) {
+        const uint64_t* d = digits.data() + k * W;
+        uint64_t* o = output + k * W;
+        o[0] = d[0];
+        for (uint64_t i = 1; i < N; ++i)
+            if (d[i] != 0) o[i] = d[i];
+        o[N] = d[N];
+    }
+}
+
+void SAPDecomposer::Decompose(std::vector<uint64_t>& output, const std::vector<uint64_t>& input, uint64_t radix) {
+    Decompose(output.data(), input.data(), radix);
+}
+
+void SAPDecomposer::HomTrunc(uint64_t* output, uint64_t* input, uint64_t radix) {
+    require(radix && (radix & (radix - 1)) == 0, "radix must be a power of two");
+    tp_params p{};
+    check(tp_context_params(m_dev.get(), &p), "tp_context_params");
+    const uint64_t N = p.ring_dim;
+    tp_context* x = m_dev.get();
+    DevBuf d_in(x, 2 * N), d_out(x, 2 * N);
+    d_in.put(input, 2 * N);
+    check(tp_sap_homtrunc_device(x, 1, (uint32_t)radix, d_in.p, d_out.p), "SAP HomTrunc");
+    d_out.get(output, 2 * N);
+}
+
+void SAPDecomposer::ResetAccumulator(uint64_t* acc, uint64_t radix) {
+    tp_params p{};
+    check(tp_context_params(m_dev.get(), &p), "tp_context_params");
+    const uint64_t N = p.ring_dim;
+    tp_context* x = m_dev.get();
+    DevBuf d(x, 2 * N);
+    d.put(acc, 2 * N);
+    check(tp_sap_reset_device(x, 1, radix, d.p), "SAP ResetAccumulator");
+    d.get(acc, 2 * N);
 }

@@ -214,6 +214,36 @@ class Context:
         check(lib().tp_modswitch_device(self.h, d.p, d.p, v.size, source_modulus, target_modulus, kind, seed))
         return d.to_host().reshape(v.shape)
 
+    def sap_decompose(self, lwes, radix, beta, restart=None, max_digits=64):
+        """SAPDecomposer::Decompose of [count][n+1] LWEs: returns (digits [count][D][N+1], counts)
+        with D the largest digit count; rows beyond an input's own count are zero."""
+        x = np.ascontiguousarray(lwes, dtype=np.uint64).reshape(-1, self.n + 1)
+        count = x.shape[0]
+        d_in = DeviceBuffer.from_host(self, x)
+        d_out = DeviceBuffer.from_host(self, np.zeros(count * max_digits * (self.N + 1), np.uint64))
+        d_cnt = DeviceBuffer(self, (count + 1) // 2)
+        r = (1 << 64) - 1 if restart is None else restart
+        check(lib().tp_sap_decompose_device(self.h, count, radix, beta, r, d_in.p, d_out.p, max_digits, d_cnt.p))
+        counts = np.frombuffer(d_cnt.to_host().tobytes(), dtype=np.uint32)[:count].copy()
+        D = int(counts.max()) if count else 0
+        out = d_out.to_host().reshape(count, max_digits, self.N + 1)[:, :D]
+        return out, counts
+
+    def sap_homtrunc(self, rlwes_ntt, radix):
+        """SAPDecomposer::HomTrunc of [count][2N] NTT-domain RLWEs."""
+        x = np.ascontiguousarray(rlwes_ntt, dtype=np.uint64).reshape(-1, 2 * self.N)
+        d_in = DeviceBuffer.from_host(self, x)
+        d_out = DeviceBuffer(self, x.size)
+        check(lib().tp_sap_homtrunc_device(self.h, x.shape[0], radix, d_in.p, d_out.p))
+        return d_out.to_host().reshape(x.shape)
+
+    def sap_reset(self, acc_ntt, radix):
+        """SAPDecomposer::ResetAccumulator of [count][2N] NTT-domain accumulators."""
+        x = np.ascontiguousarray(acc_ntt, dtype=np.uint64).reshape(-1, 2 * self.N)
+        d = DeviceBuffer.from_host(self, x)
+        check(lib().tp_sap_reset_device(self.h, x.shape[0], radix, d.p))
+        return d.to_host().reshape(x.shape)
+
     def eltwise(self, op, a, b=None, c=None, scalar=0):
         a = np.ascontiguousarray(a, dtype=np.uint64)
         bufs = [DeviceBuffer.from_host(self, np.ascontiguousarray(v, dtype=np.uint64)) if v is not None else None

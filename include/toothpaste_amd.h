@@ -163,6 +163,28 @@ int tp_eltwise_device(tp_context* ctx, int op, uint64_t* dst, const uint64_t* a,
 int tp_modswitch_device(tp_context* ctx, uint64_t* dst, const uint64_t* src, size_t words, uint64_t source_modulus,
                         uint64_t target_modulus, int type, uint64_t seed);
 
+/* SAP digit decomposition, batched (lib/include/operators/sap_decomposition.h:137-228,
+ * lib/src/operators/sap_decomposer.cpp:190-494).  One context holds the blind-rotation key
+ * (SAPDecomposer::m_rotator), the automorphism keys (m_packer, a TraceEvaluator) and, for the
+ * reset, the LWE key-switching key N -> lwe_dim (m_lwe_converter).  radix: power of two in [2, N].
+ *
+ * tp_sap_decompose_device replaces a loop of SAPDecomposer::Decompose(output, input, radix)
+ * (:375-494) over `count` inputs.  lwes [count][lwe_dim+1] of a power-of-two modulus read off the
+ * values as the reference does (:427-433); beta = Hamming weight of the LWE key (ConstructOperator,
+ * :172-176); restart = digit period of the accumulator reset (the reference's
+ * restart iteration; UINT64_MAX = never, its value when the trace noise estimate is 0).
+ * out [count][max_digits][N+1]: digit k of input i at out + (i max_digits + k)(N+1), written for
+ * k < digits[i]; digits (device, may be NULL) [count] receives each input's digit count.  Fails
+ * with TP_E_INVALID, before any launch, when an input needs more than max_digits digits.
+ *
+ * tp_sap_homtrunc_device: SAPDecomposer::HomTrunc (:190-312), in/out [count][2N] NTT domain
+ * (out may equal in).  tp_sap_reset_device: SAPDecomposer::ResetAccumulator (:314-373), acc
+ * [count][2N] NTT domain in place; any radix (the reference resets with 0 inside Decompose). */
+int tp_sap_decompose_device(tp_context* ctx, size_t count, uint32_t radix, uint32_t beta, uint64_t restart,
+                            const uint64_t* lwes, uint64_t* out, uint32_t max_digits, uint32_t* digits);
+int tp_sap_homtrunc_device(tp_context* ctx, size_t count, uint32_t radix, const uint64_t* in, uint64_t* out);
+int tp_sap_reset_device(tp_context* ctx, size_t count, uint64_t radix, uint64_t* acc);
+
 /* Negacyclic NTT over Z_Q[X]/(X^N+1) of `count` polynomials (device pointers).
  * Replace SolinasWorker::ForwardNTT / BackwardNTT (lib/src/backend/solinas.cpp:228-313);
  * dst may equal src. */

@@ -563,6 +563,7 @@ struct TraceEvaluationContext : public OperatorContext<TraceEvaluator>,
     [[nodiscard]] uint64_t GetModulus() const { return m_auto_context->GetModulus(); }
     [[nodiscard]] std::shared_ptr<MathWorker> GetNTT() const { return m_auto_context->GetNTT(); }
     [[nodiscard]] uint64_t GetGadgetBasisLog2() const { return m_auto_context->GetGadgetBasisLog2(); }
+    [[nodiscard]] uint64_t GetGadgetDigits() const { return m_auto_context->GetGadgetDigits(); }
     [[nodiscard]] KeyDistribution GetSourceKeyDistribution() const { return m_auto_context->GetSourceKeyDistribution(); }
     [[nodiscard]] long double ComputeOutputVariance(long double input_variance = 0.0) const override;
     [[nodiscard]] std::unique_ptr<TraceEvaluator> ConstructOperator(const std::vector<GenericKey>& keys) const override;
@@ -691,6 +692,94 @@ private:
     std::unique_ptr<TraceEvaluator> m_trace_eval;
     std::unique_ptr<RLWEtoRLWEConverter> m_square_eval;
     std::vector<uint64_t> m_acc, m_extraction_buffer;
+};
+
+// ------------------------------------------------------------------ SAP digit decomposition
+
+// A radix decomposer maps an encryption of a datum to encryptions of its digits
+// (lib/include/interfaces/decomposition_operator.h:13-55).
+template <typename ContextType>
+struct RadixDecomposer {
+    virtual void Decompose(uint64_t* output, const uint64_t* const input, uint64_t radix) = 0;
+    virtual void Decompose(std::vector<uint64_t>& output, const std::vector<uint64_t>& input, uint64_t radix) = 0;
+    virtual const std::shared_ptr<const ContextType> GetContext() const = 0;
+    virtual ~RadixDecomposer() = default;
+};
+
+// erfc(Q / (2 P sqrt(2 var))) (lib/src/utils/math_utils.cpp:87-96)
+long double EstimateFailureProbability(long double variance, uint64_t modulus, uint64_t plaintext_modulus);
+
+struct SAPDecomposer;
+// lib/include/operators/sap_decomposition.h:13-134, lib/src/operators/sap_decomposer.cpp:14-188
+struct SAPDecompositionContext : OperatorContext<SAPDecomposer>,
+                                 public std::enable_shared_from_this<SAPDecompositionContext> {
+    static const uint64_t IMPLICIT_SK_L0 = 32;
+    SAPDecompositionContext(std::shared_ptr<OperatorContext<BlindRotator>> blind_rotation_context,
+                            std::shared_ptr<TraceEvaluationContext> trace_context,
+                            std::shared_ptr<LWEConversionContext> lwe_conversion_context, uint64_t default_radix);
+    // Variance of a digit after the restart period.  The reference's loop counter is 32-bit
+    // (sap_decomposer.cpp:55) and never reaches a restart period of UINT64_MAX; this one stops at
+    // the fixed point, which is the reference's value whenever its loop ends.
+    long double ComputeOutputVariance(long double input_variance = 0) const override;
+    [[nodiscard]] std::shared_ptr<OperatorContext<BlindRotator>> GetBlindRotationContext() const {
+        return m_rotation_context;
+    }
+    [[nodiscard]] std::shared_ptr<TraceEvaluationContext> GetTraceContext() const { return m_packing_context; }
+    [[nodiscard]] std::shared_ptr<LWEConversionContext> GetLWEConversionContext() const { return m_conversion_context; }
+    [[nodiscard]] const uint64_t GetDefaultRadix() const { return m_default_radix; }
+    [[nodiscard]] const uint64_t GetRestartIteration() const { return m_restart_iter; }
+    // Digits after which the accumulator is refreshed (sap_decomposer.cpp:81-152); UINT64_MAX when
+    // the trace's noise estimate is 0, as it is for noise-free automorphism inputs.
+    [[nodiscard]] const uint64_t ComputeRestartIterationForRadix(uint64_t radix, uint64_t sk_hamming) const;
+    void SetBlindRotationContext(std::shared_ptr<OperatorContext<BlindRotator>> c) { m_rotation_context = std::move(c); }
+    void SetTraceContext(std::shared_ptr<TraceEvaluationContext> c) { m_packing_context = std::move(c); }
+    void SetLWEConversionContext(std::shared_ptr<LWEConversionContext> c) { m_conversion_context = std::move(c); }
+    void SetDefaultRadix(uint64_t new_radix);
+    [[nodiscard]] Container GetInputContainer() const override;
+    [[nodiscard]] Container GetOutputContainer(Container container) const override;
+    // keys = {LWE secret, RLWE secret}: rotation keys from both, automorphism keys from keys[1],
+    // the reset's key switch from keys[1] to keys[0] (sap_decomposer.cpp:167-186)
+    std::unique_ptr<SAPDecomposer> ConstructOperator(const std::vector<GenericKey>& keys) const override;
+    [[nodiscard]] OperatorID GetOperatorID() const override { return DECOMP_SAP; }
+
+private:
+    std::shared_ptr<OperatorContext<BlindRotator>> m_rotation_context;
+    std::shared_ptr<TraceEvaluationContext> m_packing_context;
+    std::shared_ptr<LWEConversionContext> m_conversion_context;
+    uint64_t m_default_radix, m_restart_iter;
+};
+
+// lib/include/operators/sap_decomposition.h:137-229.  The three operators' keys are gathered in
+// one device context, where the batched pipeline runs.
+struct SAPDecomposer : RadixDecomposer<SAPDecompositionContext> {
+    friend struct SAPDecompositionContext;
+    // digits of input (n + 1 words, power-of-two modulus read off the values) in radix, each an LWE
+    // of dimension N; writes ceil(log2 q / log2 radix) x (N + 1) words (sap_decomposer.cpp:375-494)
+    void Decompose(uint64_t* output, const uint64_t* const input, uint64_t radix) override;
+    void Decompose(std::vector<uint64_t>& output, const std::vector<uint64_t>& input, uint64_t radix) override;
+    // Batched: inputs [count][n+1] -> output [count][max_digits][N+1], digit_counts[count] (may be
+    // null).  No counterpart in the reference; one launch per step for the whole batch.
+    void DecomposeBatch(uint64_t* output, const uint64_t* inputs, uint64_t count, uint64_t radix, uint64_t max_digits,
+                        uint32_t* digit_counts = nullptr);
+    [[nodiscard]] const std::shared_ptr<const SAPDecompositionContext> GetContext() const override { return m_context; }
+    ~SAPDecomposer() override = default;
+    // RLWE(X^c) -> RLWE(X^floor(c / radix)), NTT domain; output must differ from input (:190-312)
+    void HomTrunc(uint64_t* output, uint64_t* input, uint64_t radix);
+    // refresh of an accumulator RLWE(X^c) (NTT domain, in place) (:314-373)
+    void ResetAccumulator(uint64_t* acc, uint64_t radix);
+    [[nodiscard]] tp_context* DeviceContext() const { return m_dev.get(); }
+
+private:
+    SAPDecomposer(std::shared_ptr<const SAPDecompositionContext> ctx, std::unique_ptr<BlindRotator> rotator,
+                  std::unique_ptr<TraceEvaluator> packer, std::unique_ptr<LWEtoLWEConverter> conv, uint64_t max_radix,
+                  uint64_t lwe_sk_hamming_weight, uint64_t reset_period);
+    uint64_t RestartFor(uint64_t radix) const;
+    std::shared_ptr<const SAPDecompositionContext> m_context;
+    std::unique_ptr<BlindRotator> m_rotator;
+    std::unique_ptr<TraceEvaluator> m_packer;
+    std::unique_ptr<LWEtoLWEConverter> m_lwe_converter;
+    uint64_t m_beta, m_max_radix, m_restart_iteration;
+    toothpaste_amd::DeviceContext m_dev;
 };
 
 // ------------------------------------------------------------------ function evaluation

@@ -75,6 +75,11 @@ def lib():
             "or_pack": (ctypes.c_int, [ctypes.c_void_p, u64p, u64p, U32, u64p, U32, U32]),
             "or_lwe_to_rgsw": (ctypes.c_int, [ctypes.c_void_p, u64p, u64p, u64p, U32, U32, U32, ctypes.c_int, u64p,
                                                U32, U32, u64p, U32, U32, U32, U32]),
+            "or_sap_homtrunc": (None, [ctypes.c_void_p, u64p, u64p, U32, u64p, U32, U32]),
+            "or_sap_reset": (ctypes.c_int, [ctypes.c_void_p, u64p, U32, u64p, U32, U32, U32, U64, u64p, U32, U32, U32,
+                                            ctypes.c_int]),
+            "or_sap_decompose": (ctypes.c_int, [ctypes.c_void_p, u64p, u64p, U32, U32, u64p, U32, U32, U32,
+                                                ctypes.c_int, u64p, U32, U32, U64, u64p, U32, U32]),
             "or_modswitch": (None, [u64p, ctypes.c_uint64, U64, U64, ctypes.c_int]),
             "or_pack_consecutive": (ctypes.c_int, [ctypes.c_void_p, u64p, u64p, U32, u64p, U32, U32]),
             "or_bootstrap_batch": (ctypes.c_double, [ctypes.c_void_p, u64p, u64p, ctypes.c_size_t, u64p, u64p,
@@ -292,3 +297,28 @@ def lwe_to_rgsw(N, lwe, brk, n, bl, ll, method, trace_keys, bt, dt, sq_key, bs, 
                               P(trace_keys), bt, dt, P(sq_key), bs, ds, digits, basebits)
     assert rc == 0, rc
     return out
+
+
+def sap_homtrunc(N, rlwe_ntt, radix, keys, b, d):
+    out = arr(2 * N)
+    lib().or_sap_homtrunc(Ring(N).h, P(out), P(np.ascontiguousarray(rlwe_ntt, np.uint64)), radix, P(keys), b, d)
+    return out
+
+
+def sap_reset(N, acc_ntt, radix, ksk, n_out, kb, kd, qks, brk, n, bl, ll, method):
+    acc = np.ascontiguousarray(acc_ntt, np.uint64).copy()
+    rc = lib().or_sap_reset(Ring(N).h, P(acc), radix, P(ksk), n_out, kb, kd, qks, P(brk), n, bl, ll, method)
+    assert rc == 0, rc
+    return acc
+
+
+def sap_decompose(N, lwe, radix, beta, brk, n, bl, ll, method, keys, b, d, max_digits=64, restart=None, ksk=None,
+                  kb=0, kd=0):
+    """Digits [D][N+1]; restart/ksk (N -> n, base 2^kb, kd digits) for the periodic accumulator reset."""
+    out = arr(max_digits * (N + 1))
+    r = (1 << 64) - 1 if restart is None else restart
+    k = ksk if ksk is not None else arr(1)
+    cnt = lib().or_sap_decompose(Ring(N).h, P(out), P(np.ascontiguousarray(lwe, np.uint64)), radix, beta, P(brk), n,
+                                 bl, ll, method, P(keys), b, d, r, P(k), kb, kd)
+    assert cnt >= 0
+    return out[:cnt * (N + 1)].reshape(cnt, N + 1)

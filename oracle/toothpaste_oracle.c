@@ -1051,6 +1051,200 @@ done:
 }
 
 /* ------------------------------------------------------------------------ */
+/* SAP digit decomposition (sap_decomposer.cpp:190-494).                    */
+/* ------------------------------------------------------------------------ */
+
+static uint32_t ilog2_ceil(uint64_t x) { /* IntLog2 (utils/math_utils.cpp:11-26) */
+    uint32_t r = 0;
+    if (x <= 1) return 0;
+    for (x -= 1; x; x >>= 1) ++r;
+    return r;
+}
+
+/* TraceEvaluator::EvalAuto(out, in, t) of the pointer overload: automorphism key log2(t-1) - 1,
+ * subtracted into out (trace_evaluation.cpp:172-179, automorphism_evaluation.cpp:150-159). */
+static void eval_auto_acc(const or_ring* R, uint64_t* out, const uint64_t* in, uint64_t t, const uint64_t* keys,
+                          uint32_t b, uint32_t d) {
+    const uint32_t N = R->N;
+    const uint32_t j = ilog2_ceil(t - 1);
+    uint64_t* tmp = (uint64_t*)malloc(sizeof(uint64_t) * 2 * N);
+    or_automorphism(R, tmp, in, t, keys + (size_t)(j - 1) * d * 2 * N, b, d);
+    vadd(out, out, tmp, 2 * N);
+    free(tmp);
+}
+
+/* SAPDecomposer::HomTrunc (:190-312): input RLWE (NTT) -> output RLWE (NTT) whose phase keeps
+ * X^(e >> log2 radix) of a phase X^e, through a partial trace and the automorphism tree. */
+OR_EXPORT void or_sap_homtrunc(const or_ring* R, uint64_t* output, const uint64_t* input, uint32_t radix,
+                               const uint64_t* keys, uint32_t b, uint32_t d) {
+    const uint32_t N = R->N, logN = R->logN, lr = ilog2_ceil(radix);
+    /* the reference's buffer holds log2 N slots of 2N (:511); the tree reaches slot
+     * log2(N/radix) + 2, beyond it for radix 2, so this one holds log2 N + 3 */
+    const size_t pk_words = (size_t)2 * logN * N;
+    uint64_t* pk = (uint64_t*)calloc(pk_words + (size_t)6 * N, sizeof(uint64_t));
+    uint64_t* poly = (uint64_t*)calloc(N, sizeof(uint64_t));
+    uint64_t* ares = (uint64_t*)calloc((size_t)2 * N, sizeof(uint64_t));
+    uint64_t* sp = (uint64_t*)calloc(N, sizeof(uint64_t));
+    memset(output, 0, sizeof(uint64_t) * 2 * N);
+    for (uint32_t i = N - (radix - 1); i < N; ++i) poly[i] = OR_Q - 1;  /* sum_i X^-i, i < radix */
+    poly[0] = 1;
+    ntt_fwd(R, poly);
+    for (uint32_t i = 0; i < 2 * N; ++i) pk[i] = or_mul(input[i], R->n_inv);
+    vmul(pk, pk, poly, N);
+    vmul(pk + N, pk + N, poly, N);
+    for (uint32_t i = 0; i < lr; ++i) {
+        const uint64_t t = (1ULL << (logN - i)) + 1;
+        memcpy(ares, pk, sizeof(uint64_t) * 2 * N);
+        ntt_inv(R, ares);
+        ntt_inv(R, ares + N);
+        eval_auto_acc(R, pk + 2 * N, ares, t, keys, b, d);
+        vadd(pk, pk, pk + 2 * N, 2 * N);
+        memset(pk + 2 * N, 0, sizeof(uint64_t) * 2 * N);
+    }
+    memset(pk + 2 * N, 0, sizeof(uint64_t) * (pk_words - 2 * N));
+    uint32_t lvl[128];
+    uint64_t ex[128];
+    uint32_t head = 1;
+    lvl[0] = logN - lr;
+    ex[0] = 0;
+    const uint32_t rev_bits = logN - lr;
+    while (head != 0) {
+        --head;
+        const uint32_t level = lvl[head];
+        const uint64_t e = ex[head];
+        uint64_t* cur = pk + (size_t)head * 2 * N;
+        memset(sp, 0, sizeof(uint64_t) * N);
+        if (level == 0) {
+            const uint64_t eb = rev_bits ? brv((uint32_t)e, rev_bits) : 0;
+            sp[eb] = 1;
+            ntt_fwd(R, sp);
+            vmul(cur, cur, sp, N);
+            vmul(cur + N, cur + N, sp, N);
+            vadd(output, output, cur, 2 * N);
+            memset(cur, 0, sizeof(uint64_t) * 2 * N);
+            continue;
+        }
+        sp[N - (N >> level)] = OR_Q - 1;  /* X^(-N / 2^level) */
+        ntt_fwd(R, sp);
+        memcpy(cur + 2 * N, cur, sizeof(uint64_t) * 2 * N);
+        ntt_inv(R, cur + 2 * N);
+        ntt_inv(R, cur + 3 * N);
+        eval_auto_acc(R, cur + 4 * N, cur + 2 * N, (1ULL << level) + 1, keys, b, d);
+        vsub(cur + 2 * N, cur, cur + 4 * N, 2 * N);
+        vmul(cur + 2 * N, cur + 2 * N, sp, N);
+        vmul(cur + 3 * N, cur + 3 * N, sp, N);
+        vadd(cur, cur, cur + 4 * N, 2 * N);
+        memset(cur + 4 * N, 0, sizeof(uint64_t) * 2 * N);
+        lvl[head] = level - 1;
+        ex[head] = 2 * e;
+        lvl[head + 1] = level - 1;
+        ex[head + 1] = 2 * e + 1;
+        head += 2;
+    }
+    free(pk);
+    free(poly);
+    free(ares);
+    free(sp);
+}
+
+/* SAPDecomposer::ResetAccumulator (:314-373) with the key switch and (when Qks != Q, power of two)
+ * the ROUND switch to Qks; the FLOOR switch to 2N; acc (NTT) is replaced by a fresh rotation of
+ * Q/(2N) by the extracted LWE. */
+OR_EXPORT int or_sap_reset(const or_ring* R, uint64_t* acc, uint32_t radix, const uint64_t* ksk, uint32_t n_out,
+                           uint32_t kb, uint32_t kd, uint64_t qks, const uint64_t* brk, uint32_t n, uint32_t bl,
+                           uint32_t ll, int method) {
+    const uint32_t N = R->N, rl = ilog2_ceil(radix);
+    uint64_t* poly = (uint64_t*)calloc(N, sizeof(uint64_t));
+    uint64_t* ext = (uint64_t*)calloc((size_t)N + 1, sizeof(uint64_t));
+    uint64_t* ln = (uint64_t*)calloc((size_t)n_out + 1, sizeof(uint64_t));
+    uint64_t* fresh = (uint64_t*)calloc((size_t)2 * N, sizeof(uint64_t));
+    int rc = 0;
+    for (uint32_t i = 1; i < N; ++i) poly[i] = neg((uint64_t)(((N - i) >> rl) << rl));
+    ntt_fwd(R, poly);
+    vmul(acc, acc, poly, N);
+    vmul(acc + N, acc + N, poly, N);
+    ntt_inv(R, acc);
+    ntt_inv(R, acc + N);
+    or_sample_extract(ext, acc, N);
+    if (qks != OR_Q) or_modswitch(ext, (uint64_t)N + 1, OR_Q, qks, 0);
+    rc = or_lwe_keyswitch(ln, ext, ksk, N, n_out, kb, kd);
+    if (!rc) {
+        or_modswitch(ln, (uint64_t)n_out + 1, qks, 2 * (uint64_t)N, 1);
+        fresh[N] = OR_Q / (2 * (uint64_t)N);
+        rc = or_blind_rotate(R, acc, ln, fresh, brk, n, bl, ll, method, 0);
+    }
+    free(poly);
+    free(ext);
+    free(ln);
+    free(fresh);
+    return rc;
+}
+
+/* SAPDecomposer::Decompose (:375-494): digits of an LWE of power-of-two modulus, radix digits at a
+ * time, each a blind rotation of the perturbed sub-phase followed by the x -> x mod radix
+ * extraction and HomTrunc.  beta = Hamming weight of the LWE key.  out: digits x (N+1); returns the
+ * digit count.  restart: the digit period of the accumulator reset (:480-482; UINT64_MAX = never,
+ * the reference's own value, as its trace noise estimate is 0, :101-104); the reset key switches
+ * N -> n with ksk (base 2^kb, kd digits). */
+OR_EXPORT int or_sap_decompose(const or_ring* R, uint64_t* out, const uint64_t* input, uint32_t radix, uint32_t beta,
+                               const uint64_t* brk, uint32_t n, uint32_t bl, uint32_t ll, int method,
+                               const uint64_t* keys, uint32_t b, uint32_t d, uint64_t restart, const uint64_t* ksk,
+                               uint32_t kb, uint32_t kd) {
+    const uint32_t N = R->N, rl = ilog2_ceil(radix);
+    const uint64_t mask = (uint64_t)radix - 1, lwe_mod_in = 2 * (uint64_t)N;
+    const uint64_t plo = (uint64_t)radix * beta, phi = beta;
+    uint64_t* acc = (uint64_t*)calloc((size_t)2 * N, sizeof(uint64_t));
+    uint64_t* xp = (uint64_t*)calloc(N, sizeof(uint64_t));
+    uint64_t* ext = (uint64_t*)calloc((size_t)2 * N, sizeof(uint64_t));
+    uint64_t* tmp = (uint64_t*)calloc((size_t)2 * N, sizeof(uint64_t));
+    uint64_t* cp = (uint64_t*)malloc(sizeof(uint64_t) * (n + 1));
+    uint64_t* sub = (uint64_t*)malloc(sizeof(uint64_t) * (n + 1));
+    int digit = 0;
+    acc[N] = OR_Q / (2 * (uint64_t)N);
+    ntt_fwd(R, acc + N);
+    for (uint32_t i = 1; i < N; ++i) xp[N - i] = neg(i & mask);
+    ntt_fwd(R, xp);
+    memcpy(cp, input, sizeof(uint64_t) * (n + 1));
+    /* 1 << IntLog2(a_0): the reference's x86 shift takes the count mod 64, so 1 << 64 is 1 */
+    uint64_t cm = 1ULL << (ilog2_ceil(input[0]) & 63);
+    for (uint32_t i = 1; i <= n; ++i)
+        if (cm <= input[i]) cm <<= 1;
+    while (cm > 1) {
+        for (uint32_t i = 0; i <= n; ++i) {
+            sub[i] = cp[i] & mask;
+            cp[i] >>= rl;
+        }
+        cm >>= rl;
+        sub[n] = (uint64_t)(((unsigned __int128)sub[n] + plo) % lwe_mod_in);
+        if (cm > 1) cp[n] = (cp[n] % cm + cm - phi % cm) % cm;
+        ntt_inv(R, acc);
+        ntt_inv(R, acc + N);
+        memcpy(tmp, acc, sizeof(uint64_t) * 2 * N);
+        int rc = or_blind_rotate(R, acc, sub, tmp, brk, n, bl, ll, method, 0);
+        if (rc) { digit = -1; break; }
+        vmul(ext, acc, xp, N);
+        vmul(ext + N, acc + N, xp, N);
+        ntt_inv(R, ext);
+        ntt_inv(R, ext + N);
+        or_sample_extract(out + (size_t)digit * (N + 1), ext, N);
+        if (restart != UINT64_MAX && (uint64_t)(digit + 1) % restart == 0 && cm != 0) {
+            /* :480-482: reset with radix 0, key switch N -> n, no switch to another KS modulus */
+            if (or_sap_reset(R, acc, 0, ksk, n, kb, kd, OR_Q, brk, n, bl, ll, method)) { digit = -1; break; }
+        }
+        or_sap_homtrunc(R, tmp, acc, radix, keys, b, d);
+        memcpy(acc, tmp, sizeof(uint64_t) * 2 * N);
+        ++digit;
+    }
+    free(acc);
+    free(xp);
+    free(ext);
+    free(tmp);
+    free(cp);
+    free(sub);
+    return digit;
+}
+
+/* ------------------------------------------------------------------------ */
 /* Batched bootstrap for the CPU baseline: EvalFunc (negacyclic_function_    */
 /* evaluator.cpp:92-101) + LWE key switch, one rotator per thread.          */
 /* ------------------------------------------------------------------------ */

@@ -290,6 +290,42 @@ int main(int argc, char** argv) {
             dump("rgsw_out", out);
         }
 
+        step("sap decomposition");
+        {
+            const uint64_t n = 16, qin = 1ULL << 40;
+            auto lk = secret(n, 0, 112, 1), rk = secret(N, 0, 112, 2);
+            auto br = std::make_shared<CGGIBlindRotationContext>(BINARY, Q, N, n, 1 << 16, 4, 3.2);
+            br->SetSeed(112);
+            auto trace = std::make_shared<TraceEvaluationContext>(BINARY, br->GetNTT(), 1 << 16, 4, 3.2);
+            trace->SetSeed(113);
+            auto lconv = std::make_shared<LWEConversionContext>(BINARY, Q, N, n, 1 << 4, 14, 3.2);
+            lconv->SetSeed(114);
+            auto rot = std::reinterpret_pointer_cast<OperatorContext<BlindRotator>>(br);
+            auto sap = std::make_shared<SAPDecompositionContext>(rot, trace, lconv, 32);
+            // the trace's noise estimate is 0, so the accumulator is never reset (sap_decomposer.cpp:101-104)
+            if (sap->GetRestartIteration() != UINT64_MAX) throw std::runtime_error("unexpected SAP restart period");
+            auto dec = sap->ConstructOperator({GenericKey("LWE", lk), GenericKey("RLWE", rk)});
+            auto lwe = g.below(n + 1, qin);
+            const uint64_t msg = g.below(1, qin)[0];
+            unsigned __int128 b = msg;
+            for (uint64_t i = 0; i < n; ++i) b += (unsigned __int128)lwe[i] * lk[i];
+            lwe[n] = (uint64_t)(b % qin);
+            std::vector<uint64_t> out((N + 1) * 8, 0);
+            dec->Decompose(out.data(), lwe.data(), 32);
+            dump("sap_lwe", lwe);
+            dump("sap_msg", std::vector<uint64_t>{msg});
+            dump("sap_out", out);
+            auto x = g.modq(2 * N);
+            std::vector<uint64_t> ht(2 * N);
+            dec->HomTrunc(ht.data(), x.data(), 32);
+            dump("sap_ht_in", x);
+            dump("sap_ht", ht);
+            auto r = g.modq(2 * N);
+            dump("sap_reset_in", r);
+            dec->ResetAccumulator(r.data(), 32);
+            dump("sap_reset", r);
+        }
+
         step("modulus switch");
         {
             auto v = g.modq(4096);

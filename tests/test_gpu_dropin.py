@@ -192,3 +192,24 @@ def test_lwe_to_rgsw(run, oracle):
     sq = oracle.rlwe_ksk(N, s2, s, 16, 4, 3.2, 111)
     ref = oracle.lwe_to_rgsw(N, run("rgsw_lwe"), brk, n, 16, 4, oracle.BINARY, tkeys, 16, 4, sq, 16, 4, 4, 10)
     assert np.array_equal(run("rgsw_out"), ref)
+
+
+def test_sap_decomposer(run, oracle):
+    n = 16
+    lk = oracle.secret(n, 0, 112, 1)
+    s = oracle.secret(N, 0, 112, 2)
+    brk = oracle.brk(N, lk, s, n, 16, 4, oracle.BINARY, 0, 3.2, 112)
+    keys = oracle.auto_keys(N, s, 16, 4, 3.2, 113)
+    ksk = oracle.lwe_ksk(s, lk, N, n, 4, 14, 3.2, 114)
+    beta = int(np.count_nonzero(lk))
+    lwe, msg = run("sap_lwe"), int(run("sap_msg")[0])
+    want = oracle.sap_decompose(N, lwe, 32, beta, brk, n, 16, 4, oracle.BINARY, keys, 16, 4)
+    out = run("sap_out").reshape(8, N + 1)
+    assert want.shape[0] == 8
+    assert np.array_equal(out, want)
+    for i in range(8):
+        ph = oracle.phase_lwe(out[i], s, N)
+        assert int(oracle.modswitch(np.array([ph], dtype=np.uint64), Q, 2 * N, 0)[0]) == (msg >> (5 * i)) % 32, i
+    assert np.array_equal(run("sap_ht"), oracle.sap_homtrunc(N, run("sap_ht_in"), 32, keys, 16, 4))
+    assert np.array_equal(run("sap_reset"),
+                          oracle.sap_reset(N, run("sap_reset_in"), 32, ksk, n, 4, 14, Q, brk, n, 16, 4, oracle.BINARY))

@@ -420,3 +420,75 @@ def test_lwe_to_rgsw_noise_free(oracle, basebits, digits):
                 err = (int(ph[j]) - want[j]) % Q
                 err = err if err < Q // 2 else err - Q
                 assert abs(err) <= 1, (m, i, j, err)
+
+
+def _sap_keys(oracle, N, n, seed, sigma=0.0):
+    lk = oracle.secret(n, 0, seed, 1)
+    s = oracle.secret(N, 0, seed, 2)
+    brk = oracle.brk(N, lk, s, n, 16, 4, oracle.BINARY, 0, sigma, seed)
+    keys = oracle.auto_keys(N, s, 16, 4, sigma, seed)
+    return lk, s, brk, keys
+
+
+def test_sap_homtrunc_noise_free(oracle):
+    """HomTrunc of a trivial RLWE with phase X^e keeps X^(e / radix) (sap_decomposer_tests.cpp:62-126)."""
+    N, radix = 1024, 32
+    _, s, _, keys = _sap_keys(oracle, N, 4, 41)
+    R = oracle.Ring(N)
+    for e in (0, 5, 31, 32, 100, 517, 1023):
+        x = np.zeros(2 * N, dtype=np.uint64)
+        x[N + e] = 1
+        x[N:] = R.ntt(x[N:])
+        out = oracle.sap_homtrunc(N, x, radix, keys, 16, 4)
+        ph = oracle.phase_rlwe(N, out, s)
+        want = np.zeros(N, dtype=np.uint64)
+        want[e // radix] = 1
+        assert np.array_equal(ph, want), e
+
+
+def test_sap_reset_noise_free(oracle):
+    """ResetAccumulator (sap_decomposer.cpp:314-373) of the trivial RLWE Q/2N X^e with noise-free keys.
+
+    The extraction poly leaves coefficient 0 = e' Q/2N with e' = e - e mod radix; key switching a
+    trivial sample keeps it trivial; FLOOR to Z_2N gives floor(2N e' floor(Q/2N) / Q), which is
+    e' - 1 for e' > 0 (exact rationals and the x87 quotient agree here); the noise-free blind
+    rotation of Q/2N X^0 then returns exactly Q/2N X^(that).  (The reference's TestResetAccumulator,
+    sap_decomposer_tests.cpp:128-165, expects the block of e itself; on this noise-free trivial
+    input its own code lands one coefficient lower, in the block below, as here.)"""
+    N, n, radix = 1024, 10, 32
+    lk, s, brk, _ = _sap_keys(oracle, N, n, 42)
+    ksk = oracle.lwe_ksk(s, lk, N, n, 16, 4, 0.0, 42)
+    R = oracle.Ring(N)
+    step = Q // (2 * N)
+    for e in (0, 5, 37, 300, 777, 1023):
+        x = np.zeros(2 * N, dtype=np.uint64)
+        x[N + e] = step
+        x[N:] = R.ntt(x[N:])
+        acc = oracle.sap_reset(N, x, radix, ksk, n, 16, 4, Q, brk, n, 16, 4, oracle.BINARY)
+        ep = e - e % radix
+        idx = (2 * N * ep * step) // Q
+        assert idx == (ep - 1 if ep else 0)
+        want = np.zeros(N, dtype=np.uint64)
+        want[idx] = step
+        assert np.array_equal(oracle.phase_rlwe(N, acc, s), want), e
+
+
+def test_sap_decompose_noise_free(oracle):
+    """Digit i of the output decrypts to digit i (radix 32) of the message of an LWE modulo 32^8
+    (sap_decomposer_tests.cpp:167-226)."""
+    N, n, radix = 1024, 10, 32
+    lk, s, brk, keys = _sap_keys(oracle, N, n, 43)
+    beta = int(np.count_nonzero(lk))
+    qin = radix ** 8
+    rng = np.random.default_rng(43)
+    for _ in range(2):
+        msg = int(rng.integers(0, qin))
+        a = [int(v) for v in rng.integers(0, qin, size=n)]
+        b = (sum(x * int(y) for x, y in zip(a, lk)) + msg) % qin
+        lwe = np.array(a + [b], dtype=np.uint64)
+        out = oracle.sap_decompose(N, lwe, radix, beta, brk, n, 16, 4, oracle.BINARY, keys, 16, 4)
+        assert out.shape[0] >= 8
+        for i in range(8):
+            ph = oracle.phase_lwe(out[i], s, N)
+            got = int(oracle.modswitch(np.array([ph], dtype=np.uint64), Q, 2 * N, 0)[0])
+            assert got == (msg >> (5 * i)) % radix, (i, got)
