@@ -163,8 +163,9 @@ def timed(ctx, lib, chk, fn, reps):
 
 
 def extra_workloads(tp, reps=2):
-    """The other bootstrapping workloads: uneven ternary batches (n=800, N=2048) in one launch with a
-    random 3-bit LUT per batch, and BMMP (n=1024, N=2048) followed by packing all outputs."""
+    """The other workloads: uneven ternary batches (n=800, N=2048) in one launch with a random 3-bit
+    LUT per batch, BMMP (n=1024, N=2048) followed by packing all outputs, and SAP digit
+    decomposition (N=4096, radix 32) over a batch."""
     from fhetoothpaste_amd import _lib
     from fhetoothpaste_amd.operators import DeviceBuffer
     from fhetoothpaste_amd.workload import lognormal_counts, random_3bit_lut
@@ -225,6 +226,33 @@ def extra_workloads(tp, reps=2):
         "bootstraps_per_s": round(B4 / (ms_br + ms_pack) * 1000, 1), "br_waves_per_simd": tune["waves"]}
     log("BMMP batch %.1f ms, packing %.2f ms" % (ms_br, ms_pack))
     for b in (d_lwe, d_tv, d_ext, d_perm, d_pack):
+        b.free()
+    ctx.close()
+
+    # SAP digit decomposition at the reference's test parameters (N=4096, n=10, radix 32, inputs
+    # mod 32^8 -> 8 digits), a batch of inputs per call
+    N5, n5, B5, radix = 4096, 10, 1024, 32
+    seed = SEED + 3
+    lk = tp.keygen_secret(n5, 0, seed, 1)
+    rk = tp.keygen_secret(N5, 0, seed, 2)
+    ctx = tp.Context(N5, n5, 16, 4, 0, n5, 16, 4, 0, 16, 4)
+    ctx.import_brk(tp.keygen_brk(lk, rk, N5, n5, 16, 4, 0, 0, 3.2, seed))
+    ctx.import_auto_keys(tp.keygen_auto_keys(rk, N5, 16, 4, 3.2, seed))
+    lwes = rng.integers(0, radix ** 8, size=(B5, n5 + 1), dtype=np.uint64)
+    lwes[:, 0] |= np.uint64(1 << 39)  # modulus 2^40 read off the values: 8 digits each
+    beta = int(np.count_nonzero(lk))
+    d_lwe = DeviceBuffer.from_host(ctx, lwes)
+    d_out = DeviceBuffer(ctx, B5 * 8 * (N5 + 1))
+    d_cnt = DeviceBuffer(ctx, B5 // 2)
+    ms_sap = timed(ctx, lib, chk, lambda: chk(lib.tp_sap_decompose_device(ctx.h, B5, radix, beta, (1 << 64) - 1,
+                                                                         d_lwe.p, d_out.p, 8, d_cnt.p)), reps)
+    out["sap_decomposition"] = {
+        "workload": "SAP digit decomposition, binary n=10 N=4096 b=2^16 l=4, automorphism keys b=2^16 d=4, radix 32, "
+                    "%d inputs mod 2^40 (8 digits each) per call" % B5,
+        "inputs": B5, "ms": round(ms_sap, 2), "decompositions_per_s": round(B5 / ms_sap * 1000, 1),
+        "digits_per_s": round(8 * B5 / ms_sap * 1000, 1)}
+    log("SAP decomposition: %d inputs, %.1f ms" % (B5, ms_sap))
+    for b in (d_lwe, d_out, d_cnt):
         b.free()
     ctx.close()
     return out
