@@ -179,10 +179,10 @@ br_kernel(BrArgs A) {
 #pragma unroll
                         for (int c = 0; c < kE / 2; ++c) {
                             const U64x2 va = ka[c * S::P + L], vb = kb[c * S::P + L];
-                            out[0][2 * c] = gl::add_ll(out[0][2 * c], gl::mul_lazy(d[2 * c], va.x));
-                            out[0][2 * c + 1] = gl::add_ll(out[0][2 * c + 1], gl::mul_lazy(d[2 * c + 1], va.y));
-                            out[1][2 * c] = gl::add_ll(out[1][2 * c], gl::mul_lazy(d[2 * c], vb.x));
-                            out[1][2 * c + 1] = gl::add_ll(out[1][2 * c + 1], gl::mul_lazy(d[2 * c + 1], vb.y));
+                            out[0][2 * c] = gl::mac_lazy(d[2 * c], va.x, out[0][2 * c]);
+                            out[0][2 * c + 1] = gl::mac_lazy(d[2 * c + 1], va.y, out[0][2 * c + 1]);
+                            out[1][2 * c] = gl::mac_lazy(d[2 * c], vb.x, out[1][2 * c]);
+                            out[1][2 * c + 1] = gl::mac_lazy(d[2 * c + 1], vb.y, out[1][2 * c + 1]);
                         }
                     } else {
                         // combined key sum_c w_c RGSW_c[row], formed pair by pair next to its use
@@ -199,20 +199,20 @@ br_kernel(BrArgs A) {
                                 const U64x2* k0 = reinterpret_cast<const U64x2*>(krow + p * N);
                                 const U64x2* k1 = reinterpret_cast<const U64x2*>(krow + rgsw_words + p * N);
                                 const U64x2 a0 = k0[c * S::P + L], a1 = k1[c * S::P + L];
-                                uint64_t cx = gl::add_ll(gl::mul_lazy(a0.x, w0.x), gl::mul_lazy(a1.x, w1.x));
-                                uint64_t cy = gl::add_ll(gl::mul_lazy(a0.y, w0.y), gl::mul_lazy(a1.y, w1.y));
+                                uint64_t cx = gl::mac_lazy(a1.x, w1.x, gl::mul_lazy(a0.x, w0.x));
+                                uint64_t cy = gl::mac_lazy(a1.y, w1.y, gl::mul_lazy(a0.y, w0.y));
                                 if constexpr (NC == 3) {
                                     const U64x2* k2 = reinterpret_cast<const U64x2*>(krow + 2 * rgsw_words + p * N);
                                     const U64x2 a2 = k2[c * S::P + L];
-                                    cx = gl::add_ll(cx, gl::mul_lazy(a2.x, w2.x));
-                                    cy = gl::add_ll(cy, gl::mul_lazy(a2.y, w2.y));
+                                    cx = gl::mac_lazy(a2.x, w2.x, cx);
+                                    cy = gl::mac_lazy(a2.y, w2.y, cy);
                                 }
                                 if (p == 0) {
-                                    out[0][2 * c] = gl::add_ll(out[0][2 * c], gl::mul_lazy(d[2 * c], cx));
-                                    out[0][2 * c + 1] = gl::add_ll(out[0][2 * c + 1], gl::mul_lazy(d[2 * c + 1], cy));
+                                    out[0][2 * c] = gl::mac_lazy(d[2 * c], cx, out[0][2 * c]);
+                                    out[0][2 * c + 1] = gl::mac_lazy(d[2 * c + 1], cy, out[0][2 * c + 1]);
                                 } else {
-                                    out[1][2 * c] = gl::add_ll(out[1][2 * c], gl::mul_lazy(d[2 * c], cx));
-                                    out[1][2 * c + 1] = gl::add_ll(out[1][2 * c + 1], gl::mul_lazy(d[2 * c + 1], cy));
+                                    out[1][2 * c] = gl::mac_lazy(d[2 * c], cx, out[1][2 * c]);
+                                    out[1][2 * c + 1] = gl::mac_lazy(d[2 * c + 1], cy, out[1][2 * c + 1]);
                                 }
                             }
                         });

@@ -796,7 +796,10 @@ uint32_t bit_reverse(uint32_t x, uint32_t bits) {
 //   trunc   N^-1 sum_{i < radix} X^-i                                               (:220-233)
 //   shift   [logN + 1] level l: -X^(N - N / 2^l)                                    (:280-285)
 //   leaf    [N / radix] exponent e: X^brv(e) over log2(N / radix) bits              (:266-268)
-size_t sap_poly_words(uint32_t N, uint32_t logN, uint32_t radix) { return (size_t)N * (3 + logN + N / radix); }
+//   odd     [N / (2 radix)] e: shift[1] * leaf[2e + 1], the odd leaf of a level-1 node with its shift
+size_t sap_poly_words(uint32_t N, uint32_t logN, uint32_t radix) {
+    return (size_t)N * (3 + logN + N / radix + N / radix / 2);
+}
 
 int sap_prepare(tp_context* c, uint32_t radix) {
     if (c->sap_radix == radix && c->d_sap_polys) return TP_OK;
@@ -825,6 +828,10 @@ int sap_prepare(tp_context* c, uint32_t radix) {
         p[rev_bits ? bit_reverse(e, rev_bits) : 0] = 1;
         T.forward(p);
     }
+    uint64_t* odd = leaf + (size_t)(N >> lr) * N;
+    for (uint32_t e = 0; e < (N >> lr) / 2; ++e)
+        for (uint32_t j = 0; j < N; ++j)
+            odd[(size_t)e * N + j] = gl::mul(shift[N + j], leaf[(size_t)(2 * e + 1) * N + j]);
     if (c->d_sap_polys) TP_HIP(hipFree(c->d_sap_polys));
     c->d_sap_polys = nullptr;
     c->sap_radix = 0;
@@ -890,6 +897,7 @@ int sap_homtrunc(tp_context* c, const SapWork& w, size_t count, uint32_t radix, 
     const uint64_t* trunc = c->d_sap_polys + N;
     const uint64_t* shift = trunc + N;
     const uint64_t* leaf = shift + (size_t)(logN + 1) * N;
+    const uint64_t* odd = leaf + (size_t)(N >> lr) * N;
     const size_t key_words = (size_t)c->p.rlwe_ks_digits * 2 * N;
     const tp::KeySwitchArgs A = auto_args(c);
     auto slot = [&](uint32_t h) { return w.slots + (size_t)h * rl; };
@@ -924,6 +932,14 @@ int sap_homtrunc(tp_context* c, const SapWork& w, size_t count, uint32_t radix, 
                   tp::launch_rlwe_keyswitch(c->logN, A, slot(head + 1), slot(head + 2), (int)count, (1u << level) + 1,
                                             TP_RKS_AUTOMORPHISM, 0, c->d_auto + (size_t)(level - 1) * key_words,
                                             c->stream));
+        if (level == 1) {
+            // both children are leaves: (x + y) X^brv(2e) + (x - y) X^(-N/2) X^brv(2e+1) in one pass
+            TP_LAUNCH("SAP tree node and leaves",
+                      tp::launch_sap_node_leaves(out, slot(head), slot(head + 2), leaf + (size_t)(2 * e) * N,
+                                                 odd + (size_t)e * N, rows, N, first_leaf ? 0 : 1, c->stream));
+            first_leaf = false;
+            continue;
+        }
         TP_LAUNCH("SAP tree node", tp::launch_sap_node(slot(head), slot(head + 2), slot(head + 1),
                                                        shift + (size_t)level * N, rows, N, c->stream));
         lvl[head] = level - 1;

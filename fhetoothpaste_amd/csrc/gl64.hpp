@@ -101,6 +101,14 @@ GL_FN uint64_t mul_lazy(uint64_t a, uint64_t b) {
     return reduce128_lazy(hi, lo);
 }
 
+// Lazy a * b + c for any 64-bit a, b, c: the 128-bit sum is below 2^128 ((2^64 - 1)^2 + 2^64 - 1),
+// so it is reduced once.  The 128-bit form lets the compiler chain the products and the addend
+// (fewer instructions than a product reduction followed by a lazy add).
+GL_FN uint64_t mac_lazy(uint64_t a, uint64_t b, uint64_t c) {
+    const unsigned __int128 p = (unsigned __int128)a * b + c;
+    return reduce128_lazy((uint64_t)(p >> 64), (uint64_t)p);
+}
+
 // a lazy, b < Q: one carry correction suffices (a + b < 2^64 + Q).
 GL_FN uint64_t add_lc(uint64_t a, uint64_t b) {
     uint64_t s;

@@ -82,7 +82,7 @@ hipError_t launch_blind_rotate_w1(int logN, int method, int digits, const BrArgs
 hipError_t launch_lwe_keyswitch(const KsArgs& A, hipStream_t s);
 // SAP digit decomposition (kernels_sap.hip): per-input modulus and digit count; one digit step
 // of the bookkeeping; products of rows of N words with one NTT-domain polynomial; the HomTrunc
-// tree's inner-node and leaf updates; sample extraction into digit `digit` of each row's output.
+// tree's inner-node and leaf updates (and a level-1 node fused with its two leaves); sample extraction into digit `digit` of each row's output.
 hipError_t launch_sap_init(const uint64_t* lwes, uint64_t* cm, uint32_t* digits, int n, int count, int rl,
                            hipStream_t s);
 hipError_t launch_sap_digit(uint64_t* cp, uint64_t* cm, uint64_t* sub, int n, int count, int rl, uint64_t mask,
@@ -93,6 +93,8 @@ hipError_t launch_sap_node(uint64_t* x, const uint64_t* y, uint64_t* odd, const 
                            uint32_t N, hipStream_t s);
 hipError_t launch_sap_leaf(uint64_t* out, const uint64_t* x, const uint64_t* mono, size_t rows, uint32_t N,
                            int accumulate, hipStream_t s);
+hipError_t launch_sap_node_leaves(uint64_t* out, const uint64_t* x, const uint64_t* y, const uint64_t* m_even,
+                                  const uint64_t* m_odd, size_t rows, uint32_t N, int accumulate, hipStream_t s);
 hipError_t launch_sap_extract(uint64_t* out, const uint64_t* rlwe, const uint32_t* digits, uint32_t digit,
                               size_t out_stride, int count, uint32_t N, hipStream_t s);
 constexpr int kKsPadWords = 16;  // zero words after the device key-switching key

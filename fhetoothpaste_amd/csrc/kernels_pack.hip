@@ -78,10 +78,10 @@ __device__ __forceinline__ void rlwe_ks_core(uint64_t (&out_a)[kE], uint64_t (&o
 #pragma unroll
         for (int c = 0; c < kE / 2; ++c) {
             const V2 va = ka[c * S::P + L], vb = kb[c * S::P + L];
-            out_a[2 * c] = gl::add_ll(out_a[2 * c], gl::mul_lazy(d[2 * c], va.x));
-            out_a[2 * c + 1] = gl::add_ll(out_a[2 * c + 1], gl::mul_lazy(d[2 * c + 1], va.y));
-            out_b[2 * c] = gl::add_ll(out_b[2 * c], gl::mul_lazy(d[2 * c], vb.x));
-            out_b[2 * c + 1] = gl::add_ll(out_b[2 * c + 1], gl::mul_lazy(d[2 * c + 1], vb.y));
+            out_a[2 * c] = gl::mac_lazy(d[2 * c], va.x, out_a[2 * c]);
+            out_a[2 * c + 1] = gl::mac_lazy(d[2 * c + 1], va.y, out_a[2 * c + 1]);
+            out_b[2 * c] = gl::mac_lazy(d[2 * c], vb.x, out_b[2 * c]);
+            out_b[2 * c + 1] = gl::mac_lazy(d[2 * c + 1], vb.y, out_b[2 * c + 1]);
         }
     }
 }
@@ -106,29 +106,35 @@ __device__ __forceinline__ void automorphism_eval(uint64_t (&a)[kE], uint64_t (&
 
 }  // namespace
 
-// Batched automorphism: in [count][2N] coefficients, out [count][2N] NTT domain.
+// Batched automorphism: in [count][2N] coefficients, out [count][2N] NTT domain.  b is read and
+// transformed after the key switch of a, so its registers are not live across the digit loop (two
+// waves per SIMD fit); a sync before the writes keeps an aliased in == out row intact.
 template <int LOGN>
-__global__ void __launch_bounds__(Shape<LOGN>::P) automorphism_kernel(KeySwitchArgs A, const uint64_t* in,
-                                                                      uint64_t* out, int count, uint32_t t,
-                                                                      const uint64_t* ksk) {
+__global__ void __launch_bounds__(Shape<LOGN>::P) __attribute__((amdgpu_waves_per_eu(2)))
+automorphism_kernel(KeySwitchArgs A, const uint64_t* in, uint64_t* out, int count, uint32_t t, const uint64_t* ksk) {
     using S = Shape<LOGN>;
     constexpr int N = S::N;
     __shared__ uint64_t lds[S::LDS_WORDS];
     const int L = threadIdx.x;
     for (int ct = blockIdx.x; ct < count; ct += gridDim.x) {
-        uint64_t a[kE], b[kE];
         const uint64_t* src = in + (size_t)ct * 2 * N;
+        uint64_t a[kE];
 #pragma unroll
-        for (int k = 0; k < kE; ++k) {
-            a[k] = src[idx1<LOGN>(L, k)];
-            b[k] = src[N + idx1<LOGN>(L, k)];
-        }
-        automorphism_eval<LOGN>(a, b, lds, t, A, ksk, L);
+        for (int k = 0; k < kE; ++k) a[k] = src[idx1<LOGN>(L, k)];
+        automorphism_regs<LOGN>(a, lds, t, L);
+        uint64_t oa[kE], ob[kE];
+        rlwe_ks_core<LOGN>(oa, ob, a, lds, A, ksk, L);
+        uint64_t b[kE];
+#pragma unroll
+        for (int k = 0; k < kE; ++k) b[k] = src[N + idx1<LOGN>(L, k)];
+        automorphism_regs<LOGN>(b, lds, t, L);
+        ntt_forward<LOGN>(b, lds, A.tw, L);
+        poly_sync<S::WAVES>();
         uint64_t* dst = out + (size_t)ct * 2 * N;
 #pragma unroll
         for (int k = 0; k < kE; ++k) {
-            dst[idx3(L, k)] = a[k];
-            dst[N + idx3(L, k)] = b[k];
+            dst[idx3(L, k)] = gl::neg(gl::canon(oa[k]));
+            dst[N + idx3(L, k)] = gl::sub(gl::canon(b[k]), gl::canon(ob[k]));
         }
         poly_sync<S::WAVES>();
     }
@@ -141,37 +147,37 @@ __global__ void __launch_bounds__(Shape<LOGN>::P) automorphism_kernel(KeySwitchA
 //          a' = tau_t(a), b' = NTT(tau_t(b)), then as mode 0;
 //   mode 2 (AutomorphismEvaluator::EvalSwitchOnly, :171-176): a' = tau_t(a), b' = 0, then as mode 0.
 // out (NTT domain) = r, or out + r when accumulating (the reference subtracts into its output).
+// b is read after the key switch of a (registers, as in automorphism_kernel).
 template <int LOGN>
-__global__ void __launch_bounds__(Shape<LOGN>::P) rlwe_keyswitch_kernel(KeySwitchArgs A, const uint64_t* in,
-                                                                        uint64_t* out, int count, uint32_t t,
-                                                                        int mode, int accumulate,
-                                                                        const uint64_t* ksk) {
+__global__ void __launch_bounds__(Shape<LOGN>::P) __attribute__((amdgpu_waves_per_eu(2)))
+rlwe_keyswitch_kernel(KeySwitchArgs A, const uint64_t* in, uint64_t* out, int count, uint32_t t, int mode,
+                      int accumulate, const uint64_t* ksk) {
     using S = Shape<LOGN>;
     constexpr int N = S::N;
     __shared__ uint64_t lds[S::LDS_WORDS];
     const int L = threadIdx.x;
     for (int ct = blockIdx.x; ct < count; ct += gridDim.x) {
         const uint64_t* src = in + (size_t)ct * 2 * N;
-        uint64_t a[kE], b[kE];
+        uint64_t a[kE];
 #pragma unroll
         for (int k = 0; k < kE; ++k) a[k] = src[idx1<LOGN>(L, k)];
+        if (mode != 0) automorphism_regs<LOGN>(a, lds, t, L);
+        uint64_t oa[kE], ob[kE];
+        rlwe_ks_core<LOGN>(oa, ob, a, lds, A, ksk, L);
+        uint64_t b[kE];
         if (mode == 0) {
 #pragma unroll
             for (int k = 0; k < kE; ++k) b[k] = src[N + idx3(L, k)];
+        } else if (mode == 1) {
+#pragma unroll
+            for (int k = 0; k < kE; ++k) b[k] = src[N + idx1<LOGN>(L, k)];
+            automorphism_regs<LOGN>(b, lds, t, L);
+            ntt_forward<LOGN>(b, lds, A.tw, L);
         } else {
-            automorphism_regs<LOGN>(a, lds, t, L);
-            if (mode == 1) {
 #pragma unroll
-                for (int k = 0; k < kE; ++k) b[k] = src[N + idx1<LOGN>(L, k)];
-                automorphism_regs<LOGN>(b, lds, t, L);
-                ntt_forward<LOGN>(b, lds, A.tw, L);
-            } else {
-#pragma unroll
-                for (int k = 0; k < kE; ++k) b[k] = 0;
-            }
+            for (int k = 0; k < kE; ++k) b[k] = 0;
         }
-        uint64_t oa[kE], ob[kE];
-        rlwe_ks_core<LOGN>(oa, ob, a, lds, A, ksk, L);
+        poly_sync<S::WAVES>();
         uint64_t* dst = out + (size_t)ct * 2 * N;
 #pragma unroll
         for (int k = 0; k < kE; ++k) {

@@ -102,6 +102,24 @@ __global__ void __launch_bounds__(kBlock) sap_leaf_kernel(uint64_t* __restrict__
     }
 }
 
+// A level-1 node of the tree together with its two leaves: with x the node and y its automorphism
+// image, out (+)= (x + y) * X^brv(2e) + (x - y) * (-X^(N/2) X^brv(2e+1)); the second factor is one
+// precomputed NTT-domain polynomial.
+__global__ void __launch_bounds__(kBlock) sap_node_leaves_kernel(uint64_t* __restrict__ out,
+                                                                 const uint64_t* __restrict__ x,
+                                                                 const uint64_t* __restrict__ y,
+                                                                 const uint64_t* __restrict__ m_even,
+                                                                 const uint64_t* __restrict__ m_odd, size_t words,
+                                                                 uint32_t nmask, int accumulate) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += stride) {
+        const uint64_t a = x[i], b = y[i];
+        const uint32_t j = (uint32_t)(i & nmask);
+        const uint64_t p = gl::add(gl::mul(gl::add(a, b), m_even[j]), gl::mul(gl::sub(a, b), m_odd[j]));
+        out[i] = accumulate ? gl::add(out[i], p) : p;
+    }
+}
+
 // SampleExtract at index 0 (static/sample_extraction.h:20-36) of RLWE row i ([a|b] coefficients)
 // into out + i * out_stride + digit * (N + 1), for the rows with digit < digits[i] (all rows when
 // digits is null).  Every word is written: -0 = 0 where the reference skips the write.
@@ -164,6 +182,15 @@ hipError_t launch_sap_leaf(uint64_t* out, const uint64_t* x, const uint64_t* mon
     if (words == 0) return hipSuccess;
     hipLaunchKernelGGL(sap_leaf_kernel, dim3(grid_for(words)), dim3(kBlock), 0, s, out, x, mono, words, N - 1,
                        accumulate);
+    return hipGetLastError();
+}
+
+hipError_t launch_sap_node_leaves(uint64_t* out, const uint64_t* x, const uint64_t* y, const uint64_t* m_even,
+                                  const uint64_t* m_odd, size_t rows, uint32_t N, int accumulate, hipStream_t s) {
+    const size_t words = rows * N;
+    if (words == 0) return hipSuccess;
+    hipLaunchKernelGGL(sap_node_leaves_kernel, dim3(grid_for(words)), dim3(kBlock), 0, s, out, x, y, m_even, m_odd,
+                       words, N - 1, accumulate);
     return hipGetLastError();
 }
 
