@@ -903,12 +903,12 @@ int sap_homtrunc(tp_context* c, const SapWork& w, size_t count, uint32_t radix, 
     auto slot = [&](uint32_t h) { return w.slots + (size_t)h * rl; };
     TP_LAUNCH("SAP truncation product", tp::launch_sap_poly_mul(slot(0), in, trunc, rows, N, c->stream));
     // partial trace over t = N + 1, N/2 + 1, ..., N/2^(lr-1) + 1 (:236-250)
+    // (each automorphism takes its input in the NTT domain: the reference's BackwardNTT before EvalAuto
+    // happens in registers for a, and b is permuted in the evaluation domain)
     for (uint32_t i = 0; i < lr; ++i) {
         const uint32_t j = logN - i;
-        TP_LAUNCH("SAP inverse NTT", tp::launch_ntt_batch(c->logN, slot(1), slot(0), rows, c->d_itw, true, c->stream));
-        TP_LAUNCH("SAP automorphism", tp::launch_rlwe_keyswitch(c->logN, A, slot(1), slot(0), (int)count, (1u << j) + 1,
-                                                                TP_RKS_AUTOMORPHISM, 1,
-                                                                c->d_auto + (size_t)(j - 1) * key_words, c->stream));
+        TP_LAUNCH("SAP automorphism", tp::launch_auto_ntt(c->logN, A, slot(0), slot(0), (int)count, (1u << j) + 1, 1,
+                                                          c->d_auto + (size_t)(j - 1) * key_words, c->stream));
     }
     // the clean-up tree (:266-311), in the reference's stack order and slot assignment
     uint32_t lvl[64];
@@ -926,12 +926,9 @@ int sap_homtrunc(tp_context* c, const SapWork& w, size_t count, uint32_t radix, 
             first_leaf = false;
             continue;
         }
-        TP_LAUNCH("SAP inverse NTT",
-                  tp::launch_ntt_batch(c->logN, slot(head + 1), slot(head), rows, c->d_itw, true, c->stream));
         TP_LAUNCH("SAP automorphism",
-                  tp::launch_rlwe_keyswitch(c->logN, A, slot(head + 1), slot(head + 2), (int)count, (1u << level) + 1,
-                                            TP_RKS_AUTOMORPHISM, 0, c->d_auto + (size_t)(level - 1) * key_words,
-                                            c->stream));
+                  tp::launch_auto_ntt(c->logN, A, slot(head), slot(head + 2), (int)count, (1u << level) + 1, 0,
+                                      c->d_auto + (size_t)(level - 1) * key_words, c->stream));
         if (level == 1) {
             // both children are leaves: (x + y) X^brv(2e) + (x - y) X^(-N/2) X^brv(2e+1) in one pass
             TP_LAUNCH("SAP tree node and leaves",

@@ -66,6 +66,10 @@ hipError_t launch_pack(int logN, const KeySwitchArgs& A, const uint64_t* lwes, c
 // mode 0 convert ([a coef | b NTT]), 1 automorphism ([a|b] coef), 2 automorphism of a only.
 hipError_t launch_rlwe_keyswitch(int logN, const KeySwitchArgs& A, const uint64_t* in, uint64_t* out, int count,
                                 uint32_t t, int mode, int accumulate, const uint64_t* ksk, hipStream_t s);
+// Automorphism key switch of NTT-domain [a|b] rows (a inverse-transformed in registers, b permuted
+// in the evaluation domain); out (+)= the mode-1 result; in may equal out.
+hipError_t launch_auto_ntt(int logN, const KeySwitchArgs& A, const uint64_t* in, uint64_t* out, int count,
+                           uint32_t t, int accumulate, const uint64_t* ksk, hipStream_t s);
 // Modulus switch to 2^t (kind: 0 round, 1 floor, 2 ceil, 3 random with a seeded coin).
 hipError_t launch_modswitch(uint64_t* dst, const uint64_t* src, size_t words, uint64_t sM, int t, int kind,
                             uint64_t seed, hipStream_t s);

@@ -194,6 +194,56 @@ rlwe_keyswitch_kernel(KeySwitchArgs A, const uint64_t* in, uint64_t* out, int co
     }
 }
 
+// Automorphism key switch of an NTT-domain RLWE [a|b] (reference order), as the SAP tree applies it
+// to its nodes (sap_decomposer.cpp:240-248, :287-291: BackwardNTT then EvalAuto).  a is brought to
+// coefficients in registers; b stays in the evaluation domain, where tau_t is a permutation:
+// NTT(b(X^t))[p] = NTT(b)[q] with 2 brv(q) + 1 = t (2 brv(p) + 1) mod 2N, so it is gathered, not
+// transformed.  out (+)= (-KS(a).a, b' - KS(a).b), equal to rlwe_keyswitch_kernel mode 1 on the
+// inverse transform of in.  All reads of a row precede the sync before its writes, so in may equal out.
+template <int LOGN>
+__global__ void __launch_bounds__(Shape<LOGN>::P) __attribute__((amdgpu_waves_per_eu(2)))
+auto_ntt_kernel(KeySwitchArgs A, const uint64_t* in, uint64_t* out, int count, uint32_t t, int accumulate,
+                const uint64_t* ksk) {
+    using S = Shape<LOGN>;
+    constexpr int N = S::N;
+    __shared__ uint64_t lds[S::LDS_WORDS];
+    const int L = threadIdx.x;
+    for (int ct = blockIdx.x; ct < count; ct += gridDim.x) {
+        const uint64_t* src = in + (size_t)ct * 2 * N;
+        uint64_t a[kE];
+#pragma unroll
+        for (int k = 0; k < kE; ++k) a[k] = src[idx3(L, k)];
+        ntt_inverse<LOGN>(a, lds, A.itw, L);
+#pragma unroll
+        for (int k = 0; k < kE; ++k) a[k] = gl::canon(a[k]);
+        automorphism_regs<LOGN>(a, lds, t, L);
+        uint64_t oa[kE], ob[kE];
+        rlwe_ks_core<LOGN>(oa, ob, a, lds, A, ksk, L);
+        uint64_t b[kE];
+#pragma unroll
+        for (int k = 0; k < kE; ++k) {
+            const uint32_t p = (uint32_t)idx3(L, k);
+            const uint32_t w = 2 * (__builtin_bitreverse32(p) >> (32 - LOGN)) + 1;
+            const uint32_t e = (t * w) & (2 * N - 1);
+            b[k] = src[N + (__builtin_bitreverse32((e - 1) >> 1) >> (32 - LOGN))];
+        }
+        poly_sync<S::WAVES>();
+        uint64_t* dst = out + (size_t)ct * 2 * N;
+#pragma unroll
+        for (int k = 0; k < kE; ++k) {
+            uint64_t ra = gl::neg(gl::canon(oa[k]));
+            uint64_t rb = gl::sub(gl::canon(b[k]), gl::canon(ob[k]));
+            if (accumulate) {
+                ra = gl::add(dst[idx3(L, k)], ra);
+                rb = gl::add(dst[N + idx3(L, k)], rb);
+            }
+            dst[idx3(L, k)] = ra;
+            dst[N + idx3(L, k)] = rb;
+        }
+        poly_sync<S::WAVES>();
+    }
+}
+
 // Batched trace (trace_evaluation.cpp:135-162): o = in / N; for t = N+1, N/2+1, ..., 3:
 // o += INTT(Auto_t(o)); out = NTT(o).  in coefficients, out NTT domain.  keys: log2 N key blocks,
 // block j-1 for t = 2^j + 1.
@@ -381,6 +431,13 @@ hipError_t launch_rlwe_keyswitch(int logN, const KeySwitchArgs& A, const uint64_
                                 uint32_t t, int mode, int accumulate, const uint64_t* ksk, hipStream_t s) {
     TP_LOGN_SWITCH(logN, hipLaunchKernelGGL(rlwe_keyswitch_kernel<LG>, dim3(count), dim3(Shape<LG>::P), 0, s, A, in,
                                             out, count, t, mode, accumulate, ksk));
+    return hipGetLastError();
+}
+
+hipError_t launch_auto_ntt(int logN, const KeySwitchArgs& A, const uint64_t* in, uint64_t* out, int count,
+                           uint32_t t, int accumulate, const uint64_t* ksk, hipStream_t s) {
+    TP_LOGN_SWITCH(logN, hipLaunchKernelGGL(auto_ntt_kernel<LG>, dim3(count), dim3(Shape<LG>::P), 0, s, A, in, out,
+                                            count, t, accumulate, ksk));
     return hipGetLastError();
 }
 
