@@ -160,6 +160,14 @@ br_kernel(BrArgs A) {
                         rotate_from<LOGN>(d, src, e0, L, true);
 #pragma unroll
                         for (int k = 0; k < kE; ++k) d[k] = (gl::sub(d[k], acc[0][k]) >> shift) & mask;
+                        // the first transform level on the digits themselves: cheap while they are small
+                        if (A.base_log2 <= 16) {
+#pragma unroll
+                            for (int k = 0; k < kE / 2; ++k) ct_bfly_small_first(d[k], d[k + kE / 2]);
+                        } else {
+#pragma unroll
+                            for (int k = 0; k < kE / 2; ++k) ct_bfly_pow2(d[k], d[k + kE / 2], gl::kFwdPow2[1]);
+                        }
                     } else {
                         // signed digits of x + C mod Q (gadget_decomp.h:169-198)
                         const bool last = (j == A.max_digits - 1);
@@ -170,7 +178,7 @@ br_kernel(BrArgs A) {
                         for (int k = 0; k < kE; ++k)
                             d[k] = gl::sub((gl::add(acc[0][k], A.corrector) >> shift) & dm, delta);
                     }
-                    ntt_forward<LOGN>(d, buf, A.tw, L);
+                    ntt_forward<LOGN, METHOD == 0 ? 1 : 0>(d, buf, A.tw, L);
                     const size_t row = (size_t)(h * LD + j) * 2 * N;
                     if constexpr (METHOD == 0) {
                         // key pairs are read next to their use, not hoisted over the transform

@@ -105,6 +105,16 @@ __device__ __forceinline__ void ct_bfly_pow2(uint64_t& u, uint64_t& v, int e) {
     ct_bfly_t(u, v, t, ng);
 }
 
+// First forward level on inputs below 2^16 (unsigned gadget digits): its twiddle is 2^48
+// (gl::kFwdPow2[1]), so u + v 2^48 < 2^64 needs no reduction and u - v 2^48 is represented
+// exactly by u + Q - v 2^48, which lies in (0, 2^64).
+__device__ __forceinline__ void ct_bfly_small_first(uint64_t& u, uint64_t& v) {
+    const uint64_t t = v << 48;
+    const uint64_t a = u;
+    u = a + t;
+    v = (a + gl::kQ) - t;
+}
+
 __device__ __forceinline__ void ct_bfly_mul(uint64_t& u, uint64_t& v, uint64_t w) {
     ct_bfly_t(u, v, gl::mul_lazy(v, w), false);
 }
@@ -146,10 +156,10 @@ __device__ __forceinline__ uint64_t ld_u64(const uint64_t* base, uint32_t idx) {
 
 // ---------------------------------------------------------------- forward
 
-template <int LOGN>
+template <int LOGN, int FIRST_LV = 0>
 __device__ __forceinline__ void fwd_pass1(uint64_t (&x)[kE]) {
 #pragma unroll
-    for (int lv = 0; lv < 4; ++lv) {
+    for (int lv = FIRST_LV; lv < 4; ++lv) {
         const int half = 8 >> lv;
 #pragma unroll
         for (int k = 0; k < kE; ++k) {
@@ -192,11 +202,11 @@ __device__ __forceinline__ void fwd_pass3(uint64_t (&x)[kE], const uint64_t* __r
 }
 
 // Natural-order coefficients (pass-1 layout, lazy or canonical) -> lazy NTT values in
-// pass-3 layout.
-template <int LOGN>
+// pass-3 layout.  FIRST_LV = 1 starts after the first level (a caller that has done it).
+template <int LOGN, int FIRST_LV = 0>
 __device__ __forceinline__ void ntt_forward(uint64_t (&x)[kE], uint64_t* lds, const uint64_t* tw, int L) {
     tw = opaque(tw);
-    fwd_pass1<LOGN>(x);
+    fwd_pass1<LOGN, FIRST_LV>(x);
     transpose<LOGN, Layout::kPass1, Layout::kPass2>(x, lds, L);
     fwd_pass2<LOGN>(x, opaque(tw), L);
     transpose<LOGN, Layout::kPass2, Layout::kPass3>(x, lds, L);

@@ -161,3 +161,33 @@ def test_sap_rejects(tp, binary):
     bare = tp.Context(N, binary["n"], BB, BD)
     with pytest.raises(ToothpasteError):
         bare.sap_homtrunc(np.zeros((1, 2 * N), np.uint64), 32)  # no automorphism keys
+
+
+def test_sap_decompose_reference_parameters(tp, oracle):
+    """The reference test's own setting (sap_decomposer_tests.cpp:25-42, 167-237): N=4096, n=10,
+    binary keys, gadget 2^16 x 4, radix 32, inputs mod 32^8; bit-exact against the oracle and every
+    digit decrypts."""
+    say("sap decompose N=4096")
+    n, NN = 10, 4096
+    lk = oracle.secret(n, 0, 401, 1)
+    s = oracle.secret(NN, 0, 401, 2)
+    brk = oracle.brk(NN, lk, s, n, BB, BD, oracle.BINARY, 0, 3.2, 401)
+    keys = oracle.auto_keys(NN, s, AB, AD, 3.2, 402)
+    ctx = tp.Context(NN, n, BB, BD, oracle.BINARY, 0, 0, 0, 0, AB, AD)
+    ctx.import_brk(brk)
+    ctx.import_auto_keys(keys)
+    rng = np.random.default_rng(15)
+    qin = 32 ** 8
+    msgs = [int(rng.integers(0, qin)) for _ in range(2)]
+    lwes = np.stack([_lwe(lk, m, qin, rng) for m in msgs])
+    beta = int(np.count_nonzero(lk))
+    got, counts = ctx.sap_decompose(lwes, 32, beta)
+    for c, m in enumerate(msgs):
+        want = oracle.sap_decompose(NN, lwes[c], 32, beta, brk, n, BB, BD, oracle.BINARY, keys, AB, AD)
+        assert counts[c] == want.shape[0] == 8
+        assert np.array_equal(got[c], want), c
+        for i in range(8):
+            ph = oracle.phase_lwe(got[c, i], s, NN)
+            dig = int(oracle.modswitch(np.array([ph], dtype=np.uint64), Q, 2 * NN, 0)[0])
+            assert dig == (m >> (5 * i)) % 32, (c, i)
+    ctx.close()
