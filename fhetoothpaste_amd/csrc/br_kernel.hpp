@@ -175,10 +175,21 @@ br_kernel(BrArgs A) {
                         const uint64_t dm = last ? (2 * A.least - 1) : mask;
                         const uint64_t delta = last ? A.least : (1ULL << (A.base_log2 - 1));
 #pragma unroll
-                        for (int k = 0; k < kE; ++k)
-                            d[k] = gl::sub((gl::add(acc[0][k], A.corrector) >> shift) & dm, delta);
+                        for (int k = 0; k < kE; ++k) d[k] = (gl::add(acc[0][k], A.corrector) >> shift) & dm;
+                        // the first transform level folds the offset in while the digits are small
+                        if (dm < (1ULL << 16)) {
+                            uint64_t kp, km;
+                            signed_first_consts(delta, kp, km);
+#pragma unroll
+                            for (int k = 0; k < kE / 2; ++k) ct_bfly_signed_first(d[k], d[k + kE / 2], kp, km);
+                        } else {
+#pragma unroll
+                            for (int k = 0; k < kE; ++k) d[k] = gl::sub(d[k], delta);
+#pragma unroll
+                            for (int k = 0; k < kE / 2; ++k) ct_bfly_pow2(d[k], d[k + kE / 2], gl::kFwdPow2[1]);
+                        }
                     }
-                    ntt_forward<LOGN, METHOD == 0 ? 1 : 0>(d, buf, A.tw, L);
+                    ntt_forward<LOGN, 1>(d, buf, A.tw, L);
                     const size_t row = (size_t)(h * LD + j) * 2 * N;
                     if constexpr (METHOD == 0) {
                         // key pairs are read next to their use, not hoisted over the transform

@@ -71,8 +71,20 @@ __device__ __forceinline__ void rlwe_ks_core(uint64_t (&out_a)[kE], uint64_t (&o
         const uint64_t delta = last ? A.least : (1ULL << (A.base_log2 - 1));
         uint64_t d[kE];
 #pragma unroll
-        for (int k = 0; k < kE; ++k) d[k] = gl::sub((gl::add(a[k], A.corrector) >> shift) & dm, delta);
-        ntt_forward<LOGN>(d, lds, A.tw, L);
+        for (int k = 0; k < kE; ++k) d[k] = (gl::add(a[k], A.corrector) >> shift) & dm;
+        // the first transform level folds the offset in while the digits are small
+        if (dm < (1ULL << 16)) {
+            uint64_t kp, km;
+            signed_first_consts(delta, kp, km);
+#pragma unroll
+            for (int k = 0; k < kE / 2; ++k) ct_bfly_signed_first(d[k], d[k + kE / 2], kp, km);
+        } else {
+#pragma unroll
+            for (int k = 0; k < kE; ++k) d[k] = gl::sub(d[k], delta);
+#pragma unroll
+            for (int k = 0; k < kE / 2; ++k) ct_bfly_pow2(d[k], d[k + kE / 2], gl::kFwdPow2[1]);
+        }
+        ntt_forward<LOGN, 1>(d, lds, A.tw, L);
         const V2* ka = reinterpret_cast<const V2*>(opaque(ksk + (size_t)j * 2 * N));
         const V2* kb = reinterpret_cast<const V2*>(opaque(ksk + (size_t)j * 2 * N + N));
 #pragma unroll

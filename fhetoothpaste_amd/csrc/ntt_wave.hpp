@@ -115,6 +115,23 @@ __device__ __forceinline__ void ct_bfly_small_first(uint64_t& u, uint64_t& v) {
     v = (a + gl::kQ) - t;
 }
 
+// First forward level on signed gadget digits held as u - delta with 0 <= u < 2^16: the pair
+// (u_a - delta) +- (u_b - delta) 2^48 is formed from the small u's (as above) plus two constants of
+// the digit, kp = -delta (1 + 2^48) and km = delta (2^48 - 1) mod Q (both canonical).
+__device__ __forceinline__ void ct_bfly_signed_first(uint64_t& u, uint64_t& v, uint64_t kp, uint64_t km) {
+    const uint64_t t = v << 48;
+    const uint64_t a = u;
+    u = gl::add_lc(a + t, kp);
+    v = gl::add_lc((a + gl::kQ) - t, km);
+}
+
+// The two constants of ct_bfly_signed_first for a digit offset delta < Q.
+__device__ __forceinline__ void signed_first_consts(uint64_t delta, uint64_t& kp, uint64_t& km) {
+    const uint64_t d48 = gl::mul_pow2(delta, 48);
+    kp = gl::neg(gl::add(delta, d48));
+    km = gl::sub(d48, delta);
+}
+
 __device__ __forceinline__ void ct_bfly_mul(uint64_t& u, uint64_t& v, uint64_t w) {
     ct_bfly_t(u, v, gl::mul_lazy(v, w), false);
 }
