@@ -214,11 +214,35 @@ class Context:
         check(lib().tp_modswitch_device(self.h, d.p, d.p, v.size, source_modulus, target_modulus, kind, seed))
         return d.to_host().reshape(v.shape)
 
-    def sap_decompose(self, lwes, radix, beta, restart=None, max_digits=64):
+    @staticmethod
+    def sap_digit_counts(lwes, radix):
+        """Digits per input as SAPDecomposer::Decompose reads the modulus off the values
+        (sap_decomposer.cpp:427-433, with the x86 shift: 1 << 64 is 1); sizes the output."""
+        x = np.ascontiguousarray(lwes, dtype=np.uint64)
+        x = x.reshape(-1, x.shape[-1])
+        a0 = [int(v) for v in x[:, 0]]
+        m = np.array([1 << (((v - 1).bit_length() if v > 1 else 0) & 63) for v in a0], dtype=object)
+        for j in range(1, x.shape[1]):
+            col = x[:, j].astype(object)
+            m = np.where(m <= col, m * 2, m)
+        rl = (radix - 1).bit_length()
+        counts = []
+        for v in m:
+            v = int(v) % (1 << 64)
+            d = 0
+            while v > 1:
+                v >>= rl
+                d += 1
+            counts.append(d)
+        return np.array(counts, dtype=np.int64)
+
+    def sap_decompose(self, lwes, radix, beta, restart=None, max_digits=None):
         """SAPDecomposer::Decompose of [count][n+1] LWEs: returns (digits [count][D][N+1], counts)
         with D the largest digit count; rows beyond an input's own count are zero."""
         x = np.ascontiguousarray(lwes, dtype=np.uint64).reshape(-1, self.n + 1)
         count = x.shape[0]
+        if max_digits is None:
+            max_digits = max(1, int(self.sap_digit_counts(x, radix).max()) if count else 1)
         d_in = DeviceBuffer.from_host(self, x)
         d_out = DeviceBuffer.from_host(self, np.zeros(count * max_digits * (self.N + 1), np.uint64))
         d_cnt = DeviceBuffer(self, (count + 1) // 2)
