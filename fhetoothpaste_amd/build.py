@@ -22,6 +22,7 @@ HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall"]
 SOURCES = [
     ("kernels.hip", DEVICE_FLAGS),
     ("kernels_br1.hip", DEVICE_FLAGS),
+    ("kernels_br_lat.hip", DEVICE_FLAGS),
     ("kernels_pack.hip", DEVICE_FLAGS),
     ("kernels_eltwise.hip", DEVICE_FLAGS),
     ("kernels_sap.hip", DEVICE_FLAGS),

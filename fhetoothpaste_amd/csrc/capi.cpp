@@ -150,8 +150,11 @@ int launch_eval(tp_context* c, size_t count, const uint64_t* lwes, const uint64_
     tp::BrArgs A{};
     fill_br_args(c, A, lwes, tv, tv_stride, out_lwe, out_rlwe, rlwe_ntt, count);
     if (grid <= 0) grid = (int)count;
-    hipError_t e =
-        tp::launch_blind_rotate(c->logN, (int)c->p.method, (int)c->p.br_digits, A, grid, c->br_waves, c->stream);
+    hipError_t e;
+    if (count == 1 && c->logN == 10 && c->p.br_digits >= 2 && c->p.br_digits <= 4)
+        e = tp::launch_blind_rotate_latency((int)c->p.method, (int)c->p.br_digits, A, c->stream);  // one ciphertext
+    else
+        e = tp::launch_blind_rotate(c->logN, (int)c->p.method, (int)c->p.br_digits, A, grid, c->br_waves, c->stream);
     if (e != hipSuccess) return fail(TP_E_DEVICE, std::string("blind rotation launch: ") + hipGetErrorString(e));
     return TP_OK;
 }

@@ -242,3 +242,28 @@ def test_autotune_picks_a_variant(tp, oracle):
     ctx2 = tp.Context(N, n, 12, 2, 0)
     with pytest.raises(ToothpasteError):
         ctx2.set_br_waves(1)  # not built for 2 digits
+
+
+@pytest.mark.parametrize("method,n,b,l", [(0, 20, 16, 2), (0, 24, 12, 3), (0, 16, 16, 4), (1, 14, 16, 4),
+                                          (1, 12, 14, 3), (2, 12, 14, 3), (2, 8, 16, 2)])
+@pytest.mark.parametrize("rlwe_ntt", [False, True])
+def test_single_ciphertext_latency_kernel(tp, oracle, method, n, b, l, rlwe_ntt):
+    """One ciphertext at N=1024 runs on the digit-parallel kernel (2 l waves); extracted LWE and
+    rotated accumulator (both domains) bit-exact against the oracle."""
+    N = 1024
+    say("latency kernel", method, n, b, l, rlwe_ntt)
+    dist = 1 if method == oracle.TERNARY else 0
+    lk, rk, brk = _keys(oracle, N, n, b, l, method, dist, 3.2, 60 + n + l)
+    ctx = tp.Context(N, n, b, l, method)
+    ctx.import_brk(brk)
+    rng = np.random.default_rng(n * 7 + l)
+    for trial in range(2):
+        lwe = rng.integers(0, 2 * N, size=(1, n + 1), dtype=np.uint64)
+        if trial == 1:
+            lwe[0, ::3] = 0
+        tv = rng.integers(0, Q, size=2 * N, dtype=np.uint64)
+        got, rl = ctx.eval_func_batch(lwe, tv, want_rlwe=True, rlwe_ntt=rlwe_ntt)
+        ref_rl = oracle.blind_rotate(N, lwe[0], tv, brk, n, b, l, method, not rlwe_ntt)
+        assert np.array_equal(rl[0], ref_rl), trial
+        ref_coef = ref_rl if not rlwe_ntt else oracle.blind_rotate(N, lwe[0], tv, brk, n, b, l, method, True)
+        assert np.array_equal(got[0], oracle.sample_extract(ref_coef, N)), trial
