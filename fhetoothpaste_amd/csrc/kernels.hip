@@ -95,6 +95,62 @@ __global__ void __launch_bounds__(kKsBlock) lwe_keyswitch_kernel(KsArgs A) {
     }
 }
 
+// One ciphertext: the same sums, split over the key rows instead of over ciphertexts.  A block of
+// kKsBlock threads covers one tile of kKsCols columns; thread t takes rows t, t + kKsBlock, ... of
+// the N d key rows (each a 16-word, 128-byte segment), and the partial lo / hi sums, exact in 64
+// bits like the totals they add up to, meet in LDS.  The key is read once, so the launch is HBM
+// bound instead of one lane walking all N d rows.
+__global__ void __launch_bounds__(kKsBlock) lwe_keyswitch_one_kernel(KsArgs A) {
+    __shared__ uint64_t part[2][kKsCols][kKsBlock + 1];
+    const int col0 = blockIdx.x * kKsCols;
+    const int t = threadIdx.x;
+    const int W = A.n_out + 1;
+    const int ct = A.rows ? (int)A.rows[0] : 0;
+    const uint64_t* in = A.in + (size_t)ct * A.in_stride;
+    const uint64_t mask = (1ULL << A.base_log2) - 1;
+    uint64_t lo[kKsCols], hi[kKsCols];
+#pragma unroll
+    for (int c = 0; c < kKsCols; ++c) lo[c] = hi[c] = 0;
+    const int rows = A.n_in * A.digits;
+    for (int r = t; r < rows; r += kKsBlock) {
+        const int i = r / A.digits, j = r - i * A.digits;
+        const uint64_t x = gl::add(in[i], A.corrector);
+        const uint32_t u = (uint32_t)((x >> (64 - (j + 1) * A.base_log2)) & mask);
+        const uint64_t* kr = A.ksk + (size_t)r * W + col0;
+#pragma unroll
+        for (int c = 0; c < kKsCols; ++c) {
+            const uint64_t kv = kr[c];
+            lo[c] += (uint64_t)u * (uint32_t)kv;
+            hi[c] += (uint64_t)u * (uint32_t)(kv >> 32);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < kKsCols; ++c) {
+        part[0][c][t] = lo[c];
+        part[1][c][t] = hi[c];
+    }
+    __syncthreads();
+    // thread (h, c) for h < 2, c < kKsCols adds column c's partial sums of half h
+    if (t < 2 * kKsCols) {
+        const int h = t / kKsCols, c = t % kKsCols;
+        uint64_t sum = 0;
+        for (int q = 0; q < kKsBlock; ++q) sum += part[h][c][q];
+        part[h][c][kKsBlock] = sum;
+    }
+    __syncthreads();
+    if (t < kKsCols) {
+        const int col = col0 + t;
+        if (col >= W) return;
+        const uint64_t l = part[0][t][kKsBlock], h = part[1][t][kKsBlock];
+        const uint64_t t_lo = (h << 32) + l;
+        const uint64_t carry = t_lo < l;
+        const uint64_t t_hi = (h >> 32) + carry;
+        const uint64_t acc = gl::sub(gl::reduce128(t_hi, t_lo), A.corr[col]);
+        if (col < A.n_out) A.out[col] = gl::neg(acc);
+        else A.out[col] = gl::sub(in[A.n_in], acc);
+    }
+}
+
 // ------------------------------------------------------------------ launchers
 
 template <int LOGN>
@@ -136,6 +192,11 @@ hipError_t launch_blind_rotate(int logN, int method, int digits, const BrArgs& A
 }
 
 hipError_t launch_lwe_keyswitch(const KsArgs& A, hipStream_t s) {
+    if (A.count == 1) {
+        hipLaunchKernelGGL(lwe_keyswitch_one_kernel, dim3((A.n_out + 1 + kKsCols - 1) / kKsCols), dim3(kKsBlock), 0,
+                           s, A);
+        return hipGetLastError();
+    }
     dim3 grid((A.n_out + 1 + kKsCols - 1) / kKsCols, (A.count + kKsBlock - 1) / kKsBlock);
     hipLaunchKernelGGL(lwe_keyswitch_kernel, grid, dim3(kKsBlock), 0, s, A);
     return hipGetLastError();

@@ -132,6 +132,24 @@ def test_keyswitch_batch_vs_oracle(tp, oracle):
         assert np.array_equal(got[c], oracle.lwe_keyswitch(lwes[c], ksk, N, n_out, b, d)), c
 
 
+@pytest.mark.parametrize("N,n_out,b,d", [(1024, 630, 4, 14), (2048, 800, 5, 12), (1024, 17, 16, 4)])
+def test_keyswitch_single_ciphertext(tp, oracle, N, n_out, b, d):
+    """count == 1 runs the row-split kernel (partial sums over key rows meet in LDS)."""
+    say("lwe keyswitch single", N, n_out, b, d)
+    src = oracle.secret(N, 0, 8, 2)
+    dst = oracle.secret(n_out, 0, 8, 1)
+    ksk = oracle.lwe_ksk(src, dst, N, n_out, b, d, 3.2, 8)
+    ctx = tp.Context(N, 4, 12, 3, 0, n_out, b, d)
+    ctx.import_lwe_ksk(ksk)
+    rng = np.random.default_rng(N + b)
+    for trial in range(3):
+        lwe = rng.integers(0, Q, size=(1, N + 1), dtype=np.uint64)
+        if trial == 1:
+            lwe[0] = Q - 1
+        got = ctx.lwe_keyswitch_batch(lwe)
+        assert np.array_equal(got[0], oracle.lwe_keyswitch(lwe[0], ksk, N, n_out, b, d)), trial
+
+
 def test_bootstrap_batch_cfg2_shape(tp, oracle):
     """n=630, N=1024, b=12 l=3, KS 1024->630 b=4 d=14: bit-exact and decrypts."""
     N, n, b, l, n_out, kb, kd = 1024, 630, 12, 3, 630, 4, 14
