@@ -255,6 +255,51 @@ def extra_workloads(tp, reps=2):
     for b in (d_lwe, d_out, d_cnt):
         b.free()
     ctx.close()
+
+    # LWE -> RGSW scheme switching (the reference test's shape: n=100, N=1024, gadget 2^14 x 4 for the
+    # rotation, the trace and the s^2 -> s switch, 4 output digits of base 2^14), 1024 inputs per call
+    N6, n6, B6 = 1024, 100, 1024
+    seed = SEED + 4
+    lk = tp.keygen_secret(n6, 0, seed, 1)
+    rk = tp.keygen_secret(N6, 0, seed, 2)
+    ctx = tp.Context(N6, n6, 14, 4, 0, 0, 0, 0, 0, 14, 4)
+    ctx.import_brk(tp.keygen_brk(lk, rk, N6, n6, 14, 4, 0, 0, 3.2, seed))
+    ctx.import_auto_keys(tp.keygen_auto_keys(rk, N6, 14, 4, 3.2, seed))
+    # s^2 mod X^N + 1 (binary s, small integer coefficients) for the squaring key
+    full = np.convolve(rk.astype(np.int64), rk.astype(np.int64))
+    sq_int = full[:N6].copy()
+    sq_int[:N6 - 1] -= full[N6:]
+    sq = np.array([v % Q for v in sq_int.tolist()], dtype=np.uint64)
+    ctx.import_rlwe_ksk(tp.keygen_rlwe_ksk(sq, rk, N6, 14, 4, 3.2, seed))
+    lwes = rng.integers(0, 2 * N6, size=(B6, n6 + 1), dtype=np.uint64)
+    d_lwe = DeviceBuffer.from_host(ctx, lwes)
+    d_out = DeviceBuffer(ctx, B6 * 4 * N6 * 4)
+    ms_rg = timed(ctx, lib, chk, lambda: chk(lib.tp_lwe_to_rgsw_device(ctx.h, B6, 4, 14, d_lwe.p, d_out.p)), reps)
+    out["lwe_to_rgsw"] = {
+        "workload": "LWE -> RGSW scheme switching, binary n=100 N=1024, gadget 2^14 x 4 (rotation, trace, "
+                    "s^2 -> s switch), 4 output digits of base 2^14, %d inputs per call" % B6,
+        "inputs": B6, "ms": round(ms_rg, 2), "conversions_per_s": round(B6 / ms_rg * 1000, 1)}
+    log("LWE->RGSW: %d inputs, %.1f ms" % (B6, ms_rg))
+    for b in (d_lwe, d_out):
+        b.free()
+    ctx.close()
+
+    # modulus switch Q -> 2N (ROUND, the reference's long double rounding) of a key-switched batch:
+    # 4096 LWEs of dimension 630, the link from a bootstrap's output back to a blind-rotation input
+    ctx = tp.Context(1024, 2, 16, 4)
+    words = 4096 * 631
+    d_src = DeviceBuffer.from_host(ctx, rng.integers(0, Q, size=words, dtype=np.uint64))
+    d_dst = DeviceBuffer(ctx, words)
+    ms_ms = timed(ctx, lib, chk, lambda: chk(lib.tp_modswitch_device(ctx.h, d_dst.p, d_src.p, words, Q, 2048, 0, 0)),
+                  max(reps, 5))
+    out["modulus_switch"] = {
+        "workload": "ModulusSwitch Q -> 2048 (ROUND) of 4096 LWEs of dimension 630 (2,584,576 words)",
+        "ms": round(ms_ms, 4), "words_per_s": round(words / ms_ms * 1000, 1),
+        "GBps": round(16 * words / ms_ms / 1e6, 1), "hbm_peak_GBps": 8000}
+    log("modulus switch: %d words, %.3f ms" % (words, ms_ms))
+    for b in (d_src, d_dst):
+        b.free()
+    ctx.close()
     return out
 
 

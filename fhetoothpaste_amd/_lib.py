@@ -80,6 +80,7 @@ SIGNATURES = {
     "tp_sap_decompose_device": [vp, SZ, U32, U32, U64, vp, vp, U32, vp],
     "tp_sap_homtrunc_device": [vp, SZ, U32, vp, vp],
     "tp_sap_reset_device": [vp, SZ, U64, vp],
+    "tp_lwe_to_rgsw_device": [vp, SZ, U32, U32, vp, vp],
     "tp_keygen_rlwe_ksk": [u64p, u64p, u64p, U32, U32, U32, ctypes.c_double, U64, U64],
     "tp_wire_bytes": [SZ, ctypes.POINTER(SZ)],
     "tp_wire_encode": [ctypes.c_char_p, SZ, INT, U32, ctypes.POINTER(Params), u64p, SZ],

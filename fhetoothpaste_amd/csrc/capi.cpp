@@ -68,6 +68,8 @@ struct tp_context {
     size_t sapw_words = 0;
     uint64_t* d_sap_polys = nullptr;  // SAP NTT-domain polynomials for sap_radix
     uint32_t sap_radix = 0;
+    uint64_t* d_rgsw = nullptr;       // LWE -> RGSW workspace
+    size_t rgsw_words = 0;
     hipEvent_t ev[8] = {};
 };
 
@@ -271,7 +273,7 @@ int tp_context_destroy(tp_context* c) {
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (uint64_t* p : {c->d_tw, c->d_itw, c->d_brk, c->d_ksk, c->d_corr, c->d_scratch, c->d_io, c->d_mono,
-                        c->d_auto, c->d_nodes, c->d_rksk, c->d_sapw, c->d_sap_polys})
+                        c->d_auto, c->d_nodes, c->d_rksk, c->d_sapw, c->d_sap_polys, c->d_rgsw})
         if (p) (void)hipFree(p);
     for (auto& ev : c->ev)
         if (ev) (void)hipEventDestroy(ev);
@@ -1061,6 +1063,80 @@ int tp_sap_decompose_device(tp_context* c, size_t count, uint32_t radix, uint32_
         }
         if ((rc = sap_homtrunc(c, w, count, radix, w.acc, w.acc))) return rc;
     }
+    return TP_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ LWE -> RGSW scheme switching
+//
+// LWEtoRGSWConverter::Convert (lib/src/operators/lwe_to_rgsw_conversion.cpp:163-231) over a batch:
+// one blind rotation launch with the staircase accumulator, the digit rows of every input in one
+// pass, one trace launch over all rows, one inverse NTT and one s^2 -> s key-switch launch, and the
+// left rows in one pass.
+
+extern "C" {
+
+int tp_lwe_to_rgsw_device(tp_context* c, size_t count, uint32_t digits, uint32_t basis_log2, const uint64_t* lwes,
+                          uint64_t* out) {
+    if (!c || !lwes || !out) return fail(TP_E_INVALID, "null argument");
+    if (!c->d_brk) return fail(TP_E_STATE, "blind-rotation key not imported");
+    if (!c->d_auto) return fail(TP_E_STATE, "automorphism (trace) keys not imported");
+    if (!c->d_rksk) return fail(TP_E_STATE, "squaring key (tp_import_rlwe_ksk) not imported");
+    const uint32_t N = c->N;
+    if (basis_log2 == 0 || basis_log2 > 63) return fail(TP_E_INVALID, "basis_log2 must lie in [1, 63]");
+    const uint32_t D = 64 / basis_log2 + (64 % basis_log2 ? 1 : 0);
+    if (digits == 0 || N % digits || digits > D) return fail(TP_E_INVALID, "digits must divide N and not exceed ceil(64 / basis_log2)");
+    if (count == 0) return TP_OK;
+    if (count * digits > 0x7fffffff) return fail(TP_E_INVALID, "count too large");
+    int rc = set_device(c);
+    if (rc) return rc;
+    const uint32_t block = N / digits;
+    // accumulator (:116-161): staircase t of the output gadget, centred on 0; acc = [0 | -t/2], extra = t/2
+    std::vector<uint64_t> host(3 * (size_t)N, 0), t(N, 0);
+    uint64_t scale = 1ULL << (basis_log2 * (D - digits));
+    for (uint32_t i = 0; i < block / 2; ++i) {
+        t[i] = scale;
+        t[N - i - 1] = gl::kQ - scale;
+    }
+    for (uint32_t d = 1; d < digits; ++d) {
+        scale <<= basis_log2;
+        for (uint32_t j = 0; j < block; ++j) t[d * block + j - block / 2] = scale;
+    }
+    const uint64_t inv2 = gl::inv(2);
+    for (uint32_t i = 0; i < N; ++i) {
+        host[2 * N + i] = gl::mul(t[i], inv2);
+        host[N + i] = gl::neg(host[2 * N + i]);
+    }
+    const size_t rows = count * digits, rw = rows * 2 * N;
+    const size_t words = 3 * (size_t)N + count * 2 * N + 3 * rw;
+    if (c->rgsw_words < words) {
+        if (c->d_rgsw) TP_HIP(hipFree(c->d_rgsw));
+        c->d_rgsw = nullptr;
+        c->rgsw_words = 0;
+        if (hipMalloc(&c->d_rgsw, words * 8) != hipSuccess) return fail(TP_E_NOMEM, "LWE->RGSW workspace allocation failed");
+        c->rgsw_words = words;
+    }
+    uint64_t* acc = c->d_rgsw;
+    uint64_t* extra = acc + 2 * N;
+    uint64_t* row0 = acc + 3 * N;
+    uint64_t* A = row0 + count * 2 * N;  // digit rows (coefficients), then their inverse transform
+    uint64_t* B = A + rw;                // right rows (NTT)
+    uint64_t* C = B + rw;                // left rows
+    TP_HIP(hipMemcpyAsync(acc, host.data(), 3 * (size_t)N * 8, hipMemcpyHostToDevice, c->stream));
+    TP_HIP(hipStreamSynchronize(c->stream));  // host is a local buffer
+    if ((rc = launch_eval(c, count, lwes, acc, 0, nullptr, row0, 0, 0))) return rc;
+    TP_LAUNCH("RGSW digit rows", tp::launch_rgsw_expand(A, row0, extra, (int)count, digits, block, N, c->stream));
+    TP_LAUNCH("RGSW trace", tp::launch_trace(c->logN, auto_args(c), A, B, (int)rows, c->d_auto, c->stream));
+    TP_LAUNCH("RGSW inverse NTT", tp::launch_ntt_batch(c->logN, A, B, 2 * rows, c->d_itw, true, c->stream));
+    // s^2 -> s switch of (a, 0): mode 2 with t = 1 ignores b (:221-224)
+    TP_LAUNCH("RGSW squaring switch", tp::launch_rlwe_keyswitch(c->logN, auto_args(c), A, C, (int)rows, 1,
+                                                                 TP_RKS_SWITCH_ONLY, 0, c->d_rksk, c->stream));
+    TP_LAUNCH("RGSW left rows", tp::launch_rgsw_left(C, B, rows, N, c->stream));
+    const size_t half = (size_t)digits * 2 * N * 8, pitch = 2 * half;
+    TP_HIP(hipMemcpy2DAsync(out, pitch, C, half, half, count, hipMemcpyDeviceToDevice, c->stream));
+    TP_HIP(hipMemcpy2DAsync(out + (size_t)digits * 2 * N, pitch, B, half, half, count, hipMemcpyDeviceToDevice,
+                            c->stream));
     return TP_OK;
 }
 

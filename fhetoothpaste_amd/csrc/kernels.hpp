@@ -103,6 +103,11 @@ hipError_t launch_sap_node_leaves(uint64_t* out, const uint64_t* x, const uint64
                                   const uint64_t* m_odd, size_t rows, uint32_t N, int accumulate, hipStream_t s);
 hipError_t launch_sap_extract(uint64_t* out, const uint64_t* rlwe, const uint32_t* digits, uint32_t digit,
                               size_t out_stride, int count, uint32_t N, hipStream_t s);
+// LWE -> RGSW (kernels_rgsw.hip): the digit rows of the rotated accumulator, and the left Lev rows
+// from the key-switched ones.
+hipError_t launch_rgsw_expand(uint64_t* rows, const uint64_t* row0, const uint64_t* extra, int count, uint32_t digits,
+                              uint32_t block, uint32_t N, hipStream_t s);
+hipError_t launch_rgsw_left(uint64_t* left, const uint64_t* right, size_t rows, uint32_t N, hipStream_t s);
 constexpr int kKsPadWords = 16;  // zero words after the device key-switching key
 
 }  // namespace tp

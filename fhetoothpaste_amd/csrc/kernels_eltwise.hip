@@ -36,7 +36,7 @@ __global__ void __launch_bounds__(256) eltwise_kernel(uint64_t* dst, const uint6
 // ones are reduced by one subtraction first), outputs in [0, 2^t).  kind 3 (RANDOM) picks floor or
 // ceil by a coin from a counter-based stream (seed, index), where the reference draws an unseeded one.
 __global__ void __launch_bounds__(256) modswitch_kernel(uint64_t* dst, const uint64_t* src, size_t words, uint64_t sM,
-                                                        int t, int kind, uint64_t seed) {
+                                                        uint64_t mu, int t, int kind, uint64_t seed) {
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += stride) {
         uint64_t v = src[i];
@@ -48,7 +48,7 @@ __global__ void __launch_bounds__(256) modswitch_kernel(uint64_t* dst, const uin
             z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
             k = ((z ^ (z >> 31)) >> 63) ? kMsFloor : kMsCeil;
         }
-        dst[i] = ms_switch(v, sM, t, k);
+        dst[i] = ms_switch(v, sM, mu, t, k);
     }
 }
 
@@ -59,7 +59,8 @@ hipError_t launch_modswitch(uint64_t* dst, const uint64_t* src, size_t words, ui
     if (words == 0) return hipSuccess;
     size_t groups = (words + 255) / 256;
     if (groups > 8192) groups = 8192;
-    hipLaunchKernelGGL(modswitch_kernel, dim3((unsigned)groups), dim3(256), 0, s, dst, src, words, sM, t, kind, seed);
+    hipLaunchKernelGGL(modswitch_kernel, dim3((unsigned)groups), dim3(256), 0, s, dst, src, words, sM,
+                       ms_reciprocal(sM), t, kind, seed);
     return hipGetLastError();
 }
 

@@ -28,26 +28,34 @@ MS_FN int ms_clz64(uint64_t x) {
 #endif
 }
 
+// mu = floor((2^128 - 1) / sM) - 2^64, the part of the 65-bit reciprocal of sM below 2^64
+// (sM >= 2^63), computed once per modulus on the host.
+inline uint64_t ms_reciprocal(uint64_t sM) {
+    return (uint64_t)((~(unsigned __int128)0) / sM - ((unsigned __int128)1 << 64));
+}
+
 // Round-to-nearest-even 64-bit significand of v / sM for 0 < v < sM, sM >= 2^63:
 // v / sM = m * 2^-(64 + s), 2^63 <= m < 2^64.
-MS_FN void ms_quotient(uint64_t v, uint64_t sM, uint64_t& m, int& s) {
+MS_FN void ms_quotient(uint64_t v, uint64_t sM, uint64_t mu, uint64_t& m, int& s) {
     s = ms_clz64(v);
     uint64_t w = v << s;  // top bit set
     if (w >= sM) {        // then v 2^(s-1) lies in [sM/2, sM)
         --s;
         w = v << s;
     }
-    // q = floor(w 2^64 / sM), w < sM: restoring division, one quotient bit per step
-    uint64_t q = 0, r = w;
-    for (int i = 0; i < 64; ++i) {
-        const uint64_t top = r >> 63;
-        r <<= 1;
-        q <<= 1;
-        if (top || r >= sM) {
-            r -= sM;
-            q |= 1;
-        }
+    // q = floor(w 2^64 / sM), w < sM, by the reciprocal: w (2^64 + mu) / 2^64 underestimates it by
+    // less than 2, so at most two corrections follow; r = w 2^64 - q sM ends in [0, sM)
+    uint64_t q = w + (uint64_t)(((unsigned __int128)w * mu) >> 64);
+    unsigned __int128 rr = ((unsigned __int128)w << 64) - (unsigned __int128)q * sM;
+    if (rr >= sM) {
+        rr -= sM;
+        ++q;
     }
+    if (rr >= sM) {
+        rr -= sM;
+        ++q;
+    }
+    const uint64_t r = (uint64_t)rr;
     // nearest-even on the remainder: compare 2r with sM (2r may need 65 bits)
     const bool above = (r >> 63) || (r << 1) > sM;
     const bool tie = !(r >> 63) && (r << 1) == sM;
@@ -61,12 +69,12 @@ MS_FN void ms_quotient(uint64_t v, uint64_t sM, uint64_t& m, int& s) {
     m = q;
 }
 
-// v < sM, sM >= 2^63, tM = 2^t with 1 <= t <= 62.
-MS_FN uint64_t ms_switch(uint64_t v, uint64_t sM, int t, int kind) {
+// v < sM, sM >= 2^63, mu = ms_reciprocal(sM), tM = 2^t with 1 <= t <= 62.
+MS_FN uint64_t ms_switch(uint64_t v, uint64_t sM, uint64_t mu, int t, int kind) {
     if (v == 0) return 0;
     uint64_t m;
     int s;
-    ms_quotient(v, sM, m, s);
+    ms_quotient(v, sM, mu, m, s);
     // x = tM fl(v/sM) = m 2^-(64 + s - t); sh = 64 + s - t >= 2 fractional bits
     const int sh = 64 + s - t;
     uint64_t ip, frac_half, frac_rest;  // integer part, first fractional bit, any later bit

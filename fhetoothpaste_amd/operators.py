@@ -268,6 +268,14 @@ class Context:
         check(lib().tp_sap_reset_device(self.h, x.shape[0], radix, d.p))
         return d.to_host().reshape(x.shape)
 
+    def lwe_to_rgsw(self, lwes, digits, basis_log2):
+        """LWEtoRGSWConverter::Convert of [count][n+1] LWEs -> [count][4 N digits] (left | right Lev)."""
+        x = np.ascontiguousarray(lwes, dtype=np.uint64).reshape(-1, self.n + 1)
+        d_in = DeviceBuffer.from_host(self, x)
+        d_out = DeviceBuffer(self, x.shape[0] * 4 * self.N * digits)
+        check(lib().tp_lwe_to_rgsw_device(self.h, x.shape[0], digits, basis_log2, d_in.p, d_out.p))
+        return d_out.to_host().reshape(x.shape[0], 4 * self.N * digits)
+
     def eltwise(self, op, a, b=None, c=None, scalar=0):
         a = np.ascontiguousarray(a, dtype=np.uint64)
         bufs = [DeviceBuffer.from_host(self, np.ascontiguousarray(v, dtype=np.uint64)) if v is not None else None

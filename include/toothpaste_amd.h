@@ -185,6 +185,16 @@ int tp_sap_decompose_device(tp_context* ctx, size_t count, uint32_t radix, uint3
 int tp_sap_homtrunc_device(tp_context* ctx, size_t count, uint32_t radix, const uint64_t* in, uint64_t* out);
 int tp_sap_reset_device(tp_context* ctx, size_t count, uint64_t radix, uint64_t* acc);
 
+/* LWE -> RGSW scheme switching, batched: LWEtoRGSWConverter::Convert(output, input)
+ * (lib/include/operators/lwe_to_rgsw_conversion.h:85-104, lwe_to_rgsw_conversion.cpp:163-231) over
+ * `count` inputs.  The context holds the blind-rotation key, the trace's automorphism keys
+ * (tp_import_auto_keys) and the s^2 -> s key (tp_import_rlwe_ksk), the last two with the context's
+ * rlwe_ks gadget.  lwes [count][lwe_dim+1] in Z_2N; out [count][4 N digits]: per input the left Lev
+ * (RLWE(-s m B^i), digits x 2N) then the right Lev (RLWE(m B^i)), NTT domain, written in full.
+ * digits must divide N and not exceed ceil(64 / basis_log2). */
+int tp_lwe_to_rgsw_device(tp_context* ctx, size_t count, uint32_t digits, uint32_t basis_log2, const uint64_t* lwes,
+                          uint64_t* out);
+
 /* Negacyclic NTT over Z_Q[X]/(X^N+1) of `count` polynomials (device pointers).
  * Replace SolinasWorker::ForwardNTT / BackwardNTT (lib/src/backend/solinas.cpp:228-313);
  * dst may equal src. */

@@ -29,6 +29,7 @@ int main() {
     uint64_t st = 7, checked = 0;
     for (uint64_t sM : moduli)
         for (int t : ts) {
+            const uint64_t mu = tp::ms_reciprocal(sM);
             const uint64_t tM = 1ULL << t;
             for (int i = 0; i < 40000; ++i) {
                 uint64_t v;
@@ -45,7 +46,7 @@ int main() {
                 }
                 if (v >= sM) v %= sM;
                 for (int kind = 0; kind < 3; ++kind) {
-                    const uint64_t a = tp::ms_switch(v, sM, t, kind), b = ld_switch(v, sM, tM, kind);
+                    const uint64_t a = tp::ms_switch(v, sM, mu, t, kind), b = ld_switch(v, sM, tM, kind);
                     ++checked;
                     if (a != b) {
                         std::printf("mismatch sM=%llx t=%d kind=%d v=%llu: emulated %llu, long double %llu\n",

@@ -187,3 +187,28 @@ def test_modswitch_vs_long_double(tp, oracle, tM):
     r = ctx.modswitch(v, Q, tM, 3, seed=5)
     assert np.all((r == oracle.modswitch(v, Q, tM, 1)) | (r == oracle.modswitch(v, Q, tM, 2)))
     assert np.array_equal(r, ctx.modswitch(v, Q, tM, 3, seed=5))  # the seeded coin is reproducible
+
+
+@pytest.mark.parametrize("digits,basebits,b,d", [(4, 10, 16, 4), (2, 20, 16, 4), (4, 14, 14, 4)])
+def test_lwe_to_rgsw_batch_vs_oracle(tp, oracle, digits, basebits, b, d):
+    """Batched LWEtoRGSWConverter::Convert (lwe_to_rgsw_conversion.cpp:163-231): every output word
+    equals the oracle's, for each input of the batch."""
+    N, n = 1024, 16
+    say("lwe to rgsw batch", digits, basebits, b, d)
+    lk = oracle.secret(n, 0, 120 + digits, 1)
+    s = oracle.secret(N, 0, 120 + digits, 2)
+    brk = oracle.brk(N, lk, s, n, b, d, oracle.BINARY, 0, 3.2, 121)
+    tkeys = oracle.auto_keys(N, s, b, d, 3.2, 122)
+    R = oracle.Ring(N)
+    s2 = R.intt(np.array([int(x) * int(x) % Q for x in R.ntt(s)], dtype=np.uint64))
+    sq = oracle.rlwe_ksk(N, s2, s, b, d, 3.2, 123)
+    ctx = tp.Context(N, n, b, d, 0, 0, 0, 0, 0, b, d)
+    ctx.import_brk(brk)
+    ctx.import_auto_keys(tkeys)
+    ctx.import_rlwe_ksk(sq)
+    rng = np.random.default_rng(digits + basebits)
+    lwes = rng.integers(0, 2 * N, size=(3, n + 1), dtype=np.uint64)
+    got = ctx.lwe_to_rgsw(lwes, digits, basebits)
+    for c in range(3):
+        ref = oracle.lwe_to_rgsw(N, lwes[c], brk, n, b, d, oracle.BINARY, tkeys, b, d, sq, b, d, digits, basebits)
+        assert np.array_equal(got[c], ref), c
