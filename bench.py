@@ -300,6 +300,25 @@ def extra_workloads(tp, reps=2):
     for b in (d_src, d_dst):
         b.free()
     ctx.close()
+
+    # wire format (host code): encode and decode of a blind-rotation key of the headline shape
+    # (binary n=630, N=1024, l=3: 7,741,440 words), header checks and FNV-1a checksum included
+    from fhetoothpaste_amd import wire
+    params = _lib.Params(N, n, BB, L, 0, KS_OUT, KS_B, KS_D, 0, 0)
+    key = rng.integers(0, Q, size=n * 4 * N * L, dtype=np.uint64)
+    t0 = time.perf_counter()
+    blob = wire.encode(wire.BRK, params, key, wire.NTT_DOMAIN)
+    t1 = time.perf_counter()
+    kind, _, _, back = wire.decode(blob)
+    t2 = time.perf_counter()
+    if kind != wire.BRK or not np.array_equal(back, key):
+        raise RuntimeError("wire round trip changed the key")
+    out["wire_format"] = {
+        "workload": "TPW1 encode + decode (host, one thread) of a binary n=630 N=1024 l=3 blind-rotation key "
+                    "(%d bytes)" % len(blob),
+        "encode_ms": round((t1 - t0) * 1e3, 2), "decode_ms": round((t2 - t1) * 1e3, 2),
+        "encode_GBps": round(len(blob) / (t1 - t0) / 1e9, 2), "decode_GBps": round(len(blob) / (t2 - t1) / 1e9, 2)}
+    log("wire format: encode %.1f ms, decode %.1f ms" % ((t1 - t0) * 1e3, (t2 - t1) * 1e3))
     return out
 
 

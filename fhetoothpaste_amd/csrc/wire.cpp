@@ -20,6 +20,8 @@
 #include "../../include/toothpaste_amd.h"
 #include "context_internal.hpp"
 
+static_assert(__BYTE_ORDER__ == __ORDER_LITTLE_ENDIAN__, "payload words are copied as little-endian bytes");
+
 namespace {
 
 constexpr uint32_t kVersion = 1;
@@ -99,7 +101,7 @@ int tp_wire_encode(uint8_t* buf, size_t buf_bytes, int kind, uint32_t flags, con
     for (int i = 0; i < 10; ++i) put32(buf + 16 + 4 * i, pv[i]);
     put64(buf + 56, words);
     uint8_t* payload = buf + kHeader;
-    for (size_t i = 0; i < words; ++i) put64(payload + 8 * i, data[i]);
+    std::memcpy(payload, data, words * 8);  // little-endian host: the payload's byte order
     put64(buf + 64, fnv1a(payload, words * 8));
     return TP_OK;
 }
@@ -126,7 +128,7 @@ int tp_wire_decode(const uint8_t* buf, size_t buf_bytes, int* kind, uint32_t* fl
     if (params) *params = p;
     if (data) {
         if (cap_words < n) return werr(TP_E_INVALID, "output buffer too small");
-        for (size_t i = 0; i < n; ++i) data[i] = get64(payload + 8 * i);
+        std::memcpy(data, payload, n * 8);
     }
     return TP_OK;
 }
