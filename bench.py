@@ -361,8 +361,9 @@ def main():
     ctx.import_brk(brk)
     ctx.import_lwe_ksk(ksk)
     tune = ctx.autotune(args.batch)
-    log("blind-rotation variant: %d wave(s)/SIMD (two: %.1f ms, one: %.1f ms)" % (
-        tune["waves"], tune["ms_two_waves"], tune["ms_one_wave"]))
+    log("blind-rotation variant: %d wave(s)/SIMD (two: %.1f ms, one: %.1f ms, digit-parallel: %.1f ms); "
+        "digit-parallel up to %d ciphertexts" % (tune["waves"], tune["ms_two_waves"], tune["ms_one_wave"],
+                                                 tune["ms_digit_parallel"], tune["split_max"]))
 
     B = args.batch
     rng = np.random.default_rng(SEED)
@@ -427,6 +428,32 @@ def main():
             plain_lat.append(time.perf_counter() - t)
     plain_bs = 1.0 / (sum(plain_lat) / len(plain_lat)) if plain_lat else float("nan")
 
+    # batch-size sweep: one full bootstrap launch pair (blind rotation + key switch) per batch size,
+    # with the blind rotation on br_kernel (one wave per ciphertext) and on the digit-parallel
+    # kernel (2l waves per ciphertext); HIP-event times, best of two after a warm-up
+    sweep = []
+    split_tuned = ctx.br_split_max()
+    for cnt in (1, 16, 64, 256, 512, 1024):
+        if cnt > B:
+            break
+        row = {"batch": cnt}
+        for label, split in (("one_wave_per_ct_ms", 0), ("digit_parallel_ms", cnt)):
+            ctx.set_br_split_max(split)
+            step(count=cnt)
+            best = None
+            for _ in range(2):
+                step(count=cnt, mark=True)
+                t_ms = ctypes.c_float()
+                chk(lib.tp_timer_elapsed(ctx.h, 0, 2, ctypes.byref(t_ms)))
+                best = t_ms.value if best is None else min(best, t_ms.value)
+            row[label] = round(best, 3)
+        row["bootstraps_per_s"] = round(cnt / min(row["one_wave_per_ct_ms"], row["digit_parallel_ms"]) * 1000, 1)
+        row["launched_on"] = "digit-parallel" if cnt <= split_tuned else "br_kernel"
+        sweep.append(row)
+    ctx.set_br_split_max(split_tuned)
+    log("batch sweep: " + ", ".join("%d: %.1f/%.1f ms" % (r["batch"], r["one_wave_per_ct_ms"], r["digit_parallel_ms"])
+                                    for r in sweep))
+
     # batch latency percentiles: every bootstrap of a batch completes with its batch
     lat_ms = sorted(br + ks for br, ks in zip(br_ms, ks_ms))
     p50 = float(np.percentile(lat_ms, 50))
@@ -468,7 +495,10 @@ def main():
                      "hbm_key_stream_GBps": round(key_bytes / (br_avg_s + 1e-3 * sum(ks_ms) / len(ks_ms)) / 1e9, 2)},
         "keygen_s": round(keygen_s, 2),
         "tuning": {"br_waves_per_simd": tune["waves"], "br_ms_two_waves": round(tune["ms_two_waves"], 2),
-                   "br_ms_one_wave": round(tune["ms_one_wave"], 2)},
+                   "br_ms_one_wave": round(tune["ms_one_wave"], 2),
+                   "br_ms_digit_parallel": round(tune["ms_digit_parallel"], 2),
+                   "digit_parallel_up_to": split_tuned},
+        "batch_sweep": sweep,
     }
 
     if not args.no_extra:

@@ -2,6 +2,7 @@
 // device layouts), launches.  Declared in include/toothpaste_amd.h.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -64,6 +65,7 @@ struct tp_context {
     size_t perm_len = 0;
     uint64_t* d_rksk = nullptr;  // RLWEtoRLWEConverter key, d x 2N, key-row order
     int br_waves = 2;            // blind-rotation variant: waves per SIMD (tp_set_option / tp_autotune)
+    size_t split_max = 0;        // launches of at most this many ciphertexts use the digit-parallel kernel
     uint64_t* d_sapw = nullptr;  // SAP decomposition workspace (per-input state, accumulators, tree slots)
     size_t sapw_words = 0;
     uint64_t* d_sap_polys = nullptr;  // SAP NTT-domain polynomials for sap_radix
@@ -143,6 +145,15 @@ void fill_br_args(tp_context* c, tp::BrArgs& A, const uint64_t* lwes, const uint
     A.mono = c->d_mono;
 }
 
+// The digit-parallel kernel (2 l waves per ciphertext) is built for N = 1024 with 2..4 digits.
+bool split_built(const tp_context* c) { return c->logN == 10 && c->p.br_digits >= 2 && c->p.br_digits <= 4; }
+
+hipError_t launch_br(tp_context* c, const tp::BrArgs& A, size_t count, int grid) {
+    if (split_built(c) && count <= c->split_max)
+        return tp::launch_blind_rotate_latency((int)c->p.method, (int)c->p.br_digits, A, grid, c->stream);
+    return tp::launch_blind_rotate(c->logN, (int)c->p.method, (int)c->p.br_digits, A, grid, c->br_waves, c->stream);
+}
+
 int launch_eval(tp_context* c, size_t count, const uint64_t* lwes, const uint64_t* tv, size_t tv_stride,
                 uint64_t* out_lwe, uint64_t* out_rlwe, int rlwe_ntt, int grid) {
     if (!c->d_brk) return fail(TP_E_STATE, "blind-rotation key not imported");
@@ -152,11 +163,7 @@ int launch_eval(tp_context* c, size_t count, const uint64_t* lwes, const uint64_
     tp::BrArgs A{};
     fill_br_args(c, A, lwes, tv, tv_stride, out_lwe, out_rlwe, rlwe_ntt, count);
     if (grid <= 0) grid = (int)count;
-    hipError_t e;
-    if (count == 1 && c->logN == 10 && c->p.br_digits >= 2 && c->p.br_digits <= 4)
-        e = tp::launch_blind_rotate_latency((int)c->p.method, (int)c->p.br_digits, A, c->stream);  // one ciphertext
-    else
-        e = tp::launch_blind_rotate(c->logN, (int)c->p.method, (int)c->p.br_digits, A, grid, c->br_waves, c->stream);
+    hipError_t e = launch_br(c, A, count, grid);
     if (e != hipSuccess) return fail(TP_E_DEVICE, std::string("blind rotation launch: ") + hipGetErrorString(e));
     return TP_OK;
 }
@@ -210,6 +217,9 @@ int tp_context_create(tp_context** out, const tp_params* p, int device) {
     c->device = device;
     c->N = N;
     c->logN = ilog2(N);
+    // up to two ciphertexts per CU: two rounds of the digit-parallel kernel (one work-group per CU)
+    // still beat one round of br_kernel at one wave per SIMD
+    if (split_built(c)) c->split_max = 2 * (size_t)(prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 1);
     int rc = set_device(c);
     if (rc) { delete c; return rc; }
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -611,8 +621,7 @@ int launch_chunk(tp_context* c, size_t count, const uint32_t* active, uint32_t* 
     A.state = state;
     A.tv_index = tv_index;
     A.chunk_steps = (int)chunk_steps;
-    hipError_t e = tp::launch_blind_rotate(c->logN, (int)c->p.method, (int)c->p.br_digits, A, (int)count, c->br_waves,
-                                           c->stream);
+    hipError_t e = launch_br(c, A, count, (int)count);
     if (e != hipSuccess) return fail(TP_E_DEVICE, std::string("blind rotation launch: ") + hipGetErrorString(e));
     return TP_OK;
 }
@@ -642,19 +651,34 @@ int tp_bootstrap_batch_device(tp_context* c, size_t count, const uint64_t* lwes,
 
 int tp_set_option(tp_context* c, int option, int value) {
     if (!c) return fail(TP_E_INVALID, "null context");
-    if (option != TP_OPT_BR_WAVES) return fail(TP_E_INVALID, "unknown option");
-    if (value != 1 && value != 2) return fail(TP_E_INVALID, "waves per SIMD must be 1 or 2");
-    if (value == 1 && c->p.br_digits != 3 && c->p.br_digits != 4)
-        return fail(TP_E_INVALID, "the one-wave variant is built for 3 and 4 digits");
-    c->br_waves = value;
-    return TP_OK;
+    if (option == TP_OPT_BR_WAVES) {
+        if (value != 1 && value != 2) return fail(TP_E_INVALID, "waves per SIMD must be 1 or 2");
+        if (value == 1 && c->p.br_digits != 3 && c->p.br_digits != 4)
+            return fail(TP_E_INVALID, "the one-wave variant is built for 3 and 4 digits");
+        c->br_waves = value;
+        return TP_OK;
+    }
+    if (option == TP_OPT_BR_SPLIT_MAX) {
+        if (value < 0) return fail(TP_E_INVALID, "the digit-parallel threshold must be >= 0");
+        if (value > 0 && !split_built(c))
+            return fail(TP_E_INVALID, "the digit-parallel kernel is built for N = 1024 with 2..4 digits");
+        c->split_max = (size_t)value;
+        return TP_OK;
+    }
+    return fail(TP_E_INVALID, "unknown option");
 }
 
 int tp_get_option(tp_context* c, int option, int* value) {
     if (!c || !value) return fail(TP_E_INVALID, "null argument");
-    if (option != TP_OPT_BR_WAVES) return fail(TP_E_INVALID, "unknown option");
-    *value = c->br_waves;
-    return TP_OK;
+    if (option == TP_OPT_BR_WAVES) {
+        *value = c->br_waves;
+        return TP_OK;
+    }
+    if (option == TP_OPT_BR_SPLIT_MAX) {
+        *value = (int)std::min<size_t>(c->split_max, 0x7fffffff);
+        return TP_OK;
+    }
+    return fail(TP_E_INVALID, "unknown option");
 }
 
 int tp_autotune(tp_context* c, size_t count, float* ms) {
@@ -675,9 +699,8 @@ int tp_autotune(tp_context* c, size_t count, float* ms) {
     }
     uint64_t *d_lwe = nullptr, *d_tv = nullptr, *d_out = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    float best = -1.0f;
-    int best_w = c->br_waves;
-    const int saved = c->br_waves;
+    const int saved_w = c->br_waves;
+    const size_t saved_split = c->split_max;
     rc = TP_OK;
     if (hipMalloc(&d_lwe, lwes.size() * 8) != hipSuccess || hipMalloc(&d_tv, 2 * (size_t)N * 8) != hipSuccess ||
         hipMalloc(&d_out, count * (N + 1) * 8) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
@@ -687,13 +710,14 @@ int tp_autotune(tp_context* c, size_t count, float* ms) {
     if (!rc && (hipMemcpyAsync(d_lwe, lwes.data(), lwes.size() * 8, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
                 hipMemsetAsync(d_tv, 0, 2 * (size_t)N * 8, c->stream) != hipSuccess))
         rc = fail(TP_E_DEVICE, "autotune upload");
-    for (int w = 2; w >= 1 && !rc; --w) {
+    // candidates: br_kernel at two and at one wave per SIMD, then the digit-parallel kernel
+    float t_var[3] = {-1.0f, -1.0f, -1.0f};
+    for (int v = 0; v < 3 && !rc; ++v) {
+        if (v == 1 && c->p.br_digits != 3 && c->p.br_digits != 4) continue;
+        if (v == 2 && !split_built(c)) continue;
+        c->br_waves = v == 1 ? 1 : 2;
+        c->split_max = v == 2 ? count : 0;
         float t = -1.0f;
-        if (w == 1 && c->p.br_digits != 3 && c->p.br_digits != 4) {
-            if (ms) ms[2 - w] = t;
-            continue;
-        }
-        c->br_waves = w;
         rc = launch_eval(c, count, d_lwe, d_tv, 0, d_out, nullptr, 0, 0);  // warm-up
         if (!rc && (hipEventRecord(e0, c->stream) != hipSuccess))
             rc = fail(TP_E_DEVICE, "event record");
@@ -701,13 +725,20 @@ int tp_autotune(tp_context* c, size_t count, float* ms) {
         if (!rc && (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
                     hipEventElapsedTime(&t, e0, e1) != hipSuccess))
             rc = fail(TP_E_DEVICE, "autotune timing");
-        if (ms) ms[2 - w] = t;
-        if (!rc && (best < 0 || t < best)) {
-            best = t;
-            best_w = w;
-        }
+        t_var[v] = t;
     }
-    c->br_waves = rc ? saved : best_w;
+    if (rc) {
+        c->br_waves = saved_w;
+        c->split_max = saved_split;
+    } else {
+        c->br_waves = (t_var[1] >= 0 && t_var[1] < t_var[0]) ? 1 : 2;
+        const float t_one = c->br_waves == 1 ? t_var[1] : t_var[0];
+        // the digit-parallel kernel wins at small counts: move the threshold to this count's side
+        if (t_var[2] >= 0 && t_var[2] < t_one) c->split_max = std::max(saved_split, count);
+        else c->split_max = std::min(saved_split, count - 1);
+    }
+    if (ms)
+        for (int v = 0; v < 3; ++v) ms[v] = t_var[v];
     (void)hipStreamSynchronize(c->stream);
     for (void* p : {(void*)d_lwe, (void*)d_tv, (void*)d_out})
         if (p) (void)hipFree(p);

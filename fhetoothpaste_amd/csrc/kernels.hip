@@ -51,6 +51,10 @@ __global__ void __launch_bounds__(256) ntt_batch_kernel(uint64_t* __restrict__ o
 // read through the scalar cache, so each MAC is two v_mad_u64_u32 with a scalar operand.
 constexpr int kKsCols = 16;
 constexpr int kKsBlock = 256;
+// Launches of at most this many ciphertexts use the row-split kernel below: the batched kernel's
+// time is one lane walking all N d key rows, the same for 1 and for 256 ciphertexts (about 5 ms
+// at N = 1024, d = 14), while the row-split kernel reads the key once per ciphertext (about 60 us).
+constexpr int kKsRowSplitMax = 64;
 
 __global__ void __launch_bounds__(kKsBlock) lwe_keyswitch_kernel(KsArgs A) {
     const int col0 = blockIdx.x * kKsCols;
@@ -95,18 +99,20 @@ __global__ void __launch_bounds__(kKsBlock) lwe_keyswitch_kernel(KsArgs A) {
     }
 }
 
-// One ciphertext: the same sums, split over the key rows instead of over ciphertexts.  A block of
-// kKsBlock threads covers one tile of kKsCols columns; thread t takes rows t, t + kKsBlock, ... of
-// the N d key rows (each a 16-word, 128-byte segment), and the partial lo / hi sums, exact in 64
-// bits like the totals they add up to, meet in LDS.  The key is read once, so the launch is HBM
-// bound instead of one lane walking all N d rows.
+// Few ciphertexts: the same sums, split over the key rows instead of over ciphertexts.  Block
+// (x, y) covers one tile of kKsCols columns of ciphertext y; thread t takes rows t, t + kKsBlock,
+// ... of the N d key rows (each a 16-word, 128-byte segment), and the partial lo / hi sums, exact
+// in 64 bits like the totals they add up to, meet in LDS.  Each ciphertext reads the key once
+// (the ciphertexts of a launch share it through the caches), instead of one lane per ciphertext
+// walking all N d rows.
 __global__ void __launch_bounds__(kKsBlock) lwe_keyswitch_one_kernel(KsArgs A) {
     __shared__ uint64_t part[2][kKsCols][kKsBlock + 1];
     const int col0 = blockIdx.x * kKsCols;
     const int t = threadIdx.x;
     const int W = A.n_out + 1;
-    const int ct = A.rows ? (int)A.rows[0] : 0;
+    const int ct = A.rows ? (int)A.rows[blockIdx.y] : (int)blockIdx.y;
     const uint64_t* in = A.in + (size_t)ct * A.in_stride;
+    uint64_t* out = A.out + (size_t)blockIdx.y * W;
     const uint64_t mask = (1ULL << A.base_log2) - 1;
     uint64_t lo[kKsCols], hi[kKsCols];
 #pragma unroll
@@ -146,8 +152,8 @@ __global__ void __launch_bounds__(kKsBlock) lwe_keyswitch_one_kernel(KsArgs A) {
         const uint64_t carry = t_lo < l;
         const uint64_t t_hi = (h >> 32) + carry;
         const uint64_t acc = gl::sub(gl::reduce128(t_hi, t_lo), A.corr[col]);
-        if (col < A.n_out) A.out[col] = gl::neg(acc);
-        else A.out[col] = gl::sub(in[A.n_in], acc);
+        if (col < A.n_out) out[col] = gl::neg(acc);
+        else out[col] = gl::sub(in[A.n_in], acc);
     }
 }
 
@@ -192,9 +198,9 @@ hipError_t launch_blind_rotate(int logN, int method, int digits, const BrArgs& A
 }
 
 hipError_t launch_lwe_keyswitch(const KsArgs& A, hipStream_t s) {
-    if (A.count == 1) {
-        hipLaunchKernelGGL(lwe_keyswitch_one_kernel, dim3((A.n_out + 1 + kKsCols - 1) / kKsCols), dim3(kKsBlock), 0,
-                           s, A);
+    if (A.count <= kKsRowSplitMax) {
+        hipLaunchKernelGGL(lwe_keyswitch_one_kernel, dim3((A.n_out + 1 + kKsCols - 1) / kKsCols, A.count),
+                           dim3(kKsBlock), 0, s, A);
         return hipGetLastError();
     }
     dim3 grid((A.n_out + 1 + kKsCols - 1) / kKsCols, (A.count + kKsBlock - 1) / kKsBlock);

@@ -83,8 +83,9 @@ hipError_t launch_ntt_batch(int logN, uint64_t* out, const uint64_t* in, size_t 
 hipError_t launch_blind_rotate(int logN, int method, int digits, const BrArgs& A, int grid, int waves,
                                hipStream_t s);
 hipError_t launch_blind_rotate_w1(int logN, int method, int digits, const BrArgs& A, int grid, hipStream_t s);
-// One ciphertext at N = 1024 (digits 2..4): the 2 l digit products of a step run on 2 l waves.
-hipError_t launch_blind_rotate_latency(int method, int digits, const BrArgs& A, hipStream_t s);
+// Digit-parallel variant at N = 1024 (digits 2..4): the 2 l digit products of a step run on 2 l
+// waves of one work-group per ciphertext; `grid` work-groups stride over A.count ciphertexts.
+hipError_t launch_blind_rotate_latency(int method, int digits, const BrArgs& A, int grid, hipStream_t s);
 hipError_t launch_lwe_keyswitch(const KsArgs& A, hipStream_t s);
 // SAP digit decomposition (kernels_sap.hip): per-input modulus and digit count; one digit step
 // of the bookkeeping; products of rows of N words with one NTT-domain polynomial; the HomTrunc

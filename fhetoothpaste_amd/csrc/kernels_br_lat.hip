@@ -1,11 +1,13 @@
-// Low-latency blind rotation of a single ciphertext at N = 1024 (the one-bootstrap-per-call path of
-// BlindRotator::BlindRotate, lib/include/interfaces/blindrotation_operator.h:29-40).  br_kernel runs
-// a ciphertext's 2 l digit products one after the other in one wave; here one work-group of 2 l
-// waves runs them side by side: wave (h, j) forms digit j of accumulator half h, its transform and
-// its products with key row h l + j (for ternary and BMMP with the combined key), and writes its
-// two partial products to LDS; waves 0 and 1 then add the 2 l partial products of output half 0 and
-// 1, inverse-transform them and update that half of the accumulator, which lives in LDS.  The
-// arithmetic is br_kernel's, exact in Z_Q, so the result is bit-identical.
+// Digit-parallel blind rotation at N = 1024: 2 l waves per ciphertext (the one-bootstrap-per-call
+// path of BlindRotator::BlindRotate, lib/include/interfaces/blindrotation_operator.h:29-40, and
+// launches of at most a few ciphertexts per CU).  br_kernel runs a ciphertext's 2 l digit products
+// one after the other in one wave; here one work-group of 2 l waves runs them side by side: wave
+// (h, j) forms digit j of accumulator half h, its transform and its products with key row h l + j
+// (for ternary and BMMP with the combined key), and writes its two partial products to LDS; waves 0
+// and 1 then add the 2 l partial products of output half 0 and 1, inverse-transform them and update
+// that half of the accumulator, which lives in LDS.  The arithmetic is br_kernel's, exact in Z_Q,
+// so the result is bit-identical.  Work-groups stride over the ciphertexts of the launch with
+// br_kernel's slot, test-vector and chunking rules (BrArgs).
 #include <hip/hip_runtime.h>
 #include "br_kernel.hpp"
 
@@ -30,175 +32,204 @@ __global__ void __launch_bounds__(2 * LD * 64) br_latency_kernel(BrArgs A) {
     __shared__ uint64_t red[G][2 * N];
     static_assert(S::LDS_WORDS <= 2 * N, "transpose buffer inside the product region");
 
-    const int w = threadIdx.x >> 6, L = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
     const int h = w / LD, j = w % LD;
     const uint64_t mask = (1ULL << A.base_log2) - 1;
     const size_t rgsw_words = (size_t)4 * N * LD;
-    const uint64_t* lwe = A.lwe;
-    const uint64_t* tv = A.tv;
     const int total = METHOD == 2 ? A.n / 2 : A.n;
 
-    if (w < 2) {
-        uint64_t x[kE];
-        rotate_from<LOGN>(x, tv + w * N, (uint32_t)lwe[A.n], L, false);
+    for (int ct = blockIdx.x; ct < A.count; ct += gridDim.x) {
+        // the lane index is re-read per ciphertext so that its LDS and key addresses are formed
+        // inside the loop, not held in registers across it
+        int L = threadIdx.x & 63;
+        asm volatile("" : "+v"(L));
+        const int slot = A.active ? (int)A.active[ct] : ct;
+        const uint64_t* lwe = A.lwe + (size_t)slot * (A.n + 1);
+        const uint64_t* tv =
+            A.tv + (A.tv_index ? (size_t)A.tv_index[slot] * 2 * N : (size_t)slot * A.tv_stride);
+        const int i0 = A.slot_step ? (int)A.slot_step[slot] : 0;
+        const int i1 = (A.chunk_steps > 0 && i0 + A.chunk_steps < total) ? i0 + A.chunk_steps : total;
+        if (w < 2) {
+            if (i0 == 0) {
+                uint64_t x[kE];
+                rotate_from<LOGN>(x, tv + w * N, (uint32_t)lwe[A.n], L, false);
 #pragma unroll
-        for (int k = 0; k < kE; ++k) accL[w][phys(idx1<LOGN>(L, k))] = x[k];
-    }
-    __syncthreads();
-
-    for (int i = 0; i < total; ++i) {
-        uint32_t e0 = 0, e1 = 0, e2 = 0;
-        if constexpr (METHOD == 0 || METHOD == 1) {
-            const uint32_t ai = (uint32_t)lwe[i];
-            if (ai == 0) continue;  // uniform over the work-group
-            e0 = 2 * N - ai;
-            e1 = ai;
-        } else {
-            e0 = (2 * N - (uint32_t)lwe[2 * i]) & (2 * N - 1);
-            e1 = (2 * N - (uint32_t)lwe[2 * i + 1]) & (2 * N - 1);
-            e2 = (e0 + e1) & (2 * N - 1);
-        }
-        const uint64_t* key = A.brk + (size_t)i * NC * rgsw_words;
-        uint64_t* tb = red[w];
-
-        // product phase: digit j of half h
-        uint64_t d[kE];
-        if constexpr (METHOD == 0) {
-            const int shift = A.first_shift - j * A.base_log2;
-            rotate_from<LOGN>(d, accL[h], e0, L, true);
-#pragma unroll
-            for (int k = 0; k < kE; ++k)
-                d[k] = (gl::sub(d[k], accL[h][phys(idx1<LOGN>(L, k))]) >> shift) & mask;
-            if (A.base_log2 <= 16) {
-#pragma unroll
-                for (int k = 0; k < kE / 2; ++k) ct_bfly_small_first(d[k], d[k + kE / 2]);
+                for (int k = 0; k < kE; ++k) accL[w][phys(idx1<LOGN>(L, k))] = x[k];
             } else {
+                const uint64_t* st = A.state + (size_t)slot * 2 * N + (size_t)w * N;
 #pragma unroll
-                for (int k = 0; k < kE / 2; ++k) ct_bfly_pow2(d[k], d[k + kE / 2], gl::kFwdPow2[1]);
-            }
-        } else {
-            const bool last = (j == A.max_digits - 1);
-            const int shift = last ? 0 : 64 - (j + 1) * A.base_log2;
-            const uint64_t dm = last ? (2 * A.least - 1) : mask;
-            const uint64_t delta = last ? A.least : (1ULL << (A.base_log2 - 1));
-#pragma unroll
-            for (int k = 0; k < kE; ++k) d[k] = (gl::add(accL[h][phys(idx1<LOGN>(L, k))], A.corrector) >> shift) & dm;
-            if (dm < (1ULL << 16)) {
-                uint64_t kp, km;
-                signed_first_consts(delta, kp, km);
-#pragma unroll
-                for (int k = 0; k < kE / 2; ++k) ct_bfly_signed_first(d[k], d[k + kE / 2], kp, km);
-            } else {
-#pragma unroll
-                for (int k = 0; k < kE; ++k) d[k] = gl::sub(d[k], delta);
-#pragma unroll
-                for (int k = 0; k < kE / 2; ++k) ct_bfly_pow2(d[k], d[k + kE / 2], gl::kFwdPow2[1]);
+                for (int k = 0; k < kE; ++k) accL[w][phys(idx1<LOGN>(L, k))] = st[idx1<LOGN>(L, k)];
             }
         }
-        ntt_forward<LOGN, 1>(d, tb, A.tw, L);
-        uint64_t out[2][kE];
-        const size_t row = (size_t)(h * LD + j) * 2 * N;
-        if constexpr (METHOD == 0) {
-            const U64x2* ka = reinterpret_cast<const U64x2*>(key + row);
-            const U64x2* kb = reinterpret_cast<const U64x2*>(key + row + N);
-#pragma unroll
-            for (int c = 0; c < kE / 2; ++c) {
-                const U64x2 va = ka[c * S::P + L], vb = kb[c * S::P + L];
-                out[0][2 * c] = gl::mul_lazy(d[2 * c], va.x);
-                out[0][2 * c + 1] = gl::mul_lazy(d[2 * c + 1], va.y);
-                out[1][2 * c] = gl::mul_lazy(d[2 * c], vb.x);
-                out[1][2 * c + 1] = gl::mul_lazy(d[2 * c + 1], vb.y);
+        __syncthreads();
+
+        for (int i = i0; i < i1; ++i) {
+            uint32_t e0 = 0, e1 = 0, e2 = 0;
+            if constexpr (METHOD == 0 || METHOD == 1) {
+                const uint32_t ai = (uint32_t)lwe[i];
+                if (ai == 0) continue;  // uniform over the work-group
+                e0 = 2 * N - ai;
+                e1 = ai;
+            } else {
+                e0 = (2 * N - (uint32_t)lwe[2 * i]) & (2 * N - 1);
+                e1 = (2 * N - (uint32_t)lwe[2 * i + 1]) & (2 * N - 1);
+                e2 = (e0 + e1) & (2 * N - 1);
             }
-        } else {
-            const uint64_t* krow = key + row;
-            const uint64_t* mono = A.mono;
+            const uint64_t* key = A.brk + (size_t)i * NC * rgsw_words;
+            uint64_t* tb = red[w];
+
+            // product phase: digit j of half h
+            uint64_t d[kE];
+            if constexpr (METHOD == 0) {
+                const int shift = A.first_shift - j * A.base_log2;
+                rotate_from<LOGN>(d, accL[h], e0, L, true);
 #pragma unroll
-            for (int c = 0; c < kE / 2; ++c) {
-                const U64x2 w0 = weight_pair<LOGN>(mono, e0, c, L);
-                const U64x2 w1 = weight_pair<LOGN>(mono, e1, c, L);
-                U64x2 w2{0, 0};
-                if constexpr (NC == 3) w2 = bmmp_w2_pair<LOGN>(mono, e0, e1, e2, c, L);
+                for (int k = 0; k < kE; ++k)
+                    d[k] = (gl::sub(d[k], accL[h][phys(idx1<LOGN>(L, k))]) >> shift) & mask;
+                if (A.base_log2 <= 16) {
 #pragma unroll
-                for (int p = 0; p < 2; ++p) {
-                    const U64x2* k0 = reinterpret_cast<const U64x2*>(krow + p * N);
-                    const U64x2* k1 = reinterpret_cast<const U64x2*>(krow + rgsw_words + p * N);
-                    const U64x2 a0 = k0[c * S::P + L], a1 = k1[c * S::P + L];
-                    uint64_t cx = gl::mac_lazy(a1.x, w1.x, gl::mul_lazy(a0.x, w0.x));
-                    uint64_t cy = gl::mac_lazy(a1.y, w1.y, gl::mul_lazy(a0.y, w0.y));
-                    if constexpr (NC == 3) {
-                        const U64x2* k2 = reinterpret_cast<const U64x2*>(krow + 2 * rgsw_words + p * N);
-                        const U64x2 a2 = k2[c * S::P + L];
-                        cx = gl::mac_lazy(a2.x, w2.x, cx);
-                        cy = gl::mac_lazy(a2.y, w2.y, cy);
+                    for (int k = 0; k < kE / 2; ++k) ct_bfly_small_first(d[k], d[k + kE / 2]);
+                } else {
+#pragma unroll
+                    for (int k = 0; k < kE / 2; ++k) ct_bfly_pow2(d[k], d[k + kE / 2], gl::kFwdPow2[1]);
+                }
+            } else {
+                const bool last = (j == A.max_digits - 1);
+                const int shift = last ? 0 : 64 - (j + 1) * A.base_log2;
+                const uint64_t dm = last ? (2 * A.least - 1) : mask;
+                const uint64_t delta = last ? A.least : (1ULL << (A.base_log2 - 1));
+#pragma unroll
+                for (int k = 0; k < kE; ++k)
+                    d[k] = (gl::add(accL[h][phys(idx1<LOGN>(L, k))], A.corrector) >> shift) & dm;
+                if (dm < (1ULL << 16)) {
+                    uint64_t kp, km;
+                    signed_first_consts(delta, kp, km);
+#pragma unroll
+                    for (int k = 0; k < kE / 2; ++k) ct_bfly_signed_first(d[k], d[k + kE / 2], kp, km);
+                } else {
+#pragma unroll
+                    for (int k = 0; k < kE; ++k) d[k] = gl::sub(d[k], delta);
+#pragma unroll
+                    for (int k = 0; k < kE / 2; ++k) ct_bfly_pow2(d[k], d[k + kE / 2], gl::kFwdPow2[1]);
+                }
+            }
+            ntt_forward<LOGN, 1>(d, tb, A.tw, L);
+            uint64_t out[2][kE];
+            const size_t row = (size_t)(h * LD + j) * 2 * N;
+            if constexpr (METHOD == 0) {
+                const U64x2* ka = reinterpret_cast<const U64x2*>(key + row);
+                const U64x2* kb = reinterpret_cast<const U64x2*>(key + row + N);
+#pragma unroll
+                for (int c = 0; c < kE / 2; ++c) {
+                    const U64x2 va = ka[c * S::P + L], vb = kb[c * S::P + L];
+                    out[0][2 * c] = gl::mul_lazy(d[2 * c], va.x);
+                    out[0][2 * c + 1] = gl::mul_lazy(d[2 * c + 1], va.y);
+                    out[1][2 * c] = gl::mul_lazy(d[2 * c], vb.x);
+                    out[1][2 * c + 1] = gl::mul_lazy(d[2 * c + 1], vb.y);
+                }
+            } else {
+                const uint64_t* krow = key + row;
+                const uint64_t* mono = A.mono;
+#pragma unroll
+                for (int c = 0; c < kE / 2; ++c) {
+                    const U64x2 w0 = weight_pair<LOGN>(mono, e0, c, L);
+                    const U64x2 w1 = weight_pair<LOGN>(mono, e1, c, L);
+                    U64x2 w2{0, 0};
+                    if constexpr (NC == 3) w2 = bmmp_w2_pair<LOGN>(mono, e0, e1, e2, c, L);
+#pragma unroll
+                    for (int p = 0; p < 2; ++p) {
+                        const U64x2* k0 = reinterpret_cast<const U64x2*>(krow + p * N);
+                        const U64x2* k1 = reinterpret_cast<const U64x2*>(krow + rgsw_words + p * N);
+                        const U64x2 a0 = k0[c * S::P + L], a1 = k1[c * S::P + L];
+                        uint64_t cx = gl::mac_lazy(a1.x, w1.x, gl::mul_lazy(a0.x, w0.x));
+                        uint64_t cy = gl::mac_lazy(a1.y, w1.y, gl::mul_lazy(a0.y, w0.y));
+                        if constexpr (NC == 3) {
+                            const U64x2* k2 = reinterpret_cast<const U64x2*>(krow + 2 * rgsw_words + p * N);
+                            const U64x2 a2 = k2[c * S::P + L];
+                            cx = gl::mac_lazy(a2.x, w2.x, cx);
+                            cy = gl::mac_lazy(a2.y, w2.y, cy);
+                        }
+                        out[p][2 * c] = gl::mul_lazy(d[2 * c], cx);
+                        out[p][2 * c + 1] = gl::mul_lazy(d[2 * c + 1], cy);
                     }
-                    out[p][2 * c] = gl::mul_lazy(d[2 * c], cx);
-                    out[p][2 * c + 1] = gl::mul_lazy(d[2 * c + 1], cy);
+                }
+            }
+            // the wave's transpose buffer is done with (its own barrier inside the transform)
+#pragma unroll
+            for (int k = 0; k < kE; ++k) {
+                tb[idx3(L, k)] = out[0][k];
+                tb[N + idx3(L, k)] = out[1][k];
+            }
+            __syncthreads();
+
+            // reduction phase: wave p adds the partial products of output half p and updates it
+            if (w < 2) {
+                uint64_t s[kE], prev[kE];
+#pragma unroll
+                for (int k = 0; k < kE; ++k) {
+                    s[k] = red[0][w * N + idx3(L, k)];
+                    prev[k] = accL[w][phys(idx1<LOGN>(L, k))];
+                }
+#pragma unroll
+                for (int g = 1; g < G; ++g)
+#pragma unroll
+                    for (int k = 0; k < kE; ++k) s[k] = gl::add_ll(s[k], red[g][w * N + idx3(L, k)]);
+                ntt_inverse<LOGN>(s, accL[w], A.itw, L);
+                poly_sync<S::WAVES>();
+#pragma unroll
+                for (int k = 0; k < kE; ++k) accL[w][phys(idx1<LOGN>(L, k))] = gl::canon(gl::add_lc(s[k], prev[k]));
+            }
+            __syncthreads();
+        }
+
+        if (A.slot_step && threadIdx.x == 0) A.slot_step[slot] = (uint32_t)i1;
+        if (w < 2) {
+            uint64_t x[kE];
+#pragma unroll
+            for (int k = 0; k < kE; ++k) x[k] = accL[w][phys(idx1<LOGN>(L, k))];
+            if (i1 < total) {
+                // chunk boundary: park the accumulator half for the next launch
+                uint64_t* st = A.state + (size_t)slot * 2 * N + (size_t)w * N;
+#pragma unroll
+                for (int k = 0; k < kE; ++k) st[idx1<LOGN>(L, k)] = x[k];
+            } else {
+                // outputs (one half per wave): sample extraction at index 0 and optional RLWE
+                if (A.out_lwe) {
+                    uint64_t* o = A.out_lwe + (size_t)slot * (N + 1);
+#pragma unroll
+                    for (int k = 0; k < kE; ++k) {
+                        const int jj = idx1<LOGN>(L, k);
+                        if (w == 0) o[jj == 0 ? 0 : N - jj] = jj == 0 ? x[k] : gl::neg(x[k]);
+                        else if (jj == 0) o[N] = x[k];
+                    }
+                }
+                if (A.out_rlwe) {
+                    uint64_t* o = A.out_rlwe + (size_t)slot * 2 * N + (size_t)w * N;
+                    if (A.out_rlwe_ntt) {
+                        poly_sync<S::WAVES>();
+                        ntt_forward<LOGN>(x, accL[w], A.tw, L);
+#pragma unroll
+                        for (int k = 0; k < kE; ++k) o[idx3(L, k)] = gl::canon(x[k]);
+                    } else {
+#pragma unroll
+                        for (int k = 0; k < kE; ++k) o[idx1<LOGN>(L, k)] = x[k];
+                    }
                 }
             }
         }
-        // the wave's transpose buffer is done with (its own barrier inside the transform)
-#pragma unroll
-        for (int k = 0; k < kE; ++k) {
-            tb[idx3(L, k)] = out[0][k];
-            tb[N + idx3(L, k)] = out[1][k];
-        }
+        // the next ciphertext's prologue rewrites the accumulator halves
         __syncthreads();
-
-        // reduction phase: wave p adds the partial products of output half p and updates it
-        if (w < 2) {
-            uint64_t s[kE], prev[kE];
-#pragma unroll
-            for (int k = 0; k < kE; ++k) {
-                s[k] = red[0][w * N + idx3(L, k)];
-                prev[k] = accL[w][phys(idx1<LOGN>(L, k))];
-            }
-#pragma unroll
-            for (int g = 1; g < G; ++g)
-#pragma unroll
-                for (int k = 0; k < kE; ++k) s[k] = gl::add_ll(s[k], red[g][w * N + idx3(L, k)]);
-            ntt_inverse<LOGN>(s, accL[w], A.itw, L);
-            poly_sync<S::WAVES>();
-#pragma unroll
-            for (int k = 0; k < kE; ++k) accL[w][phys(idx1<LOGN>(L, k))] = gl::canon(gl::add_lc(s[k], prev[k]));
-        }
-        __syncthreads();
-    }
-
-    // outputs (waves 0 and 1, one half each): sample extraction at index 0 and optional RLWE
-    if (w < 2) {
-        uint64_t x[kE];
-#pragma unroll
-        for (int k = 0; k < kE; ++k) x[k] = accL[w][phys(idx1<LOGN>(L, k))];
-        if (A.out_lwe) {
-#pragma unroll
-            for (int k = 0; k < kE; ++k) {
-                const int jj = idx1<LOGN>(L, k);
-                if (w == 0) A.out_lwe[jj == 0 ? 0 : N - jj] = jj == 0 ? x[k] : gl::neg(x[k]);
-                else if (jj == 0) A.out_lwe[N] = x[k];
-            }
-        }
-        if (A.out_rlwe) {
-            uint64_t* o = A.out_rlwe + (size_t)w * N;
-            if (A.out_rlwe_ntt) {
-                poly_sync<S::WAVES>();
-                ntt_forward<LOGN>(x, accL[w], A.tw, L);
-#pragma unroll
-                for (int k = 0; k < kE; ++k) o[idx3(L, k)] = gl::canon(x[k]);
-            } else {
-#pragma unroll
-                for (int k = 0; k < kE; ++k) o[idx1<LOGN>(L, k)] = x[k];
-            }
-        }
     }
 }
 
 }  // namespace
 
-// One ciphertext (A.count == 1, no chunking) at N = 1024; digits 2..4.
-hipError_t launch_blind_rotate_latency(int method, int digits, const BrArgs& A, hipStream_t s) {
+// N = 1024, digits 2..4, A.count ciphertexts over `grid` work-groups of 2 * digits waves.
+hipError_t launch_blind_rotate_latency(int method, int digits, const BrArgs& A, int grid, hipStream_t s) {
+    if (grid <= 0) return hipErrorInvalidValue;
 #define TP_LAT_CASE(LD, M) \
     if (digits == LD && method == M) { \
-        hipLaunchKernelGGL((br_latency_kernel<LD, M>), dim3(1), dim3(2 * LD * 64), 0, s, A); \
+        hipLaunchKernelGGL((br_latency_kernel<LD, M>), dim3(grid), dim3(2 * LD * 64), 0, s, A); \
         return hipGetLastError(); \
     }
     TP_LAT_CASE(2, 0) TP_LAT_CASE(2, 1) TP_LAT_CASE(2, 2)

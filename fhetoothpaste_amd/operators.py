@@ -172,16 +172,31 @@ class Context:
         return d_out.to_host()
 
     def autotune(self, count):
-        """Time the blind-rotation variants on `count` synthetic ciphertexts and keep the faster;
-        returns {"waves": chosen, "ms_two_waves": .., "ms_one_wave": .. (-1 if not built)}."""
-        ms = (ctypes.c_float * 2)()
+        """Time the blind-rotation variants on `count` synthetic ciphertexts (tp_autotune): keeps the
+        faster one-wave-per-ciphertext variant and moves the digit-parallel threshold to the side of
+        `count` that won; returns {"waves", "split_max", "ms_two_waves", "ms_one_wave",
+        "ms_digit_parallel"} (-1 for a variant that is not built)."""
+        ms = (ctypes.c_float * 3)()
         check(lib().tp_autotune(self.h, count, ms))
-        w = ctypes.c_int()
-        check(lib().tp_get_option(self.h, T.TP_OPT_BR_WAVES, ctypes.byref(w)))
-        return {"waves": w.value, "ms_two_waves": ms[0], "ms_one_wave": ms[1]}
+        return {"waves": self.br_waves(), "split_max": self.br_split_max(), "ms_two_waves": ms[0],
+                "ms_one_wave": ms[1], "ms_digit_parallel": ms[2]}
 
     def set_br_waves(self, waves):
         check(lib().tp_set_option(self.h, T.TP_OPT_BR_WAVES, waves))
+
+    def br_waves(self):
+        w = ctypes.c_int()
+        check(lib().tp_get_option(self.h, T.TP_OPT_BR_WAVES, ctypes.byref(w)))
+        return w.value
+
+    def set_br_split_max(self, count):
+        """Launches of at most `count` ciphertexts use the digit-parallel kernel (0: never)."""
+        check(lib().tp_set_option(self.h, T.TP_OPT_BR_SPLIT_MAX, count))
+
+    def br_split_max(self):
+        v = ctypes.c_int()
+        check(lib().tp_get_option(self.h, T.TP_OPT_BR_SPLIT_MAX, ctypes.byref(v)))
+        return v.value
 
     def import_rlwe_ksk(self, key):
         key = np.ascontiguousarray(key, dtype=np.uint64)

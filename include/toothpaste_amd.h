@@ -229,13 +229,21 @@ int tp_lwe_keyswitch_batch_device(tp_context* ctx, size_t count, const uint64_t*
 int tp_bootstrap_batch_device(tp_context* ctx, size_t count, const uint64_t* lwes, const uint64_t* tv,
                               size_t tv_stride, uint64_t* out);
 
-/* Tuning.  TP_OPT_BR_WAVES selects the blind-rotation kernel variant: 2 (default; 256 registers
- * per lane, two ciphertexts' waves share a SIMD) or 1 (512 registers, no spills in the ternary
- * and BMMP key combination; built for 3 and 4 digits).  No counterpart in the reference.
- * tp_autotune times both variants on `count` synthetic ciphertexts with the imported key, keeps
- * the faster one for this context and reports ms[0] (two waves) and ms[1] (one wave, -1 if not
- * built); the context's later launches, including the server's, use it. */
+/* Tuning (no counterpart in the reference).
+ * TP_OPT_BR_WAVES selects the one-wave-per-ciphertext blind-rotation kernel: 2 (default; 256
+ * registers per lane, two ciphertexts' waves share a SIMD) or 1 (512 registers, no spills in the
+ * ternary and BMMP key combination; built for 3 and 4 digits).
+ * TP_OPT_BR_SPLIT_MAX: launches of at most this many ciphertexts use the digit-parallel kernel
+ * instead (2 l waves per ciphertext, one work-group of them per CU; N = 1024 with 2..4 digits
+ * only, 0 switches it off).  Default: twice the device's CU count for contexts it is built for,
+ * else 0.  Both kernels give bit-identical results.
+ * tp_autotune times the three on `count` synthetic ciphertexts with the imported key, keeps the
+ * faster one-wave-per-ciphertext variant, moves the digit-parallel threshold to the side of
+ * `count` that won, and reports ms[0] (two waves per SIMD), ms[1] (one wave) and ms[2]
+ * (digit-parallel), -1 where a variant is not built; ms must hold 3 floats.  The context's later
+ * launches, including the server's chunks, follow these settings. */
 #define TP_OPT_BR_WAVES 1
+#define TP_OPT_BR_SPLIT_MAX 2
 int tp_set_option(tp_context* ctx, int option, int value);
 int tp_get_option(tp_context* ctx, int option, int* value);
 int tp_autotune(tp_context* ctx, size_t count, float* ms);

@@ -91,11 +91,13 @@ def _sign_tv(N):
 
 
 @pytest.mark.parametrize("N,n,b,l,count", [(1024, 500, 16, 4, 3), (1024, 630, 12, 3, 5)])
-def test_eval_func_batch_vs_oracle(tp, oracle, N, n, b, l, count):
-    say("eval_func batch", N, n, b, l, count)
+@pytest.mark.parametrize("split", [False, True], ids=["one-wave-per-ct", "digit-parallel"])
+def test_eval_func_batch_vs_oracle(tp, oracle, N, n, b, l, count, split):
+    say("eval_func batch", N, n, b, l, count, split)
     lk, rk, brk = _keys(oracle, N, n, b, l, oracle.BINARY, 0, 3.2, 7)
     ctx = tp.Context(N, n, b, l, 0)
     ctx.import_brk(brk)
+    ctx.set_br_split_max(count if split else 0)
     lwes = oracle.random_mod_pow2(count * (n + 1), 2 * N, 7, 50).reshape(count, n + 1)
     tv = _sign_tv(N)
     got, rl = ctx.eval_func_batch(lwes, tv, want_rlwe=True, rlwe_ntt=False)
@@ -114,16 +116,17 @@ def test_eval_func_batch_vs_oracle(tp, oracle, N, n, b, l, count):
         assert abs(err) < Q // 16, (c, p)
 
 
-def test_keyswitch_batch_vs_oracle(tp, oracle):
+@pytest.mark.parametrize("count", [37, 70])
+def test_keyswitch_batch_vs_oracle(tp, oracle, count):
+    """Up to 64 ciphertexts run the row-split kernel, more the one-lane-per-ciphertext kernel."""
     N, n_out, b, d = 1024, 630, 4, 14
-    say("lwe keyswitch", N, n_out, b, d)
+    say("lwe keyswitch", N, n_out, b, d, count)
     src = oracle.secret(N, 0, 3, 2)
     dst = oracle.secret(n_out, 0, 3, 1)
     ksk = oracle.lwe_ksk(src, dst, N, n_out, b, d, 3.2, 3)
     ctx = tp.Context(N, 4, 12, 3, 0, n_out, b, d)
     ctx.import_lwe_ksk(ksk)
     rng = np.random.default_rng(5)
-    count = 37
     lwes = rng.integers(0, Q, size=(count, N + 1), dtype=np.uint64)
     lwes[0] = Q - 1
     lwes[1] = 0
@@ -229,6 +232,8 @@ def test_one_wave_variant_matches_oracle(tp, oracle, method, N, n, b, l):
     lk, rk, brk = _keys(oracle, N, n, b, l, method, dist, 3.2, 60 + n)
     ctx = tp.Context(N, n, b, l, method)
     ctx.import_brk(brk)
+    if N == 1024:
+        ctx.set_br_split_max(0)  # keep these launches on br_kernel
     lwes = oracle.random_mod_pow2(3 * (n + 1), 2 * N, 61, 62 + method).reshape(3, n + 1)
     tv = np.random.default_rng(n).integers(0, Q, size=2 * N, dtype=np.uint64)
     ctx.set_br_waves(1)
@@ -246,10 +251,16 @@ def test_autotune_picks_a_variant(tp, oracle):
     lk, rk, brk = _keys(oracle, N, n, b, l, oracle.TERNARY, 1, 3.2, 70)
     ctx = tp.Context(N, n, b, l, oracle.TERNARY)
     ctx.import_brk(brk)
+    default_split = ctx.br_split_max()
+    assert default_split >= 2
     r = ctx.autotune(512)
-    assert r["waves"] in (1, 2) and r["ms_two_waves"] > 0 and r["ms_one_wave"] > 0
+    assert r["waves"] in (1, 2) and r["ms_two_waves"] > 0 and r["ms_one_wave"] > 0 and r["ms_digit_parallel"] > 0
     chosen = r["ms_one_wave"] if r["waves"] == 1 else r["ms_two_waves"]
     assert chosen <= min(r["ms_one_wave"], r["ms_two_waves"])
+    if r["ms_digit_parallel"] < chosen:
+        assert r["split_max"] >= 512
+    else:
+        assert r["split_max"] < 512
     # the chosen variant still computes the same rotation
     lwes = oracle.random_mod_pow2(2 * (n + 1), 2 * N, 71, 72).reshape(2, n + 1)
     tv = np.random.default_rng(5).integers(0, Q, size=2 * N, dtype=np.uint64)
@@ -260,6 +271,14 @@ def test_autotune_picks_a_variant(tp, oracle):
     ctx2 = tp.Context(N, n, 12, 2, 0)
     with pytest.raises(ToothpasteError):
         ctx2.set_br_waves(1)  # not built for 2 digits
+    ctx3 = tp.Context(2048, n, b, l, 0)
+    assert ctx3.br_split_max() == 0
+    with pytest.raises(ToothpasteError):
+        ctx3.set_br_split_max(4)  # digit-parallel kernel is built for N = 1024 only
+    ctx3.import_brk(tp.keygen_brk(tp.keygen_secret(n, 0, 71, 1), tp.keygen_secret(2048, 0, 71, 2), 2048, n, b, l,
+                                  0, 0, 3.2, 71))
+    r3 = ctx3.autotune(4)
+    assert r3["ms_digit_parallel"] == -1 and r3["split_max"] == 0
 
 
 @pytest.mark.parametrize("method,n,b,l", [(0, 20, 16, 2), (0, 24, 12, 3), (0, 16, 16, 4), (1, 14, 16, 4),
@@ -285,3 +304,33 @@ def test_single_ciphertext_latency_kernel(tp, oracle, method, n, b, l, rlwe_ntt)
         assert np.array_equal(rl[0], ref_rl), trial
         ref_coef = ref_rl if not rlwe_ntt else oracle.blind_rotate(N, lwe[0], tv, brk, n, b, l, method, True)
         assert np.array_equal(got[0], oracle.sample_extract(ref_coef, N)), trial
+
+
+@pytest.mark.parametrize("method,n,b,l", [(0, 24, 12, 3), (1, 14, 16, 4), (2, 12, 14, 3), (0, 18, 16, 2)])
+def test_digit_parallel_batch_matches_oracle(tp, oracle, method, n, b, l):
+    """A batch on the digit-parallel kernel (2 l waves per ciphertext, one test vector per
+    ciphertext, both output domains) equals br_kernel's launch and the oracle, bit for bit."""
+    N, count = 1024, 6
+    say("digit-parallel batch", method, n, b, l)
+    dist = 1 if method == oracle.TERNARY else 0
+    lk, rk, brk = _keys(oracle, N, n, b, l, method, dist, 3.2, 90 + n + l)
+    ctx = tp.Context(N, n, b, l, method)
+    ctx.import_brk(brk)
+    rng = np.random.default_rng(91 + n)
+    lwes = rng.integers(0, 2 * N, size=(count, n + 1), dtype=np.uint64)
+    lwes[2, ::2] = 0
+    tvs = rng.integers(0, Q, size=(count, 2 * N), dtype=np.uint64)
+    outs = {}
+    for split in (0, count):
+        ctx.set_br_split_max(split)
+        for rlwe_ntt in (False, True):
+            outs[split, rlwe_ntt] = ctx.eval_func_batch(lwes, tvs, want_rlwe=True, rlwe_ntt=rlwe_ntt)
+    for rlwe_ntt in (False, True):
+        assert np.array_equal(outs[0, rlwe_ntt][0], outs[count, rlwe_ntt][0])
+        assert np.array_equal(outs[0, rlwe_ntt][1], outs[count, rlwe_ntt][1])
+    for c in range(count):
+        ref = oracle.blind_rotate(N, lwes[c], tvs[c], brk, n, b, l, method, True)
+        assert np.array_equal(outs[count, False][1][c], ref), c
+        assert np.array_equal(outs[count, False][0][c], oracle.sample_extract(ref, N)), c
+        ref_ntt = oracle.blind_rotate(N, lwes[c], tvs[c], brk, n, b, l, method, False)
+        assert np.array_equal(outs[count, True][1][c], ref_ntt), c

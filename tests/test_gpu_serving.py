@@ -106,14 +106,34 @@ def test_bootstrap_ternary_set_vs_oracle(sets, oracle):
 
 
 @pytest.mark.parametrize("chunk,slots", [(37, 48), (1000, 4096)])
-def test_mix_equals_alone(tp, sets, oracle, chunk, slots):
+@pytest.mark.parametrize("split", [True, False], ids=["digit-parallel", "one-wave-per-ct"])
+def test_mix_equals_alone(tp, sets, oracle, chunk, slots, split):
+    """Each request's result in the serving mix equals its result alone.  The references run alone
+    on br_kernel in one launch; the server's chunks of the N = 1024 set run on the digit-parallel
+    kernel (at most twice the CU count in flight) or, with its threshold at 0, on br_kernel."""
     from fhetoothpaste_amd.serving import Server
     counts = [1, 37, 5, 64, 2, 19, 90, 3, 11, 40]
-    say("serving mix chunk", chunk, "slots", slots, "counts", counts)
+    say("serving mix chunk", chunk, "slots", slots, "split", split, "counts", counts)
     reqs = make_requests(sets, oracle, counts, 31 + chunk)
     names = list(SETS)
+    defaults = {name: sets[name]["ctx"].br_split_max() for name in names}
+    for name in names:
+        sets[name]["ctx"].set_br_split_max(0)
     alone = [sets[name]["ctx"].bootstrap_batch(lwes, tv) for name, lwes, tv in reqs]
+    for name in names:
+        sets[name]["ctx"].set_br_split_max(defaults[name] if split else 0)
     ctxs = [sets[name]["ctx"] for name in names]
+    try:
+        _serve_and_compare(Server, ctxs, reqs, names, alone, counts, chunk, slots)
+    finally:
+        for name in names:
+            sets[name]["ctx"].set_br_split_max(defaults[name])
+    for i in (1, 2):
+        name, lwes, tv = reqs[i]
+        check_decrypts(sets[name], lwes, tv, alone[i])
+
+
+def _serve_and_compare(Server, ctxs, reqs, names, alone, counts, chunk, slots):
     with Server(ctxs, chunk_steps=chunk, max_slots_per_set=slots) as srv:
         handles = []
         for i, (name, lwes, tv) in enumerate(reqs):
@@ -131,9 +151,6 @@ def test_mix_equals_alone(tp, sets, oracle, chunk, slots):
         assert st["ciphertexts"] == sum(counts)
         if chunk < 630:
             assert st["launches"] > 2 * (630 // chunk)
-    for i in (1, 2):
-        name, lwes, tv = reqs[i]
-        check_decrypts(sets[name], lwes, tv, alone[i])
 
 
 def test_cancel_and_reuse(tp, sets, oracle):
