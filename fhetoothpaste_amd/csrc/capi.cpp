@@ -66,8 +66,10 @@ struct tp_context {
     uint64_t* d_rksk = nullptr;  // RLWEtoRLWEConverter key, d x 2N, key-row order
     int br_waves = 2;            // blind-rotation variant: waves per SIMD (tp_set_option / tp_autotune)
     size_t split_max = 0;        // launches of at most this many ciphertexts use the digit-parallel kernel
-    uint64_t* d_sapw = nullptr;  // SAP decomposition workspace (per-input state, accumulators, tree slots)
+    uint64_t* d_sapw = nullptr;  // SAP decomposition workspace (per-input state, accumulators, tree root)
     size_t sapw_words = 0;
+    uint64_t* d_sapt = nullptr;  // SAP HomTrunc tree levels (two buffers of nodes x inputs RLWEs)
+    size_t sapt_words = 0;
     uint64_t* d_sap_polys = nullptr;  // SAP NTT-domain polynomials for sap_radix
     uint32_t sap_radix = 0;
     uint64_t* d_rgsw = nullptr;       // LWE -> RGSW workspace
@@ -283,7 +285,7 @@ int tp_context_destroy(tp_context* c) {
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (uint64_t* p : {c->d_tw, c->d_itw, c->d_brk, c->d_ksk, c->d_corr, c->d_scratch, c->d_io, c->d_mono,
-                        c->d_auto, c->d_nodes, c->d_rksk, c->d_sapw, c->d_sap_polys, c->d_rgsw})
+                        c->d_auto, c->d_nodes, c->d_rksk, c->d_sapw, c->d_sapt, c->d_sap_polys, c->d_rgsw})
         if (p) (void)hipFree(p);
     for (auto& ev : c->ev)
         if (ev) (void)hipEventDestroy(ev);
@@ -885,7 +887,7 @@ struct SapWork {
 
 int sap_workspace(tp_context* c, size_t count, uint32_t lr, SapWork& w) {
     const size_t N = c->N, n1 = (size_t)c->p.lwe_dim + 1, k1 = (size_t)c->p.ks_out_dim + 1;
-    const size_t slots = (size_t)c->logN - lr + 3;
+    const size_t slots = 1;  // the HomTrunc root (the tree levels live in d_sapt)
     const size_t rl = 2 * N * count;
     const size_t words = 2 * n1 * count + 2 * count + 3 * rl + slots * rl + (N + 1) * count + k1 * count + 3 * N;
     if (c->sapw_words < words) {
@@ -927,6 +929,10 @@ int sap_upload_fresh(tp_context* c, const SapWork& w) {
     } while (0)
 
 // HomTrunc (:190-312) of count RLWEs `in` (NTT) into `out` (NTT); out may equal in.
+// RLWEs per HomTrunc tree buffer: inputs are taken in passes so that a pass's level-1 nodes fit
+// (2 buffers of 2^16 RLWEs: 8 GiB at N = 4096, 2 GiB at N = 1024).
+constexpr size_t kSapTreeRlwes = (size_t)1 << 16;
+
 int sap_homtrunc(tp_context* c, const SapWork& w, size_t count, uint32_t radix, const uint64_t* in, uint64_t* out) {
     const uint32_t N = c->N, logN = (uint32_t)c->logN, lr = int_log2_ceil(radix);
     const size_t rows = 2 * count, rl = 2 * (size_t)N * count;
@@ -946,40 +952,45 @@ int sap_homtrunc(tp_context* c, const SapWork& w, size_t count, uint32_t radix, 
         TP_LAUNCH("SAP automorphism", tp::launch_auto_ntt(c->logN, A, slot(0), slot(0), (int)count, (1u << j) + 1, 1,
                                                           c->d_auto + (size_t)(j - 1) * key_words, c->stream));
     }
-    // the clean-up tree (:266-311), in the reference's stack order and slot assignment
-    uint32_t lvl[64];
-    uint32_t ex[64];
-    uint32_t head = 1;
-    lvl[0] = logN - lr;
-    ex[0] = 0;
-    bool first_leaf = true;
-    while (head != 0) {
-        --head;
-        const uint32_t level = lvl[head], e = ex[head];
-        if (level == 0) {
-            TP_LAUNCH("SAP leaf", tp::launch_sap_leaf(out, slot(head), leaf + (size_t)e * N, rows, N,
-                                                      first_leaf ? 0 : 1, c->stream));
-            first_leaf = false;
-            continue;
+    // the clean-up tree (:266-311), level by level: level l holds the 2^(L0 - l) nodes of every input,
+    // node e having children 2e (x + y) and 2e + 1 ((x - y) X^-(N >> l)); the leaf sum is exact in
+    // Z_Q, so this order gives the reference's depth-first stack walk's result
+    const uint32_t L0 = logN - lr;
+    if (L0 == 0) {
+        TP_LAUNCH("SAP leaf", tp::launch_sap_leaf(out, slot(0), leaf, rows, N, 0, c->stream));
+        return TP_OK;
+    }
+    const size_t deep = (size_t)1 << (L0 - 1);  // nodes of level 1
+    const size_t group = std::min(count, std::max<size_t>(1, kSapTreeRlwes / deep));  // inputs per pass
+    const size_t buf_words = deep * group * 2 * N;
+    if (c->sapt_words < 2 * buf_words) {
+        if (c->d_sapt) TP_HIP(hipFree(c->d_sapt));
+        c->d_sapt = nullptr;
+        c->sapt_words = 0;
+        if (hipMalloc(&c->d_sapt, 2 * buf_words * 8) != hipSuccess)
+            return fail(TP_E_NOMEM, "SAP tree workspace allocation failed");
+        c->sapt_words = 2 * buf_words;
+    }
+    for (size_t g0 = 0; g0 < count; g0 += group) {
+        const int gc = (int)std::min(group, count - g0);
+        const uint64_t* X = slot(0) + g0 * 2 * N;  // the root: one node per input
+        int nodes = 1, which = 0;
+        for (uint32_t level = L0; level >= 1; --level) {
+            uint64_t* dst = c->d_sapt + (size_t)which * buf_words;
+            const uint64_t* key = c->d_auto + (size_t)(level - 1) * key_words;
+            if (level > 1) {
+                TP_LAUNCH("SAP tree level", tp::launch_sap_tree(c->logN, A, X, dst, shift + (size_t)level * N, nullptr,
+                                                                nodes, gc, (1u << level) + 1, 0, key, c->stream));
+                nodes *= 2;
+            } else {
+                TP_LAUNCH("SAP tree leaves", tp::launch_sap_tree(c->logN, A, X, dst, leaf, odd, nodes, gc,
+                                                                 (1u << level) + 1, 1, key, c->stream));
+            }
+            X = dst;
+            which ^= 1;
         }
-        TP_LAUNCH("SAP automorphism",
-                  tp::launch_auto_ntt(c->logN, A, slot(head), slot(head + 2), (int)count, (1u << level) + 1, 0,
-                                      c->d_auto + (size_t)(level - 1) * key_words, c->stream));
-        if (level == 1) {
-            // both children are leaves: (x + y) X^brv(2e) + (x - y) X^(-N/2) X^brv(2e+1) in one pass
-            TP_LAUNCH("SAP tree node and leaves",
-                      tp::launch_sap_node_leaves(out, slot(head), slot(head + 2), leaf + (size_t)(2 * e) * N,
-                                                 odd + (size_t)e * N, rows, N, first_leaf ? 0 : 1, c->stream));
-            first_leaf = false;
-            continue;
-        }
-        TP_LAUNCH("SAP tree node", tp::launch_sap_node(slot(head), slot(head + 2), slot(head + 1),
-                                                       shift + (size_t)level * N, rows, N, c->stream));
-        lvl[head] = level - 1;
-        ex[head] = 2 * e;
-        lvl[head + 1] = level - 1;
-        ex[head + 1] = 2 * e + 1;
-        head += 2;
+        TP_LAUNCH("SAP leaf sum", tp::launch_sap_sum_nodes(out + g0 * 2 * N, X, nodes, 2 * (size_t)gc, N, 0,
+                                                           c->stream));
     }
     return TP_OK;
 }

@@ -70,6 +70,12 @@ hipError_t launch_rlwe_keyswitch(int logN, const KeySwitchArgs& A, const uint64_
 // in the evaluation domain); out (+)= the mode-1 result; in may equal out.
 hipError_t launch_auto_ntt(int logN, const KeySwitchArgs& A, const uint64_t* in, uint64_t* out, int count,
                            uint32_t t, int accumulate, const uint64_t* ksk, hipStream_t s);
+// One level of the SAP HomTrunc tree over nodes x count NTT-domain RLWEs X[e][i] (t = 2^level + 1):
+// mode 0 writes the children x + y and (x - y) polys0 to out[2e][i], out[2e + 1][i]; mode 1 writes
+// the leaf contributions (x + y) polys0[2e] + (x - y) polys1[e] to out[e][i].
+hipError_t launch_sap_tree(int logN, const KeySwitchArgs& A, const uint64_t* X, uint64_t* out, const uint64_t* polys0,
+                           const uint64_t* polys1, int nodes, int count, uint32_t t, int mode, const uint64_t* ksk,
+                           hipStream_t s);
 // Modulus switch to 2^t (kind: 0 round, 1 floor, 2 ceil, 3 random with a seeded coin).
 hipError_t launch_modswitch(uint64_t* dst, const uint64_t* src, size_t words, uint64_t sM, int t, int kind,
                             uint64_t seed, hipStream_t s);
@@ -88,20 +94,20 @@ hipError_t launch_blind_rotate_w1(int logN, int method, int digits, const BrArgs
 hipError_t launch_blind_rotate_latency(int method, int digits, const BrArgs& A, int grid, hipStream_t s);
 hipError_t launch_lwe_keyswitch(const KsArgs& A, hipStream_t s);
 // SAP digit decomposition (kernels_sap.hip): per-input modulus and digit count; one digit step
-// of the bookkeeping; products of rows of N words with one NTT-domain polynomial; the HomTrunc
-// tree's inner-node and leaf updates (and a level-1 node fused with its two leaves); sample extraction into digit `digit` of each row's output.
+// of the bookkeeping; products of rows of N words with one NTT-domain polynomial; a HomTrunc leaf
+// whose tree is a single leaf; the sum of the tree's leaf contributions; sample extraction into
+// digit `digit` of each row's output.
 hipError_t launch_sap_init(const uint64_t* lwes, uint64_t* cm, uint32_t* digits, int n, int count, int rl,
                            hipStream_t s);
 hipError_t launch_sap_digit(uint64_t* cp, uint64_t* cm, uint64_t* sub, int n, int count, int rl, uint64_t mask,
                             uint64_t plo, uint64_t phi, uint64_t two_n, hipStream_t s);
 hipError_t launch_sap_poly_mul(uint64_t* dst, const uint64_t* src, const uint64_t* poly, size_t rows, uint32_t N,
                                hipStream_t s);
-hipError_t launch_sap_node(uint64_t* x, const uint64_t* y, uint64_t* odd, const uint64_t* shift, size_t rows,
-                           uint32_t N, hipStream_t s);
 hipError_t launch_sap_leaf(uint64_t* out, const uint64_t* x, const uint64_t* mono, size_t rows, uint32_t N,
                            int accumulate, hipStream_t s);
-hipError_t launch_sap_node_leaves(uint64_t* out, const uint64_t* x, const uint64_t* y, const uint64_t* m_even,
-                                  const uint64_t* m_odd, size_t rows, uint32_t N, int accumulate, hipStream_t s);
+// out (+)= sum over `nodes` consecutive blocks of rows x N words of P.
+hipError_t launch_sap_sum_nodes(uint64_t* out, const uint64_t* P, int nodes, size_t rows, uint32_t N, int accumulate,
+                                hipStream_t s);
 hipError_t launch_sap_extract(uint64_t* out, const uint64_t* rlwe, const uint32_t* digits, uint32_t digit,
                               size_t out_stride, int count, uint32_t N, hipStream_t s);
 // LWE -> RGSW (kernels_rgsw.hip): the digit rows of the rotated accumulator, and the left Lev rows

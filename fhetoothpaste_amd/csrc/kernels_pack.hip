@@ -210,8 +210,37 @@ rlwe_keyswitch_kernel(KeySwitchArgs A, const uint64_t* in, uint64_t* out, int co
 // to its nodes (sap_decomposer.cpp:240-248, :287-291: BackwardNTT then EvalAuto).  a is brought to
 // coefficients in registers; b stays in the evaluation domain, where tau_t is a permutation:
 // NTT(b(X^t))[p] = NTT(b)[q] with 2 brv(q) + 1 = t (2 brv(p) + 1) mod 2N, so it is gathered, not
-// transformed.  out (+)= (-KS(a).a, b' - KS(a).b), equal to rlwe_keyswitch_kernel mode 1 on the
-// inverse transform of in.  All reads of a row precede the sync before its writes, so in may equal out.
+// transformed.  y = (-KS(a).a, b' - KS(a).b), canonical, pass-3 layout (NTT index idx3(L, k)),
+// equal to rlwe_keyswitch_kernel mode 1 on the inverse transform of src.  Ends with a sync after
+// its last read of src.
+template <int LOGN>
+__device__ __forceinline__ void auto_ntt_core(uint64_t (&ya)[kE], uint64_t (&yb)[kE], const uint64_t* src,
+                                              uint64_t* lds, uint32_t t, const KeySwitchArgs& A,
+                                              const uint64_t* ksk, int L) {
+    using S = Shape<LOGN>;
+    constexpr int N = S::N;
+    uint64_t a[kE];
+#pragma unroll
+    for (int k = 0; k < kE; ++k) a[k] = src[idx3(L, k)];
+    ntt_inverse<LOGN>(a, lds, A.itw, L);
+#pragma unroll
+    for (int k = 0; k < kE; ++k) a[k] = gl::canon(a[k]);
+    automorphism_regs<LOGN>(a, lds, t, L);
+    uint64_t oa[kE], ob[kE];
+    rlwe_ks_core<LOGN>(oa, ob, a, lds, A, ksk, L);
+#pragma unroll
+    for (int k = 0; k < kE; ++k) {
+        const uint32_t p = (uint32_t)idx3(L, k);
+        const uint32_t w = 2 * (__builtin_bitreverse32(p) >> (32 - LOGN)) + 1;
+        const uint32_t e = (t * w) & (2 * N - 1);
+        const uint64_t b = src[N + (__builtin_bitreverse32((e - 1) >> 1) >> (32 - LOGN))];
+        ya[k] = gl::neg(gl::canon(oa[k]));
+        yb[k] = gl::sub(gl::canon(b), gl::canon(ob[k]));
+    }
+    poly_sync<S::WAVES>();
+}
+
+// out (+)= tau_t(in) key-switched (auto_ntt_core), one RLWE per work-group; in may equal out.
 template <int LOGN>
 __global__ void __launch_bounds__(Shape<LOGN>::P) __attribute__((amdgpu_waves_per_eu(2)))
 auto_ntt_kernel(KeySwitchArgs A, const uint64_t* in, uint64_t* out, int count, uint32_t t, int accumulate,
@@ -221,36 +250,65 @@ auto_ntt_kernel(KeySwitchArgs A, const uint64_t* in, uint64_t* out, int count, u
     __shared__ uint64_t lds[S::LDS_WORDS];
     const int L = threadIdx.x;
     for (int ct = blockIdx.x; ct < count; ct += gridDim.x) {
-        const uint64_t* src = in + (size_t)ct * 2 * N;
-        uint64_t a[kE];
-#pragma unroll
-        for (int k = 0; k < kE; ++k) a[k] = src[idx3(L, k)];
-        ntt_inverse<LOGN>(a, lds, A.itw, L);
-#pragma unroll
-        for (int k = 0; k < kE; ++k) a[k] = gl::canon(a[k]);
-        automorphism_regs<LOGN>(a, lds, t, L);
-        uint64_t oa[kE], ob[kE];
-        rlwe_ks_core<LOGN>(oa, ob, a, lds, A, ksk, L);
-        uint64_t b[kE];
-#pragma unroll
-        for (int k = 0; k < kE; ++k) {
-            const uint32_t p = (uint32_t)idx3(L, k);
-            const uint32_t w = 2 * (__builtin_bitreverse32(p) >> (32 - LOGN)) + 1;
-            const uint32_t e = (t * w) & (2 * N - 1);
-            b[k] = src[N + (__builtin_bitreverse32((e - 1) >> 1) >> (32 - LOGN))];
-        }
-        poly_sync<S::WAVES>();
+        uint64_t ya[kE], yb[kE];
+        auto_ntt_core<LOGN>(ya, yb, in + (size_t)ct * 2 * N, lds, t, A, ksk, L);
         uint64_t* dst = out + (size_t)ct * 2 * N;
 #pragma unroll
         for (int k = 0; k < kE; ++k) {
-            uint64_t ra = gl::neg(gl::canon(oa[k]));
-            uint64_t rb = gl::sub(gl::canon(b[k]), gl::canon(ob[k]));
+            uint64_t ra = ya[k], rb = yb[k];
             if (accumulate) {
                 ra = gl::add(dst[idx3(L, k)], ra);
                 rb = gl::add(dst[N + idx3(L, k)], rb);
             }
             dst[idx3(L, k)] = ra;
             dst[N + idx3(L, k)] = rb;
+        }
+        poly_sync<S::WAVES>();
+    }
+}
+
+// One level of the SAP HomTrunc tree (sap_decomposer.cpp:266-311) for all its nodes and inputs:
+// RLWE r = e * count + i of X is node e of input i (NTT domain); y = tau_t(x) key-switched
+// (t = 2^level + 1).  mode 0 (inner node, :288-302): children x + y -> node 2e and
+// (x - y) * polys0 -> node 2e + 1 of out ([2 nodes][count] RLWEs).  mode 1 (level 1, both children
+// leaves, :266-277): (x + y) * polys0[2e] + (x - y) * polys1[e] -> out[r]; the caller sums over e.
+template <int LOGN>
+__global__ void __launch_bounds__(Shape<LOGN>::P) __attribute__((amdgpu_waves_per_eu(2)))
+sap_tree_kernel(KeySwitchArgs A, const uint64_t* X, uint64_t* out, const uint64_t* polys0, const uint64_t* polys1,
+                int nodes, int count, uint32_t t, int mode, const uint64_t* ksk) {
+    using S = Shape<LOGN>;
+    constexpr int N = S::N;
+    __shared__ uint64_t lds[S::LDS_WORDS];
+    const int L = threadIdx.x;
+    const int total = nodes * count;
+    for (int r = blockIdx.x; r < total; r += gridDim.x) {
+        const int e = r / count, i = r - e * count;
+        const uint64_t* x = X + (size_t)r * 2 * N;
+        uint64_t ya[kE], yb[kE];
+        auto_ntt_core<LOGN>(ya, yb, x, lds, t, A, ksk, L);
+        if (mode == 0) {
+            uint64_t* ev = out + ((size_t)(2 * e) * count + i) * 2 * N;
+            uint64_t* od = out + ((size_t)(2 * e + 1) * count + i) * 2 * N;
+#pragma unroll
+            for (int k = 0; k < kE; ++k) {
+                const int q = idx3(L, k);
+                const uint64_t xa = x[q], xb = x[N + q], sh = polys0[q];
+                ev[q] = gl::add(xa, ya[k]);
+                ev[N + q] = gl::add(xb, yb[k]);
+                od[q] = gl::mul(gl::sub(xa, ya[k]), sh);
+                od[N + q] = gl::mul(gl::sub(xb, yb[k]), sh);
+            }
+        } else {
+            uint64_t* o = out + (size_t)r * 2 * N;
+            const uint64_t* me = polys0 + (size_t)(2 * e) * N;
+            const uint64_t* mo = polys1 + (size_t)e * N;
+#pragma unroll
+            for (int k = 0; k < kE; ++k) {
+                const int q = idx3(L, k);
+                const uint64_t xa = x[q], xb = x[N + q], m0 = me[q], m1 = mo[q];
+                o[q] = gl::add(gl::mul(gl::add(xa, ya[k]), m0), gl::mul(gl::sub(xa, ya[k]), m1));
+                o[N + q] = gl::add(gl::mul(gl::add(xb, yb[k]), m0), gl::mul(gl::sub(xb, yb[k]), m1));
+            }
         }
         poly_sync<S::WAVES>();
     }
@@ -450,6 +508,17 @@ hipError_t launch_auto_ntt(int logN, const KeySwitchArgs& A, const uint64_t* in,
                            uint32_t t, int accumulate, const uint64_t* ksk, hipStream_t s) {
     TP_LOGN_SWITCH(logN, hipLaunchKernelGGL(auto_ntt_kernel<LG>, dim3(count), dim3(Shape<LG>::P), 0, s, A, in, out,
                                             count, t, accumulate, ksk));
+    return hipGetLastError();
+}
+
+hipError_t launch_sap_tree(int logN, const KeySwitchArgs& A, const uint64_t* X, uint64_t* out, const uint64_t* polys0,
+                           const uint64_t* polys1, int nodes, int count, uint32_t t, int mode, const uint64_t* ksk,
+                           hipStream_t s) {
+    const long long total = (long long)nodes * count;
+    if (total <= 0) return hipSuccess;
+    if (total > 0x7fffffff) return hipErrorInvalidValue;
+    TP_LOGN_SWITCH(logN, hipLaunchKernelGGL(sap_tree_kernel<LG>, dim3((unsigned)total), dim3(Shape<LG>::P), 0, s, A, X,
+                                            out, polys0, polys1, nodes, count, t, mode, ksk));
     return hipGetLastError();
 }
 

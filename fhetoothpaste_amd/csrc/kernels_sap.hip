@@ -1,7 +1,8 @@
 // Kernels of the SAP digit decomposition (lib/src/operators/sap_decomposer.cpp:190-494) for a
 // batch of inputs: the per-digit LWE bookkeeping, the NTT-domain products with fixed polynomials
-// of HomTrunc's automorphism tree, and sample extraction into the digit outputs.  The blind
-// rotations, NTTs and automorphism key switches reuse the library's kernels.  All HBM-bound
+// of HomTrunc, the sum of the tree's leaf contributions, and sample extraction into the digit
+// outputs.  The blind rotations, NTTs and the tree's automorphism key switches (sap_tree_kernel)
+// are in the library's other kernel files.  All HBM-bound
 // grid-stride loops over canonical words; rows of N words share one polynomial (both halves of
 // an RLWE [a|b] are multiplied by the same one, as the reference does at :243-246, :293-298).
 #include <hip/hip_runtime.h>
@@ -78,19 +79,6 @@ __global__ void __launch_bounds__(kBlock) sap_poly_mul_kernel(uint64_t* dst, con
         dst[i] = gl::mul(src[i], poly[i & nmask]);
 }
 
-// Inner node of the HomTrunc tree (sap_decomposer.cpp:288-302) given x = the node and y = its
-// automorphism image: odd = (x - y) * shift, x = x + y.
-__global__ void __launch_bounds__(kBlock) sap_node_kernel(uint64_t* __restrict__ x, const uint64_t* __restrict__ y,
-                                                          uint64_t* __restrict__ odd, const uint64_t* __restrict__ shift,
-                                                          size_t words, uint32_t nmask) {
-    const size_t stride = (size_t)gridDim.x * blockDim.x;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += stride) {
-        const uint64_t a = x[i], b = y[i];
-        odd[i] = gl::mul(gl::sub(a, b), shift[i & nmask]);
-        x[i] = gl::add(a, b);
-    }
-}
-
 // Leaf of the tree (sap_decomposer.cpp:266-277): out (+)= x * monomial.
 __global__ void __launch_bounds__(kBlock) sap_leaf_kernel(uint64_t* __restrict__ out, const uint64_t* __restrict__ x,
                                                           const uint64_t* __restrict__ mono, size_t words,
@@ -102,21 +90,16 @@ __global__ void __launch_bounds__(kBlock) sap_leaf_kernel(uint64_t* __restrict__
     }
 }
 
-// A level-1 node of the tree together with its two leaves: with x the node and y its automorphism
-// image, out (+)= (x + y) * X^brv(2e) + (x - y) * (-X^(N/2) X^brv(2e+1)); the second factor is one
-// precomputed NTT-domain polynomial.
-__global__ void __launch_bounds__(kBlock) sap_node_leaves_kernel(uint64_t* __restrict__ out,
-                                                                 const uint64_t* __restrict__ x,
-                                                                 const uint64_t* __restrict__ y,
-                                                                 const uint64_t* __restrict__ m_even,
-                                                                 const uint64_t* __restrict__ m_odd, size_t words,
-                                                                 uint32_t nmask, int accumulate) {
+// Sum over the level-1 nodes of the level-synchronous tree: out[w] (+)= sum_e P[e][w], w < words
+// (the per-node leaf contributions of every input, rows of 2N words).
+__global__ void __launch_bounds__(kBlock) sap_sum_nodes_kernel(uint64_t* __restrict__ out,
+                                                               const uint64_t* __restrict__ P, int nodes,
+                                                               size_t words, int accumulate) {
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += stride) {
-        const uint64_t a = x[i], b = y[i];
-        const uint32_t j = (uint32_t)(i & nmask);
-        const uint64_t p = gl::add(gl::mul(gl::add(a, b), m_even[j]), gl::mul(gl::sub(a, b), m_odd[j]));
-        out[i] = accumulate ? gl::add(out[i], p) : p;
+        uint64_t acc = accumulate ? out[i] : 0;
+        for (int e = 0; e < nodes; ++e) acc = gl::add(acc, P[(size_t)e * words + i]);
+        out[i] = acc;
     }
 }
 
@@ -168,14 +151,6 @@ hipError_t launch_sap_poly_mul(uint64_t* dst, const uint64_t* src, const uint64_
     return hipGetLastError();
 }
 
-hipError_t launch_sap_node(uint64_t* x, const uint64_t* y, uint64_t* odd, const uint64_t* shift, size_t rows,
-                           uint32_t N, hipStream_t s) {
-    const size_t words = rows * N;
-    if (words == 0) return hipSuccess;
-    hipLaunchKernelGGL(sap_node_kernel, dim3(grid_for(words)), dim3(kBlock), 0, s, x, y, odd, shift, words, N - 1);
-    return hipGetLastError();
-}
-
 hipError_t launch_sap_leaf(uint64_t* out, const uint64_t* x, const uint64_t* mono, size_t rows, uint32_t N,
                            int accumulate, hipStream_t s) {
     const size_t words = rows * N;
@@ -185,12 +160,12 @@ hipError_t launch_sap_leaf(uint64_t* out, const uint64_t* x, const uint64_t* mon
     return hipGetLastError();
 }
 
-hipError_t launch_sap_node_leaves(uint64_t* out, const uint64_t* x, const uint64_t* y, const uint64_t* m_even,
-                                  const uint64_t* m_odd, size_t rows, uint32_t N, int accumulate, hipStream_t s) {
+hipError_t launch_sap_sum_nodes(uint64_t* out, const uint64_t* P, int nodes, size_t rows, uint32_t N, int accumulate,
+                                hipStream_t s) {
     const size_t words = rows * N;
     if (words == 0) return hipSuccess;
-    hipLaunchKernelGGL(sap_node_leaves_kernel, dim3(grid_for(words)), dim3(kBlock), 0, s, out, x, y, m_even, m_odd,
-                       words, N - 1, accumulate);
+    hipLaunchKernelGGL(sap_sum_nodes_kernel, dim3(grid_for(words)), dim3(kBlock), 0, s, out, P, nodes, words,
+                       accumulate);
     return hipGetLastError();
 }
 
