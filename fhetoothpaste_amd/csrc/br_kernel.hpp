@@ -14,16 +14,11 @@ __device__ __forceinline__ int key_pos(int L, int k) {
     return ((k >> 1) * (2 * Shape<LOGN>::P)) + 2 * L + (k & 1);
 }
 
-struct alignas(16) U64x2 {
-    uint64_t x, y;
-};
-
 template <int LOGN>
 __device__ __forceinline__ void load_key_row(uint64_t (&kv)[kE], const uint64_t* __restrict__ row, int L) {
-    const U64x2* r = reinterpret_cast<const U64x2*>(row);
 #pragma unroll
     for (int c = 0; c < kE / 2; ++c) {
-        const U64x2 v = r[c * Shape<LOGN>::P + L];
+        const U64x2 v = ld_pair(row, c * Shape<LOGN>::P + L);
         kv[2 * c] = v.x;
         kv[2 * c + 1] = v.y;
     }
@@ -34,7 +29,7 @@ __device__ __forceinline__ void load_key_row(uint64_t (&kv)[kE], const uint64_t*
 template <int LOGN>
 __device__ __forceinline__ U64x2 mono_pair(const uint64_t* __restrict__ mono, uint32_t e, int c, int L) {
     constexpr int N = Shape<LOGN>::N;
-    const U64x2 v = reinterpret_cast<const U64x2*>(mono + (size_t)(e & (N - 1)) * N)[c * Shape<LOGN>::P + L];
+    const U64x2 v = ld_pair(mono + (size_t)(e & (N - 1)) * N, c * Shape<LOGN>::P + L);
     if (e >= (uint32_t)N) return U64x2{gl::neg(v.x), gl::neg(v.y)};
     return v;
 }
@@ -192,17 +187,28 @@ br_kernel(BrArgs A) {
                     ntt_forward<LOGN, 1>(d, buf, A.tw, L);
                     const size_t row = (size_t)(h * LD + j) * 2 * N;
                     if constexpr (METHOD == 0) {
-                        // key pairs are read next to their use, not hoisted over the transform
-                        const U64x2* ka = reinterpret_cast<const U64x2*>(opaque(key + row));
-                        const U64x2* kb = reinterpret_cast<const U64x2*>(opaque(key + row + N));
+                        // key pairs are read next to their use, not hoisted over the transform,
+                        // four pairs of both rows at a time (one memory latency per group)
+                        const uint64_t* ka = opaque(key + row);
+                        const uint64_t* kb = opaque(key + row + N);
+                        static_for<2>([&](auto G) __attribute__((always_inline)) {
+                            constexpr int c0 = 4 * G;
+                            U64x2 va[4], vb[4];
 #pragma unroll
-                        for (int c = 0; c < kE / 2; ++c) {
-                            const U64x2 va = ka[c * S::P + L], vb = kb[c * S::P + L];
-                            out[0][2 * c] = gl::mac_lazy(d[2 * c], va.x, out[0][2 * c]);
-                            out[0][2 * c + 1] = gl::mac_lazy(d[2 * c + 1], va.y, out[0][2 * c + 1]);
-                            out[1][2 * c] = gl::mac_lazy(d[2 * c], vb.x, out[1][2 * c]);
-                            out[1][2 * c + 1] = gl::mac_lazy(d[2 * c + 1], vb.y, out[1][2 * c + 1]);
-                        }
+                            for (int q = 0; q < 4; ++q) {
+                                va[q] = ld_pair(ka, (c0 + q) * S::P + L);
+                                vb[q] = ld_pair(kb, (c0 + q) * S::P + L);
+                            }
+                            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                            for (int q = 0; q < 4; ++q) {
+                                const int c = c0 + q;
+                                out[0][2 * c] = gl::mac_lazy(d[2 * c], va[q].x, out[0][2 * c]);
+                                out[0][2 * c + 1] = gl::mac_lazy(d[2 * c + 1], va[q].y, out[0][2 * c + 1]);
+                                out[1][2 * c] = gl::mac_lazy(d[2 * c], vb[q].x, out[1][2 * c]);
+                                out[1][2 * c + 1] = gl::mac_lazy(d[2 * c + 1], vb[q].y, out[1][2 * c + 1]);
+                            }
+                        });
                     } else {
                         // combined key sum_c w_c RGSW_c[row], formed pair by pair next to its use
                         const uint64_t* krow = opaque(key + row);
@@ -215,14 +221,12 @@ br_kernel(BrArgs A) {
                             if constexpr (NC == 3) w2 = bmmp_w2_pair<LOGN>(mono, e0, e1, e2, c, L);
 #pragma unroll
                             for (int p = 0; p < 2; ++p) {
-                                const U64x2* k0 = reinterpret_cast<const U64x2*>(krow + p * N);
-                                const U64x2* k1 = reinterpret_cast<const U64x2*>(krow + rgsw_words + p * N);
-                                const U64x2 a0 = k0[c * S::P + L], a1 = k1[c * S::P + L];
+                                const U64x2 a0 = ld_pair(krow + p * N, c * S::P + L);
+                                const U64x2 a1 = ld_pair(krow + rgsw_words + p * N, c * S::P + L);
                                 uint64_t cx = gl::mac_lazy(a1.x, w1.x, gl::mul_lazy(a0.x, w0.x));
                                 uint64_t cy = gl::mac_lazy(a1.y, w1.y, gl::mul_lazy(a0.y, w0.y));
                                 if constexpr (NC == 3) {
-                                    const U64x2* k2 = reinterpret_cast<const U64x2*>(krow + 2 * rgsw_words + p * N);
-                                    const U64x2 a2 = k2[c * S::P + L];
+                                    const U64x2 a2 = ld_pair(krow + 2 * rgsw_words + p * N, c * S::P + L);
                                     cx = gl::mac_lazy(a2.x, w2.x, cx);
                                     cy = gl::mac_lazy(a2.y, w2.y, cy);
                                 }

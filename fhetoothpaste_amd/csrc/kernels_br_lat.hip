@@ -117,11 +117,9 @@ __global__ void __launch_bounds__(2 * LD * 64) br_latency_kernel(BrArgs A) {
             uint64_t out[2][kE];
             const size_t row = (size_t)(h * LD + j) * 2 * N;
             if constexpr (METHOD == 0) {
-                const U64x2* ka = reinterpret_cast<const U64x2*>(key + row);
-                const U64x2* kb = reinterpret_cast<const U64x2*>(key + row + N);
 #pragma unroll
                 for (int c = 0; c < kE / 2; ++c) {
-                    const U64x2 va = ka[c * S::P + L], vb = kb[c * S::P + L];
+                    const U64x2 va = ld_pair(key + row, c * S::P + L), vb = ld_pair(key + row + N, c * S::P + L);
                     out[0][2 * c] = gl::mul_lazy(d[2 * c], va.x);
                     out[0][2 * c + 1] = gl::mul_lazy(d[2 * c + 1], va.y);
                     out[1][2 * c] = gl::mul_lazy(d[2 * c], vb.x);
@@ -138,14 +136,12 @@ __global__ void __launch_bounds__(2 * LD * 64) br_latency_kernel(BrArgs A) {
                     if constexpr (NC == 3) w2 = bmmp_w2_pair<LOGN>(mono, e0, e1, e2, c, L);
 #pragma unroll
                     for (int p = 0; p < 2; ++p) {
-                        const U64x2* k0 = reinterpret_cast<const U64x2*>(krow + p * N);
-                        const U64x2* k1 = reinterpret_cast<const U64x2*>(krow + rgsw_words + p * N);
-                        const U64x2 a0 = k0[c * S::P + L], a1 = k1[c * S::P + L];
+                        const U64x2 a0 = ld_pair(krow + p * N, c * S::P + L);
+                        const U64x2 a1 = ld_pair(krow + rgsw_words + p * N, c * S::P + L);
                         uint64_t cx = gl::mac_lazy(a1.x, w1.x, gl::mul_lazy(a0.x, w0.x));
                         uint64_t cy = gl::mac_lazy(a1.y, w1.y, gl::mul_lazy(a0.y, w0.y));
                         if constexpr (NC == 3) {
-                            const U64x2* k2 = reinterpret_cast<const U64x2*>(krow + 2 * rgsw_words + p * N);
-                            const U64x2 a2 = k2[c * S::P + L];
+                            const U64x2 a2 = ld_pair(krow + 2 * rgsw_words + p * N, c * S::P + L);
                             cx = gl::mac_lazy(a2.x, w2.x, cx);
                             cy = gl::mac_lazy(a2.y, w2.y, cy);
                         }

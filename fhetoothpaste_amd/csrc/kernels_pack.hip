@@ -9,10 +9,6 @@ namespace tp {
 
 namespace {
 
-struct alignas(16) V2 {
-    uint64_t x, y;
-};
-
 // x[k] <- coefficient idx1(L,k) of tau_t(p), p given by the same lanes' registers (coefficients,
 // canonical): out[(i t) mod N] = +-p[i] (utils/math_utils.cpp:70-85), through LDS.
 template <int LOGN>
@@ -85,16 +81,27 @@ __device__ __forceinline__ void rlwe_ks_core(uint64_t (&out_a)[kE], uint64_t (&o
             for (int k = 0; k < kE / 2; ++k) ct_bfly_pow2(d[k], d[k + kE / 2], gl::kFwdPow2[1]);
         }
         ntt_forward<LOGN, 1>(d, lds, A.tw, L);
-        const V2* ka = reinterpret_cast<const V2*>(opaque(ksk + (size_t)j * 2 * N));
-        const V2* kb = reinterpret_cast<const V2*>(opaque(ksk + (size_t)j * 2 * N + N));
+        const uint64_t* ka = opaque(ksk + (size_t)j * 2 * N);
+        const uint64_t* kb = opaque(ksk + (size_t)j * 2 * N + N);
+        // four pairs of both rows at a time (one memory latency per group)
+        static_for<2>([&](auto G) __attribute__((always_inline)) {
+            constexpr int c0 = 4 * G;
+            U64x2 va[4], vb[4];
 #pragma unroll
-        for (int c = 0; c < kE / 2; ++c) {
-            const V2 va = ka[c * S::P + L], vb = kb[c * S::P + L];
-            out_a[2 * c] = gl::mac_lazy(d[2 * c], va.x, out_a[2 * c]);
-            out_a[2 * c + 1] = gl::mac_lazy(d[2 * c + 1], va.y, out_a[2 * c + 1]);
-            out_b[2 * c] = gl::mac_lazy(d[2 * c], vb.x, out_b[2 * c]);
-            out_b[2 * c + 1] = gl::mac_lazy(d[2 * c + 1], vb.y, out_b[2 * c + 1]);
-        }
+            for (int q = 0; q < 4; ++q) {
+                va[q] = ld_pair(ka, (c0 + q) * S::P + L);
+                vb[q] = ld_pair(kb, (c0 + q) * S::P + L);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int c = c0 + q;
+                out_a[2 * c] = gl::mac_lazy(d[2 * c], va[q].x, out_a[2 * c]);
+                out_a[2 * c + 1] = gl::mac_lazy(d[2 * c + 1], va[q].y, out_a[2 * c + 1]);
+                out_b[2 * c] = gl::mac_lazy(d[2 * c], vb[q].x, out_b[2 * c]);
+                out_b[2 * c + 1] = gl::mac_lazy(d[2 * c + 1], vb[q].y, out_b[2 * c + 1]);
+            }
+        });
     }
 }
 

@@ -165,10 +165,38 @@ __device__ __forceinline__ const uint64_t* opaque_after(const uint64_t* p, uint6
     return p;
 }
 
+// Global-memory views of table pointers.  The pointers above come out of an asm statement, which
+// hides that they point to global memory; without the cast the loads would be flat loads, whose
+// completion is counted with the LDS operations' (every LDS wait would also wait for them).
+typedef const __attribute__((address_space(1))) char g_char;
+typedef const __attribute__((address_space(1))) uint64_t g_u64;
+
 // Element idx of a table at a wave-uniform base: a 32-bit byte offset lets the load use the
 // scalar base register plus one offset register, with the constant part in the immediate.
 __device__ __forceinline__ uint64_t ld_u64(const uint64_t* base, uint32_t idx) {
-    return *reinterpret_cast<const uint64_t*>(reinterpret_cast<const char*>(base) + (size_t)(idx * 8u));
+    return *(g_u64*)((g_char*)base + (size_t)(idx * 8u));
+}
+
+// Two adjacent words (16-byte aligned) of a global table: pair i = words 2i, 2i + 1.
+struct alignas(16) U64x2 {
+    uint64_t x, y;
+};
+typedef const __attribute__((address_space(1))) U64x2 g_u64x2;
+__device__ __forceinline__ U64x2 ld_pair(const uint64_t* base, uint32_t i) {
+    g_u64x2* p = (g_u64x2*)((g_char*)base + (size_t)(i * 16u));
+    U64x2 r;
+    r.x = p->x;
+    r.y = p->y;
+    return r;
+}
+
+// The twiddles of one transform level: w[b] = table[first + b], b < nb (nb <= 8), issued together
+// ahead of the level's butterflies (one memory latency per level instead of one per load).
+template <int NB>
+__device__ __forceinline__ void ld_level(uint64_t (&w)[8], const uint64_t* t, uint32_t first) {
+#pragma unroll
+    for (int b = 0; b < NB; ++b) w[b] = ld_u64(t, first + b);
+    __builtin_amdgcn_sched_barrier(0);
 }
 
 // ---------------------------------------------------------------- forward
@@ -190,32 +218,36 @@ __device__ __forceinline__ void fwd_pass1(uint64_t (&x)[kE]) {
 template <int LOGN>
 __device__ __forceinline__ void fwd_pass2(uint64_t (&x)[kE], const uint64_t* __restrict__ tw, int L) {
     constexpr int lb = LOGN - 8;
-#pragma unroll
-    for (int lv = 4; lv < 8; ++lv) {
-        const int half = 8 >> (lv - 4);
+    static_for<4>([&](auto I) __attribute__((always_inline)) {
+        constexpr int lv = 4 + I;
+        constexpr int half = 8 >> (lv - 4);
         const uint64_t* t = opaque_after(tw, x[0]);
+        // blocks ((L >> lb) << (lv - 4)) + (k >> (8 - lv)): 2^(lv - 4) of them per lane
+        uint64_t w[8];
+        ld_level<(1 << (lv - 4))>(w, t, (1u << lv) + (((uint32_t)L >> lb) << (lv - 4)));
 #pragma unroll
         for (int k = 0; k < kE; ++k) {
             if (k & half) continue;
-            const uint32_t blk = (((uint32_t)L >> lb) << (lv - 4)) + (k >> (8 - lv));
-            ct_bfly_mul(x[k], x[k + half], ld_u64(t, (1u << lv) + blk));
+            ct_bfly_mul(x[k], x[k + half], w[k >> (8 - lv)]);
         }
-    }
+    });
 }
 
 template <int LOGN>
 __device__ __forceinline__ void fwd_pass3(uint64_t (&x)[kE], const uint64_t* __restrict__ tw, int L) {
-#pragma unroll
-    for (int lv = 8; lv < LOGN; ++lv) {
-        const int half = 1 << (LOGN - 1 - lv);
+    static_for<LOGN - 8>([&](auto I) __attribute__((always_inline)) {
+        constexpr int lv = 8 + I;
+        constexpr int sh = LOGN - lv;  // blocks (16 L + k) >> sh: 16 >> sh of them per lane
+        constexpr int half = 1 << (LOGN - 1 - lv);
         const uint64_t* t = opaque_after(tw, x[0]);
+        uint64_t w[8];
+        ld_level<(16 >> sh)>(w, t, (1u << lv) + (((uint32_t)L << 4) >> sh));
 #pragma unroll
         for (int k = 0; k < kE; ++k) {
             if (k & half) continue;
-            const uint32_t blk = (((uint32_t)L << 4) + k) >> (LOGN - lv);
-            ct_bfly_mul(x[k], x[k + half], ld_u64(t, (1u << lv) + blk));
+            ct_bfly_mul(x[k], x[k + half], w[k >> sh]);
         }
-    }
+    });
 }
 
 // Natural-order coefficients (pass-1 layout, lazy or canonical) -> lazy NTT values in
@@ -234,33 +266,36 @@ __device__ __forceinline__ void ntt_forward(uint64_t (&x)[kE], uint64_t* lds, co
 
 template <int LOGN>
 __device__ __forceinline__ void inv_pass3(uint64_t (&x)[kE], const uint64_t* __restrict__ itw, int L) {
-#pragma unroll
-    for (int lv = LOGN - 1; lv >= 8; --lv) {
-        const int half = 1 << (LOGN - 1 - lv);
+    static_for<LOGN - 8>([&](auto I) __attribute__((always_inline)) {
+        constexpr int lv = LOGN - 1 - I;
+        constexpr int sh = LOGN - lv;
+        constexpr int half = 1 << (LOGN - 1 - lv);
         const uint64_t* t = opaque_after(itw, x[0]);
+        uint64_t w[8];
+        ld_level<(16 >> sh)>(w, t, (1u << lv) + (((uint32_t)L << 4) >> sh));
 #pragma unroll
         for (int k = 0; k < kE; ++k) {
             if (k & half) continue;
-            const uint32_t blk = (((uint32_t)L << 4) + k) >> (LOGN - lv);
-            gs_bfly_mul(x[k], x[k + half], ld_u64(t, (1u << lv) + blk));
+            gs_bfly_mul(x[k], x[k + half], w[k >> sh]);
         }
-    }
+    });
 }
 
 template <int LOGN>
 __device__ __forceinline__ void inv_pass2(uint64_t (&x)[kE], const uint64_t* __restrict__ itw, int L) {
     constexpr int lb = LOGN - 8;
-#pragma unroll
-    for (int lv = 7; lv >= 4; --lv) {
-        const int half = 8 >> (lv - 4);
+    static_for<4>([&](auto I) __attribute__((always_inline)) {
+        constexpr int lv = 7 - I;
+        constexpr int half = 8 >> (lv - 4);
         const uint64_t* t = opaque_after(itw, x[0]);
+        uint64_t w[8];
+        ld_level<(1 << (lv - 4))>(w, t, (1u << lv) + (((uint32_t)L >> lb) << (lv - 4)));
 #pragma unroll
         for (int k = 0; k < kE; ++k) {
             if (k & half) continue;
-            const uint32_t blk = (((uint32_t)L >> lb) << (lv - 4)) + (k >> (8 - lv));
-            gs_bfly_mul(x[k], x[k + half], ld_u64(t, (1u << lv) + blk));
+            gs_bfly_mul(x[k], x[k + half], w[k >> (8 - lv)]);
         }
-    }
+    });
 }
 
 // Levels m = 8..1 with power-of-two inverse twiddles; N^-1 = 2^(192 - LOGN) is merged into
