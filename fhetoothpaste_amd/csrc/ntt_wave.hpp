@@ -215,88 +215,51 @@ __device__ __forceinline__ void fwd_pass1(uint64_t (&x)[kE]) {
     }
 }
 
-template <int LOGN>
-__device__ __forceinline__ void fwd_pass2(uint64_t (&x)[kE], const uint64_t* __restrict__ tw, int L) {
-    constexpr int lb = LOGN - 8;
-    static_for<4>([&](auto I) __attribute__((always_inline)) {
-        constexpr int lv = 4 + I;
-        constexpr int half = 8 >> (lv - 4);
-        const uint64_t* t = opaque_after(tw, x[0]);
-        // blocks ((L >> lb) << (lv - 4)) + (k >> (8 - lv)): 2^(lv - 4) of them per lane
-        uint64_t w[8];
-        ld_level<(1 << (lv - 4))>(w, t, (1u << lv) + (((uint32_t)L >> lb) << (lv - 4)));
-#pragma unroll
-        for (int k = 0; k < kE; ++k) {
-            if (k & half) continue;
-            ct_bfly_mul(x[k], x[k + half], w[k >> (8 - lv)]);
-        }
-    });
-}
-
-template <int LOGN>
-__device__ __forceinline__ void fwd_pass3(uint64_t (&x)[kE], const uint64_t* __restrict__ tw, int L) {
-    static_for<LOGN - 8>([&](auto I) __attribute__((always_inline)) {
-        constexpr int lv = 8 + I;
-        constexpr int sh = LOGN - lv;  // blocks (16 L + k) >> sh: 16 >> sh of them per lane
-        constexpr int half = 1 << (LOGN - 1 - lv);
-        const uint64_t* t = opaque_after(tw, x[0]);
-        uint64_t w[8];
-        ld_level<(16 >> sh)>(w, t, (1u << lv) + (((uint32_t)L << 4) >> sh));
-#pragma unroll
-        for (int k = 0; k < kE; ++k) {
-            if (k & half) continue;
-            ct_bfly_mul(x[k], x[k + half], w[k >> sh]);
-        }
-    });
-}
+// Levels 4 .. LOGN - 1 take their twiddles from the table: level lv of pass 2 (lv < 8, layout
+// idx2) has 2^(lv - 4) distinct blocks per lane, level lv of pass 3 (layout idx3) 16 >> (LOGN - lv);
+// butterfly k of the lane uses w[tw_slot(k)] of the level's list.
+template <int LOGN, int LV>
+struct Level {
+    static constexpr int NB = LV < 8 ? (1 << (LV - 4)) : (16 >> (LOGN - LV));
+    static constexpr int HALF = LV < 8 ? (8 >> (LV - 4)) : (1 << (LOGN - 1 - LV));
+    static constexpr int SLOT_SHIFT = LV < 8 ? 8 - LV : LOGN - LV;
+    __device__ static uint32_t first(int L) {
+        if constexpr (LV < 8) return (1u << LV) + (((uint32_t)L >> (LOGN - 8)) << (LV - 4));
+        else return (1u << LV) + (((uint32_t)L << 4) >> (LOGN - LV));
+    }
+};
 
 // Natural-order coefficients (pass-1 layout, lazy or canonical) -> lazy NTT values in
-// pass-3 layout.  FIRST_LV = 1 starts after the first level (a caller that has done it).
+// pass-3 layout.  FIRST_LV = 1 starts after the first level (a caller that has done it).  Each
+// table level's twiddles are requested while the level before it computes (level 4's before the
+// first transpose), so their memory latency overlaps arithmetic.
 template <int LOGN, int FIRST_LV = 0>
 __device__ __forceinline__ void ntt_forward(uint64_t (&x)[kE], uint64_t* lds, const uint64_t* tw, int L) {
     tw = opaque(tw);
     fwd_pass1<LOGN, FIRST_LV>(x);
+    uint64_t w[8];
+    ld_level<Level<LOGN, 4>::NB>(w, opaque_after(tw, x[0]), Level<LOGN, 4>::first(L));
     transpose<LOGN, Layout::kPass1, Layout::kPass2>(x, lds, L);
-    fwd_pass2<LOGN>(x, opaque(tw), L);
-    transpose<LOGN, Layout::kPass2, Layout::kPass3>(x, lds, L);
-    fwd_pass3<LOGN>(x, opaque(tw), L);
+    static_for<LOGN - 4>([&](auto I) __attribute__((always_inline)) {
+        constexpr int lv = 4 + I;
+        using Lv = Level<LOGN, lv>;
+        if constexpr (lv == 8) transpose<LOGN, Layout::kPass2, Layout::kPass3>(x, lds, L);
+        uint64_t wn[8];
+        if constexpr (lv + 1 < LOGN)
+            ld_level<Level<LOGN, lv + 1>::NB>(wn, opaque_after(tw, x[0]), Level<LOGN, lv + 1>::first(L));
+#pragma unroll
+        for (int k = 0; k < kE; ++k) {
+            if (k & Lv::HALF) continue;
+            ct_bfly_mul(x[k], x[k + Lv::HALF], w[k >> Lv::SLOT_SHIFT]);
+        }
+        if constexpr (lv + 1 < LOGN) {
+#pragma unroll
+            for (int b = 0; b < 8; ++b) w[b] = wn[b];
+        }
+    });
 }
 
 // ---------------------------------------------------------------- inverse
-
-template <int LOGN>
-__device__ __forceinline__ void inv_pass3(uint64_t (&x)[kE], const uint64_t* __restrict__ itw, int L) {
-    static_for<LOGN - 8>([&](auto I) __attribute__((always_inline)) {
-        constexpr int lv = LOGN - 1 - I;
-        constexpr int sh = LOGN - lv;
-        constexpr int half = 1 << (LOGN - 1 - lv);
-        const uint64_t* t = opaque_after(itw, x[0]);
-        uint64_t w[8];
-        ld_level<(16 >> sh)>(w, t, (1u << lv) + (((uint32_t)L << 4) >> sh));
-#pragma unroll
-        for (int k = 0; k < kE; ++k) {
-            if (k & half) continue;
-            gs_bfly_mul(x[k], x[k + half], w[k >> sh]);
-        }
-    });
-}
-
-template <int LOGN>
-__device__ __forceinline__ void inv_pass2(uint64_t (&x)[kE], const uint64_t* __restrict__ itw, int L) {
-    constexpr int lb = LOGN - 8;
-    static_for<4>([&](auto I) __attribute__((always_inline)) {
-        constexpr int lv = 7 - I;
-        constexpr int half = 8 >> (lv - 4);
-        const uint64_t* t = opaque_after(itw, x[0]);
-        uint64_t w[8];
-        ld_level<(1 << (lv - 4))>(w, t, (1u << lv) + (((uint32_t)L >> lb) << (lv - 4)));
-#pragma unroll
-        for (int k = 0; k < kE; ++k) {
-            if (k & half) continue;
-            gs_bfly_mul(x[k], x[k + half], w[k >> (8 - lv)]);
-        }
-    });
-}
 
 // Levels m = 8..1 with power-of-two inverse twiddles; N^-1 = 2^(192 - LOGN) is merged into
 // the last level (both of its factors are powers of two).
@@ -325,13 +288,30 @@ __device__ __forceinline__ void inv_pass1(uint64_t (&x)[kE]) {
     }
 }
 
-// Lazy NTT values in pass-3 layout -> lazy natural-order coefficients in pass-1 layout.
+// Lazy NTT values in pass-3 layout -> lazy natural-order coefficients in pass-1 layout (table
+// levels LOGN - 1 .. 4 with the next level's twiddles requested during the current one).
 template <int LOGN>
 __device__ __forceinline__ void ntt_inverse(uint64_t (&x)[kE], uint64_t* lds, const uint64_t* itw, int L) {
     itw = opaque(itw);
-    inv_pass3<LOGN>(x, itw, L);
-    transpose<LOGN, Layout::kPass3, Layout::kPass2>(x, lds, L);
-    inv_pass2<LOGN>(x, opaque(itw), L);
+    uint64_t w[8];
+    ld_level<Level<LOGN, LOGN - 1>::NB>(w, opaque_after(itw, x[0]), Level<LOGN, LOGN - 1>::first(L));
+    static_for<LOGN - 4>([&](auto I) __attribute__((always_inline)) {
+        constexpr int lv = LOGN - 1 - I;
+        using Lv = Level<LOGN, lv>;
+        if constexpr (lv == 7) transpose<LOGN, Layout::kPass3, Layout::kPass2>(x, lds, L);
+        uint64_t wn[8];
+        if constexpr (lv > 4)
+            ld_level<Level<LOGN, lv - 1>::NB>(wn, opaque_after(itw, x[0]), Level<LOGN, lv - 1>::first(L));
+#pragma unroll
+        for (int k = 0; k < kE; ++k) {
+            if (k & Lv::HALF) continue;
+            gs_bfly_mul(x[k], x[k + Lv::HALF], w[k >> Lv::SLOT_SHIFT]);
+        }
+        if constexpr (lv > 4) {
+#pragma unroll
+            for (int b = 0; b < 8; ++b) w[b] = wn[b];
+        }
+    });
     transpose<LOGN, Layout::kPass2, Layout::kPass1>(x, lds, L);
     inv_pass1<LOGN>(x);
 }
