@@ -80,26 +80,38 @@ __device__ __forceinline__ void rlwe_ks_core(uint64_t (&out_a)[kE], uint64_t (&o
 #pragma unroll
             for (int k = 0; k < kE / 2; ++k) ct_bfly_pow2(d[k], d[k + kE / 2], gl::kFwdPow2[1]);
         }
-        ntt_forward<LOGN, 1>(d, lds, A.tw, L);
+        // key pairs of both rows, two at a time: the first two are requested during the transform's
+        // last level, each next two while the current two are used
         const uint64_t* ka = opaque(ksk + (size_t)j * 2 * N);
         const uint64_t* kb = opaque(ksk + (size_t)j * 2 * N + N);
-        // four pairs of both rows at a time (one memory latency per group)
-        static_for<2>([&](auto G) __attribute__((always_inline)) {
-            constexpr int c0 = 4 * G;
-            U64x2 va[4], vb[4];
+        U64x2 va[2], vb[2];
+        auto load_group = [&](U64x2 (&ga)[2], U64x2 (&gb)[2], int c0) __attribute__((always_inline)) {
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                va[q] = ld_pair(ka, (c0 + q) * S::P + L);
-                vb[q] = ld_pair(kb, (c0 + q) * S::P + L);
+            for (int q = 0; q < 2; ++q) {
+                ga[q] = ld_pair(ka, (c0 + q) * S::P + L);
+                gb[q] = ld_pair(kb, (c0 + q) * S::P + L);
             }
             __builtin_amdgcn_sched_barrier(0);
+        };
+        ntt_forward<LOGN, 1>(d, lds, A.tw, L, [&]() __attribute__((always_inline)) { load_group(va, vb, 0); });
+        static_for<4>([&](auto G) __attribute__((always_inline)) {
+            constexpr int c0 = 2 * G;
+            U64x2 na[2], nb[2];
+            if constexpr (G < 3) load_group(na, nb, c0 + 2);
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
+            for (int q = 0; q < 2; ++q) {
                 const int c = c0 + q;
                 out_a[2 * c] = gl::mac_lazy(d[2 * c], va[q].x, out_a[2 * c]);
                 out_a[2 * c + 1] = gl::mac_lazy(d[2 * c + 1], va[q].y, out_a[2 * c + 1]);
                 out_b[2 * c] = gl::mac_lazy(d[2 * c], vb[q].x, out_b[2 * c]);
                 out_b[2 * c + 1] = gl::mac_lazy(d[2 * c + 1], vb[q].y, out_b[2 * c + 1]);
+            }
+            if constexpr (G < 3) {
+#pragma unroll
+                for (int q = 0; q < 2; ++q) {
+                    va[q] = na[q];
+                    vb[q] = nb[q];
+                }
             }
         });
     }

@@ -233,8 +233,15 @@ struct Level {
 // pass-3 layout.  FIRST_LV = 1 starts after the first level (a caller that has done it).  Each
 // table level's twiddles are requested while the level before it computes (level 4's before the
 // first transpose), so their memory latency overlaps arithmetic.
-template <int LOGN, int FIRST_LV = 0>
-__device__ __forceinline__ void ntt_forward(uint64_t (&x)[kE], uint64_t* lds, const uint64_t* tw, int L) {
+struct NoPrefetch {
+    __device__ void operator()() const {}
+};
+
+// `at_last` runs at the start of the last level (a caller's loads for what follows the transform
+// are then in flight during that level).
+template <int LOGN, int FIRST_LV = 0, class AtLast = NoPrefetch>
+__device__ __forceinline__ void ntt_forward(uint64_t (&x)[kE], uint64_t* lds, const uint64_t* tw, int L,
+                                            AtLast&& at_last = AtLast{}) {
     tw = opaque(tw);
     fwd_pass1<LOGN, FIRST_LV>(x);
     uint64_t w[8];
@@ -247,6 +254,8 @@ __device__ __forceinline__ void ntt_forward(uint64_t (&x)[kE], uint64_t* lds, co
         uint64_t wn[8];
         if constexpr (lv + 1 < LOGN)
             ld_level<Level<LOGN, lv + 1>::NB>(wn, opaque_after(tw, x[0]), Level<LOGN, lv + 1>::first(L));
+        else
+            at_last();
 #pragma unroll
         for (int k = 0; k < kE; ++k) {
             if (k & Lv::HALF) continue;
