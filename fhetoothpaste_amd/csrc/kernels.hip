@@ -26,9 +26,10 @@ __global__ void __launch_bounds__(256) ntt_batch_kernel(uint64_t* __restrict__ o
 #pragma unroll
             for (int k = 0; k < kE; ++k) x[k] = src[idx1<LOGN>(L, k)];
             ntt_forward<LOGN>(x, lds[g], tw, L);
+            transpose<LOGN, Layout::kPass3, Layout::kPass1>(x, lds[g], L);  // coalesced writes
             if (live)
 #pragma unroll
-                for (int k = 0; k < kE; ++k) out[p * S::N + idx3(L, k)] = gl::canon(x[k]);
+                for (int k = 0; k < kE; ++k) out[p * S::N + idx1<LOGN>(L, k)] = gl::canon(x[k]);
         } else {
 #pragma unroll
             for (int k = 0; k < kE; ++k) x[k] = src[idx3(L, k)];

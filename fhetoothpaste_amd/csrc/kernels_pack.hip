@@ -160,12 +160,20 @@ automorphism_kernel(KeySwitchArgs A, const uint64_t* in, uint64_t* out, int coun
         for (int k = 0; k < kE; ++k) b[k] = src[N + idx1<LOGN>(L, k)];
         automorphism_regs<LOGN>(b, lds, t, L);
         ntt_forward<LOGN>(b, lds, A.tw, L);
-        poly_sync<S::WAVES>();
+#pragma unroll
+        for (int k = 0; k < kE; ++k) {
+            oa[k] = gl::neg(gl::canon(oa[k]));
+            ob[k] = gl::sub(gl::canon(b[k]), gl::canon(ob[k]));
+        }
+        // coalesced writes (NTT index L + P k); the transposes' syncs also order every read of in
+        // before the writes
+        transpose<LOGN, Layout::kPass3, Layout::kPass1>(oa, lds, L);
+        transpose<LOGN, Layout::kPass3, Layout::kPass1>(ob, lds, L);
         uint64_t* dst = out + (size_t)ct * 2 * N;
 #pragma unroll
         for (int k = 0; k < kE; ++k) {
-            dst[idx3(L, k)] = gl::neg(gl::canon(oa[k]));
-            dst[N + idx3(L, k)] = gl::sub(gl::canon(b[k]), gl::canon(ob[k]));
+            dst[idx1<LOGN>(L, k)] = oa[k];
+            dst[N + idx1<LOGN>(L, k)] = ob[k];
         }
         poly_sync<S::WAVES>();
     }
@@ -208,18 +216,26 @@ rlwe_keyswitch_kernel(KeySwitchArgs A, const uint64_t* in, uint64_t* out, int co
 #pragma unroll
             for (int k = 0; k < kE; ++k) b[k] = 0;
         }
-        poly_sync<S::WAVES>();
+#pragma unroll
+        for (int k = 0; k < kE; ++k) {
+            oa[k] = gl::neg(gl::canon(oa[k]));
+            ob[k] = gl::sub(gl::canon(b[k]), gl::canon(ob[k]));
+        }
+        // coalesced writes (NTT index L + P k); the transposes' syncs also order every read of in
+        // before the writes
+        transpose<LOGN, Layout::kPass3, Layout::kPass1>(oa, lds, L);
+        transpose<LOGN, Layout::kPass3, Layout::kPass1>(ob, lds, L);
         uint64_t* dst = out + (size_t)ct * 2 * N;
 #pragma unroll
         for (int k = 0; k < kE; ++k) {
-            uint64_t ra = gl::neg(gl::canon(oa[k]));
-            uint64_t rb = gl::sub(gl::canon(b[k]), gl::canon(ob[k]));
+            const int q = idx1<LOGN>(L, k);
+            uint64_t ra = oa[k], rb = ob[k];
             if (accumulate) {
-                ra = gl::add(dst[idx3(L, k)], ra);
-                rb = gl::add(dst[N + idx3(L, k)], rb);
+                ra = gl::add(dst[q], ra);
+                rb = gl::add(dst[N + q], rb);
             }
-            dst[idx3(L, k)] = ra;
-            dst[N + idx3(L, k)] = rb;
+            dst[q] = ra;
+            dst[N + q] = rb;
         }
         poly_sync<S::WAVES>();
     }
@@ -271,16 +287,20 @@ auto_ntt_kernel(KeySwitchArgs A, const uint64_t* in, uint64_t* out, int count, u
     for (int ct = blockIdx.x; ct < count; ct += gridDim.x) {
         uint64_t ya[kE], yb[kE];
         auto_ntt_core<LOGN>(ya, yb, in + (size_t)ct * 2 * N, lds, t, A, ksk, L);
+        // coalesced writes (NTT index L + P k)
+        transpose<LOGN, Layout::kPass3, Layout::kPass1>(ya, lds, L);
+        transpose<LOGN, Layout::kPass3, Layout::kPass1>(yb, lds, L);
         uint64_t* dst = out + (size_t)ct * 2 * N;
 #pragma unroll
         for (int k = 0; k < kE; ++k) {
+            const int q = idx1<LOGN>(L, k);
             uint64_t ra = ya[k], rb = yb[k];
             if (accumulate) {
-                ra = gl::add(dst[idx3(L, k)], ra);
-                rb = gl::add(dst[N + idx3(L, k)], rb);
+                ra = gl::add(dst[q], ra);
+                rb = gl::add(dst[N + q], rb);
             }
-            dst[idx3(L, k)] = ra;
-            dst[N + idx3(L, k)] = rb;
+            dst[q] = ra;
+            dst[N + q] = rb;
         }
         poly_sync<S::WAVES>();
     }
@@ -305,12 +325,16 @@ sap_tree_kernel(KeySwitchArgs A, const uint64_t* X, uint64_t* out, const uint64_
         const uint64_t* x = X + (size_t)r * 2 * N;
         uint64_t ya[kE], yb[kE];
         auto_ntt_core<LOGN>(ya, yb, x, lds, t, A, ksk, L);
+        // to the lane-contiguous layout (NTT index L + P k), so that the node arithmetic's reads of
+        // x and the polynomials and its writes are coalesced
+        transpose<LOGN, Layout::kPass3, Layout::kPass1>(ya, lds, L);
+        transpose<LOGN, Layout::kPass3, Layout::kPass1>(yb, lds, L);
         if (mode == 0) {
             uint64_t* ev = out + ((size_t)(2 * e) * count + i) * 2 * N;
             uint64_t* od = out + ((size_t)(2 * e + 1) * count + i) * 2 * N;
 #pragma unroll
             for (int k = 0; k < kE; ++k) {
-                const int q = idx3(L, k);
+                const int q = idx1<LOGN>(L, k);
                 const uint64_t xa = x[q], xb = x[N + q], sh = polys0[q];
                 ev[q] = gl::add(xa, ya[k]);
                 ev[N + q] = gl::add(xb, yb[k]);
@@ -323,7 +347,7 @@ sap_tree_kernel(KeySwitchArgs A, const uint64_t* X, uint64_t* out, const uint64_
             const uint64_t* mo = polys1 + (size_t)e * N;
 #pragma unroll
             for (int k = 0; k < kE; ++k) {
-                const int q = idx3(L, k);
+                const int q = idx1<LOGN>(L, k);
                 const uint64_t xa = x[q], xb = x[N + q], m0 = me[q], m1 = mo[q];
                 o[q] = gl::add(gl::mul(gl::add(xa, ya[k]), m0), gl::mul(gl::sub(xa, ya[k]), m1));
                 o[N + q] = gl::add(gl::mul(gl::add(xb, yb[k]), m0), gl::mul(gl::sub(xb, yb[k]), m1));
@@ -376,8 +400,9 @@ __global__ void __launch_bounds__(Shape<LOGN>::P) trace_kernel(KeySwitchArgs A, 
 #pragma unroll
             for (int k = 0; k < kE; ++k) y[k] = h ? o[1][k] : o[0][k];
             ntt_forward<LOGN>(y, lds, A.tw, L);
+            transpose<LOGN, Layout::kPass3, Layout::kPass1>(y, lds, L);  // coalesced writes
 #pragma unroll
-            for (int k = 0; k < kE; ++k) dst[h * N + idx3(L, k)] = gl::canon(y[k]);
+            for (int k = 0; k < kE; ++k) dst[h * N + idx1<LOGN>(L, k)] = gl::canon(y[k]);
         }
         poly_sync<S::WAVES>();
     }
