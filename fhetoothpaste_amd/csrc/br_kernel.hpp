@@ -223,24 +223,56 @@ br_kernel(BrArgs A) {
                             }
                         });
                     } else {
-                        ntt_forward<LOGN, 1>(d, buf, A.tw, L);
-                        // combined key sum_c w_c RGSW_c[row], formed pair by pair next to its use
+                        // combined key sum_c w_c RGSW_c[row], formed pair by pair next to its use; the
+                        // monomial and key pairs of pair c + 1 are requested while pair c is used (the
+                        // first ones during the transform's last level)
                         const uint64_t* krow = opaque(key + row);
                         const uint64_t* mono = opaque_after(A.mono, d[0]);  // weights re-read per digit
+                        struct PairLoads {
+                            U64x2 m[3];      // NTT(X^e) at e0, e1, e2 (raw table rows)
+                            U64x2 k[NC][2];  // key pairs of RGSW_c, output halves 0 and 1
+                        };
+                        auto load_pair = [&](PairLoads& pl, int c) __attribute__((always_inline)) {
+                            const uint32_t es[3] = {e0, e1, e2};
+#pragma unroll
+                            for (int q = 0; q < (NC == 3 ? 3 : 2); ++q)
+                                pl.m[q] = ld_pair(mono + (size_t)(es[q] & (N - 1)) * N, c * S::P + L);
+#pragma unroll
+                            for (int r = 0; r < NC; ++r)
+#pragma unroll
+                                for (int p = 0; p < 2; ++p)
+                                    pl.k[r][p] = ld_pair(krow + r * rgsw_words + p * N, c * S::P + L);
+                            __builtin_amdgcn_sched_barrier(0);
+                        };
+                        // NTT(X^e) for e in [0, 2N): X^e = -X^(e-N) for e >= N
+                        auto mono_val = [&](const U64x2& v, uint32_t e) __attribute__((always_inline)) {
+                            return e >= (uint32_t)N ? U64x2{gl::neg(v.x), gl::neg(v.y)} : v;
+                        };
+                        PairLoads cur;
+                        ntt_forward<LOGN, 1>(d, buf, A.tw, L, [&]() __attribute__((always_inline)) {
+                            load_pair(cur, 0);
+                        });
                         static_for<kE / 2>([&](auto C) __attribute__((always_inline)) {
                             constexpr int c = C;
-                            const U64x2 w0 = weight_pair<LOGN>(mono, e0, c, L);
-                            const U64x2 w1 = weight_pair<LOGN>(mono, e1, c, L);
+                            PairLoads nxt;
+                            if constexpr (c + 1 < kE / 2) load_pair(nxt, c + 1);
+                            const U64x2 m0 = mono_val(cur.m[0], e0), m1 = mono_val(cur.m[1], e1);
+                            const U64x2 w0{gl::sub(m0.x, 1), gl::sub(m0.y, 1)};  // X^e0 - 1
+                            const U64x2 w1{gl::sub(m1.x, 1), gl::sub(m1.y, 1)};  // X^e1 - 1
                             U64x2 w2{0, 0};
-                            if constexpr (NC == 3) w2 = bmmp_w2_pair<LOGN>(mono, e0, e1, e2, c, L);
+                            if constexpr (NC == 3) {
+                                // X^(e0+e1) - X^e0 - X^e1 + 1 (bmmp_blind_rotation.cpp:280-323)
+                                const U64x2 m2 = mono_val(cur.m[2], e2);
+                                w2 = U64x2{gl::add(gl::sub(gl::sub(m2.x, m0.x), m1.x), 1),
+                                           gl::add(gl::sub(gl::sub(m2.y, m0.y), m1.y), 1)};
+                            }
 #pragma unroll
                             for (int p = 0; p < 2; ++p) {
-                                const U64x2 a0 = ld_pair(krow + p * N, c * S::P + L);
-                                const U64x2 a1 = ld_pair(krow + rgsw_words + p * N, c * S::P + L);
+                                const U64x2 a0 = cur.k[0][p], a1 = cur.k[1][p];
                                 uint64_t cx = gl::mac_lazy(a1.x, w1.x, gl::mul_lazy(a0.x, w0.x));
                                 uint64_t cy = gl::mac_lazy(a1.y, w1.y, gl::mul_lazy(a0.y, w0.y));
                                 if constexpr (NC == 3) {
-                                    const U64x2 a2 = ld_pair(krow + 2 * rgsw_words + p * N, c * S::P + L);
+                                    const U64x2 a2 = cur.k[NC - 1][p];
                                     cx = gl::mac_lazy(a2.x, w2.x, cx);
                                     cy = gl::mac_lazy(a2.y, w2.y, cy);
                                 }
@@ -252,6 +284,7 @@ br_kernel(BrArgs A) {
                                     out[1][2 * c + 1] = gl::mac_lazy(d[2 * c + 1], cy, out[1][2 * c + 1]);
                                 }
                             }
+                            if constexpr (c + 1 < kE / 2) cur = nxt;
                         });
                     }
                 }
