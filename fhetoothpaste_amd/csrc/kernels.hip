@@ -50,7 +50,13 @@ __global__ void __launch_bounds__(256) ntt_batch_kernel(uint64_t* __restrict__ o
 // GEMM layout: one ciphertext per lane (its digits are per-lane work), a tile of kKsCols key
 // columns per block; the key row segment K[i][j][col0 .. col0 + kKsCols) is wave-uniform and is
 // read through the scalar cache, so each MAC is two v_mad_u64_u32 with a scalar operand.
-constexpr int kKsCols = 16;
+// 8 columns: twice the resident waves of 16 to cover the scalar key loads' latency (4.5 -> 4.3 ms
+// at 4096 ciphertexts, N = 1024, d = 14); 4 columns spend more on the loads than they hide (7.9 ms)
+constexpr int kKsCols = 8;
+static_assert(kKsCols <= kKsPadWords, "a partial last tile reads into the zero padding");
+// tile width of the row-split kernel below (its threads walk rows, not ciphertexts)
+constexpr int kKsSplitCols = 16;
+static_assert(kKsSplitCols <= kKsPadWords, "a partial last tile reads into the zero padding");
 constexpr int kKsBlock = 256;
 // Launches of at most this many ciphertexts use the row-split kernel below: the batched kernel's
 // time is one lane walking all N d key rows, the same for 1 and for 256 ciphertexts (about 5 ms
@@ -101,23 +107,23 @@ __global__ void __launch_bounds__(kKsBlock) lwe_keyswitch_kernel(KsArgs A) {
 }
 
 // Few ciphertexts: the same sums, split over the key rows instead of over ciphertexts.  Block
-// (x, y) covers one tile of kKsCols columns of ciphertext y; thread t takes rows t, t + kKsBlock,
+// (x, y) covers one tile of kKsSplitCols columns of ciphertext y; thread t takes rows t, t + kKsBlock,
 // ... of the N d key rows (each a 16-word, 128-byte segment), and the partial lo / hi sums, exact
 // in 64 bits like the totals they add up to, meet in LDS.  Each ciphertext reads the key once
 // (the ciphertexts of a launch share it through the caches), instead of one lane per ciphertext
 // walking all N d rows.
 __global__ void __launch_bounds__(kKsBlock) lwe_keyswitch_one_kernel(KsArgs A) {
-    __shared__ uint64_t part[2][kKsCols][kKsBlock + 1];
-    const int col0 = blockIdx.x * kKsCols;
+    __shared__ uint64_t part[2][kKsSplitCols][kKsBlock + 1];
+    const int col0 = blockIdx.x * kKsSplitCols;
     const int t = threadIdx.x;
     const int W = A.n_out + 1;
     const int ct = A.rows ? (int)A.rows[blockIdx.y] : (int)blockIdx.y;
     const uint64_t* in = A.in + (size_t)ct * A.in_stride;
     uint64_t* out = A.out + (size_t)blockIdx.y * W;
     const uint64_t mask = (1ULL << A.base_log2) - 1;
-    uint64_t lo[kKsCols], hi[kKsCols];
+    uint64_t lo[kKsSplitCols], hi[kKsSplitCols];
 #pragma unroll
-    for (int c = 0; c < kKsCols; ++c) lo[c] = hi[c] = 0;
+    for (int c = 0; c < kKsSplitCols; ++c) lo[c] = hi[c] = 0;
     const int rows = A.n_in * A.digits;
     for (int r = t; r < rows; r += kKsBlock) {
         const int i = r / A.digits, j = r - i * A.digits;
@@ -125,27 +131,27 @@ __global__ void __launch_bounds__(kKsBlock) lwe_keyswitch_one_kernel(KsArgs A) {
         const uint32_t u = (uint32_t)((x >> (64 - (j + 1) * A.base_log2)) & mask);
         const uint64_t* kr = A.ksk + (size_t)r * W + col0;
 #pragma unroll
-        for (int c = 0; c < kKsCols; ++c) {
+        for (int c = 0; c < kKsSplitCols; ++c) {
             const uint64_t kv = kr[c];
             lo[c] += (uint64_t)u * (uint32_t)kv;
             hi[c] += (uint64_t)u * (uint32_t)(kv >> 32);
         }
     }
 #pragma unroll
-    for (int c = 0; c < kKsCols; ++c) {
+    for (int c = 0; c < kKsSplitCols; ++c) {
         part[0][c][t] = lo[c];
         part[1][c][t] = hi[c];
     }
     __syncthreads();
-    // thread (h, c) for h < 2, c < kKsCols adds column c's partial sums of half h
-    if (t < 2 * kKsCols) {
-        const int h = t / kKsCols, c = t % kKsCols;
+    // thread (h, c) for h < 2, c < kKsSplitCols adds column c's partial sums of half h
+    if (t < 2 * kKsSplitCols) {
+        const int h = t / kKsSplitCols, c = t % kKsSplitCols;
         uint64_t sum = 0;
         for (int q = 0; q < kKsBlock; ++q) sum += part[h][c][q];
         part[h][c][kKsBlock] = sum;
     }
     __syncthreads();
-    if (t < kKsCols) {
+    if (t < kKsSplitCols) {
         const int col = col0 + t;
         if (col >= W) return;
         const uint64_t l = part[0][t][kKsBlock], h = part[1][t][kKsBlock];
@@ -200,7 +206,7 @@ hipError_t launch_blind_rotate(int logN, int method, int digits, const BrArgs& A
 
 hipError_t launch_lwe_keyswitch(const KsArgs& A, hipStream_t s) {
     if (A.count <= kKsRowSplitMax) {
-        hipLaunchKernelGGL(lwe_keyswitch_one_kernel, dim3((A.n_out + 1 + kKsCols - 1) / kKsCols, A.count),
+        hipLaunchKernelGGL(lwe_keyswitch_one_kernel, dim3((A.n_out + 1 + kKsSplitCols - 1) / kKsSplitCols, A.count),
                            dim3(kKsBlock), 0, s, A);
         return hipGetLastError();
     }
