@@ -24,32 +24,6 @@ __device__ __forceinline__ void load_key_row(uint64_t (&kv)[kE], const uint64_t*
     }
 }
 
-// NTT(X^e) at the lane's NTT positions (2c, 2c+1), e in [0, 2N): the table holds NTT(X^e)
-// for e < N in key-row order; X^e = -X^(e-N) for e >= N.
-template <int LOGN>
-__device__ __forceinline__ U64x2 mono_pair(const uint64_t* __restrict__ mono, uint32_t e, int c, int L) {
-    constexpr int N = Shape<LOGN>::N;
-    const U64x2 v = ld_pair(mono + (size_t)(e & (N - 1)) * N, c * Shape<LOGN>::P + L);
-    if (e >= (uint32_t)N) return U64x2{gl::neg(v.x), gl::neg(v.y)};
-    return v;
-}
-
-// NTT(X^e) - 1
-template <int LOGN>
-__device__ __forceinline__ U64x2 weight_pair(const uint64_t* __restrict__ mono, uint32_t e, int c, int L) {
-    const U64x2 v = mono_pair<LOGN>(mono, e, c, L);
-    return U64x2{gl::sub(v.x, 1), gl::sub(v.y, 1)};
-}
-
-// BMMP third weight X^(a0+a1) - X^a0 - X^a1 + 1 (bmmp_blind_rotation.cpp:280-323).
-template <int LOGN>
-__device__ __forceinline__ U64x2 bmmp_w2_pair(const uint64_t* __restrict__ mono, uint32_t e0, uint32_t e1,
-                                              uint32_t e2, int c, int L) {
-    const U64x2 m2 = mono_pair<LOGN>(mono, e2, c, L), m0 = mono_pair<LOGN>(mono, e0, c, L),
-                m1 = mono_pair<LOGN>(mono, e1, c, L);
-    return U64x2{gl::add(gl::sub(gl::sub(m2.x, m0.x), m1.x), 1), gl::add(gl::sub(gl::sub(m2.y, m0.y), m1.y), 1)};
-}
-
 // ------------------------------------------------------------------ blind rotation
 
 // acc_h[k] = coefficient idx1(L,k) of X^e * src(X) (negacyclic), e in [0, 2N).

@@ -113,42 +113,97 @@ __global__ void __launch_bounds__(2 * LD * 64) br_latency_kernel(BrArgs A) {
                     for (int k = 0; k < kE / 2; ++k) ct_bfly_pow2(d[k], d[k + kE / 2], gl::kFwdPow2[1]);
                 }
             }
-            ntt_forward<LOGN, 1>(d, tb, A.tw, L);
             uint64_t out[2][kE];
             const size_t row = (size_t)(h * LD + j) * 2 * N;
             if constexpr (METHOD == 0) {
+                // key pairs of both rows, two at a time: the first two requested during the
+                // transform's last level, each next two while the current two are used
+                const uint64_t* ka = key + row;
+                const uint64_t* kb = key + row + N;
+                U64x2 va[2], vb[2];
+                auto load_group = [&](U64x2 (&ga)[2], U64x2 (&gb)[2], int c0) __attribute__((always_inline)) {
 #pragma unroll
-                for (int c = 0; c < kE / 2; ++c) {
-                    const U64x2 va = ld_pair(key + row, c * S::P + L), vb = ld_pair(key + row + N, c * S::P + L);
-                    out[0][2 * c] = gl::mul_lazy(d[2 * c], va.x);
-                    out[0][2 * c + 1] = gl::mul_lazy(d[2 * c + 1], va.y);
-                    out[1][2 * c] = gl::mul_lazy(d[2 * c], vb.x);
-                    out[1][2 * c + 1] = gl::mul_lazy(d[2 * c + 1], vb.y);
-                }
+                    for (int q = 0; q < 2; ++q) {
+                        ga[q] = ld_pair(ka, (c0 + q) * S::P + L);
+                        gb[q] = ld_pair(kb, (c0 + q) * S::P + L);
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
+                };
+                ntt_forward<LOGN, 1>(d, tb, A.tw, L, [&]() __attribute__((always_inline)) { load_group(va, vb, 0); });
+                static_for<4>([&](auto G) __attribute__((always_inline)) {
+                    constexpr int c0 = 2 * G;
+                    U64x2 na[2], nb[2];
+                    if constexpr (G < 3) load_group(na, nb, c0 + 2);
+#pragma unroll
+                    for (int q = 0; q < 2; ++q) {
+                        const int c = c0 + q;
+                        out[0][2 * c] = gl::mul_lazy(d[2 * c], va[q].x);
+                        out[0][2 * c + 1] = gl::mul_lazy(d[2 * c + 1], va[q].y);
+                        out[1][2 * c] = gl::mul_lazy(d[2 * c], vb[q].x);
+                        out[1][2 * c + 1] = gl::mul_lazy(d[2 * c + 1], vb[q].y);
+                    }
+                    if constexpr (G < 3) {
+#pragma unroll
+                        for (int q = 0; q < 2; ++q) {
+                            va[q] = na[q];
+                            vb[q] = nb[q];
+                        }
+                    }
+                });
             } else {
+                // combined key, pair by pair; the monomial rows and key pairs of pair c + 1 are
+                // requested while pair c is used (the first during the transform's last level)
                 const uint64_t* krow = key + row;
                 const uint64_t* mono = A.mono;
+                struct PairLoads {
+                    U64x2 m[3];
+                    U64x2 k[NC][2];
+                };
+                auto load_pair = [&](PairLoads& pl, int c) __attribute__((always_inline)) {
+                    const uint32_t es[3] = {e0, e1, e2};
 #pragma unroll
-                for (int c = 0; c < kE / 2; ++c) {
-                    const U64x2 w0 = weight_pair<LOGN>(mono, e0, c, L);
-                    const U64x2 w1 = weight_pair<LOGN>(mono, e1, c, L);
+                    for (int q = 0; q < (NC == 3 ? 3 : 2); ++q)
+                        pl.m[q] = ld_pair(mono + (size_t)(es[q] & (N - 1)) * N, c * S::P + L);
+#pragma unroll
+                    for (int r = 0; r < NC; ++r)
+#pragma unroll
+                        for (int p = 0; p < 2; ++p) pl.k[r][p] = ld_pair(krow + r * rgsw_words + p * N, c * S::P + L);
+                    __builtin_amdgcn_sched_barrier(0);
+                };
+                auto mono_val = [&](const U64x2& v, uint32_t e) __attribute__((always_inline)) {
+                    return e >= (uint32_t)N ? U64x2{gl::neg(v.x), gl::neg(v.y)} : v;
+                };
+                PairLoads cur;
+                ntt_forward<LOGN, 1>(d, tb, A.tw, L, [&]() __attribute__((always_inline)) { load_pair(cur, 0); });
+                static_for<kE / 2>([&](auto C) __attribute__((always_inline)) {
+                    constexpr int c = C;
+                    PairLoads nxt;
+                    if constexpr (c + 1 < kE / 2) load_pair(nxt, c + 1);
+                    const U64x2 m0 = mono_val(cur.m[0], e0), m1 = mono_val(cur.m[1], e1);
+                    const U64x2 w0{gl::sub(m0.x, 1), gl::sub(m0.y, 1)};  // X^e0 - 1
+                    const U64x2 w1{gl::sub(m1.x, 1), gl::sub(m1.y, 1)};  // X^e1 - 1
                     U64x2 w2{0, 0};
-                    if constexpr (NC == 3) w2 = bmmp_w2_pair<LOGN>(mono, e0, e1, e2, c, L);
+                    if constexpr (NC == 3) {
+                        // X^(e0+e1) - X^e0 - X^e1 + 1 (bmmp_blind_rotation.cpp:280-323)
+                        const U64x2 m2 = mono_val(cur.m[2], e2);
+                        w2 = U64x2{gl::add(gl::sub(gl::sub(m2.x, m0.x), m1.x), 1),
+                                   gl::add(gl::sub(gl::sub(m2.y, m0.y), m1.y), 1)};
+                    }
 #pragma unroll
                     for (int p = 0; p < 2; ++p) {
-                        const U64x2 a0 = ld_pair(krow + p * N, c * S::P + L);
-                        const U64x2 a1 = ld_pair(krow + rgsw_words + p * N, c * S::P + L);
+                        const U64x2 a0 = cur.k[0][p], a1 = cur.k[1][p];
                         uint64_t cx = gl::mac_lazy(a1.x, w1.x, gl::mul_lazy(a0.x, w0.x));
                         uint64_t cy = gl::mac_lazy(a1.y, w1.y, gl::mul_lazy(a0.y, w0.y));
                         if constexpr (NC == 3) {
-                            const U64x2 a2 = ld_pair(krow + 2 * rgsw_words + p * N, c * S::P + L);
+                            const U64x2 a2 = cur.k[NC - 1][p];
                             cx = gl::mac_lazy(a2.x, w2.x, cx);
                             cy = gl::mac_lazy(a2.y, w2.y, cy);
                         }
                         out[p][2 * c] = gl::mul_lazy(d[2 * c], cx);
                         out[p][2 * c + 1] = gl::mul_lazy(d[2 * c + 1], cy);
                     }
-                }
+                    if constexpr (c + 1 < kE / 2) cur = nxt;
+                });
             }
             // the wave's transpose buffer is done with (its own barrier inside the transform)
 #pragma unroll
