@@ -47,13 +47,15 @@ __global__ void __launch_bounds__(256) ntt_batch_kernel(uint64_t* __restrict__ o
 // (lwe_conversion.cpp:205-239).  With unsigned digits u_ij = d_ij + delta_j the sum is
 // sum u_ij K - corr[k], corr[k] = sum_ij delta_j K[i][j][k] precomputed per key.  u_ij < 2^b, so
 // u K_lo and u K_hi are accumulated exactly in 64 bits and reduced once at the end.
-// GEMM layout: one ciphertext per lane (its digits are per-lane work), a tile of kKsCols key
-// columns per block; the key row segment K[i][j][col0 .. col0 + kKsCols) is wave-uniform and is
+// GEMM layout: one ciphertext per lane (its digits are per-lane work), a tile of COLS key
+// columns per block; the key row segment K[i][j][col0 .. col0 + COLS) is wave-uniform and is
 // read through the scalar cache, so each MAC is two v_mad_u64_u32 with a scalar operand.
-// 8 columns: twice the resident waves of 16 to cover the scalar key loads' latency (4.5 -> 4.3 ms
-// at 4096 ciphertexts, N = 1024, d = 14); 4 columns spend more on the loads than they hide (7.9 ms)
-constexpr int kKsCols = 8;
-static_assert(kKsCols <= kKsPadWords, "a partial last tile reads into the zero padding");
+// Tile width: 8 columns give twice the resident waves of 16 to cover the scalar key loads' latency
+// (4.5 -> 4.3 ms at 4096 ciphertexts, N = 1024, d = 14), but once the grid holds more waves than a
+// SIMD keeps resident the extra loads cost more than they hide (5180 ciphertexts at N = 2048,
+// d = 12: 11.4 ms at 16, 18.3 ms at 8); the launcher picks by waves per SIMD.  4 columns: 7.9 ms.
+constexpr int kKsColsMax = 16;
+static_assert(kKsColsMax <= kKsPadWords, "a partial last tile reads into the zero padding");
 // tile width of the row-split kernel below (its threads walk rows, not ciphertexts)
 constexpr int kKsSplitCols = 16;
 static_assert(kKsSplitCols <= kKsPadWords, "a partial last tile reads into the zero padding");
@@ -63,18 +65,20 @@ constexpr int kKsBlock = 256;
 // at N = 1024, d = 14), while the row-split kernel reads the key once per ciphertext (about 60 us).
 constexpr int kKsRowSplitMax = 64;
 
+template <int COLS>
 __global__ void __launch_bounds__(kKsBlock) lwe_keyswitch_kernel(KsArgs A) {
-    const int col0 = blockIdx.x * kKsCols;
+    static_assert(COLS <= kKsColsMax, "a partial last tile reads into the zero padding");
+    const int col0 = blockIdx.x * COLS;
     const int c = blockIdx.y * kKsBlock + threadIdx.x;
     const int W = A.n_out + 1;
     const bool live = c < A.count;
     const int ct = live ? (A.rows ? (int)A.rows[c] : c) : 0;
     const uint64_t* in = A.in + (size_t)ct * A.in_stride;
-    uint64_t lo[kKsCols], hi[kKsCols];
+    uint64_t lo[COLS], hi[COLS];
 #pragma unroll
-    for (int t = 0; t < kKsCols; ++t) lo[t] = hi[t] = 0;
+    for (int t = 0; t < COLS; ++t) lo[t] = hi[t] = 0;
     const uint64_t mask = (1ULL << A.base_log2) - 1;
-    // the device key carries kKsCols words of zero padding, so a partial last tile reads in
+    // the device key carries kKsColsMax words of zero padding, so a partial last tile reads in
     // bounds; its extra columns are discarded
     const uint64_t* krow = A.ksk + col0;
     for (int i = 0; i < A.n_in; ++i) {
@@ -83,7 +87,7 @@ __global__ void __launch_bounds__(kKsBlock) lwe_keyswitch_kernel(KsArgs A) {
         for (int j = 0; j < A.digits; ++j, kr += W) {
             const uint32_t u = (uint32_t)((x >> (64 - (j + 1) * A.base_log2)) & mask);
 #pragma unroll
-            for (int t = 0; t < kKsCols; ++t) {
+            for (int t = 0; t < COLS; ++t) {
                 const uint64_t kv = kr[t];
                 lo[t] += (uint64_t)u * (uint32_t)kv;
                 hi[t] += (uint64_t)u * (uint32_t)(kv >> 32);
@@ -92,7 +96,7 @@ __global__ void __launch_bounds__(kKsBlock) lwe_keyswitch_kernel(KsArgs A) {
     }
     if (!live) return;
 #pragma unroll
-    for (int t = 0; t < kKsCols; ++t) {
+    for (int t = 0; t < COLS; ++t) {
         const int col = col0 + t;
         if (col >= W) break;
         // total = hi * 2^32 + lo, both < 2^62
@@ -210,8 +214,23 @@ hipError_t launch_lwe_keyswitch(const KsArgs& A, hipStream_t s) {
                            dim3(kKsBlock), 0, s, A);
         return hipGetLastError();
     }
-    dim3 grid((A.n_out + 1 + kKsCols - 1) / kKsCols, (A.count + kKsBlock - 1) / kKsBlock);
-    hipLaunchKernelGGL(lwe_keyswitch_kernel, grid, dim3(kKsBlock), 0, s, A);
+    static int simds = 0;  // SIMDs of the device (4 per CU), read once
+    if (simds == 0) {
+        int dev = 0, cus = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+        simds = 4 * cus;
+    }
+    const int blocks_y = (A.count + kKsBlock - 1) / kKsBlock;
+    const int tiles8 = (A.n_out + 1 + 7) / 8;
+    const long long waves8 = (long long)blocks_y * tiles8 * (kKsBlock / 64);
+    if (waves8 <= 5LL * simds) {
+        hipLaunchKernelGGL(lwe_keyswitch_kernel<8>, dim3(tiles8, blocks_y), dim3(kKsBlock), 0, s, A);
+    } else {
+        hipLaunchKernelGGL(lwe_keyswitch_kernel<16>, dim3((A.n_out + 1 + 15) / 16, blocks_y), dim3(kKsBlock), 0, s,
+                           A);
+    }
     return hipGetLastError();
 }
 
