@@ -114,11 +114,22 @@ br_kernel(BrArgs A) {
 #pragma unroll 1
             for (int h = 0; h < 2; ++h) {
                 if constexpr (METHOD == 0) {
-                    // the half being decomposed is parked in LDS so that every digit can re-read
-                    // its rotation X^e0 acc_h - acc_h (kept out of registers)
+                    // the polynomial the digits are taken from, X^e0 acc_h - acc_h, is formed once
+                    // per half (through the LDS copy of acc_h for the rotation) and parked in LDS,
+                    // where every digit re-reads it (kept out of registers)
                     poly_sync<S::WAVES>();
 #pragma unroll
                     for (int k = 0; k < kE; ++k) src[phys(idx1<LOGN>(L, k))] = acc[0][k];
+                    poly_sync<S::WAVES>();
+                    uint64_t t[kE];
+                    int Lh = L;  // re-read per half: the rotation's addresses are formed here
+                    asm volatile("" : "+v"(Lh));
+                    rotate_from<LOGN>(t, src, e0, Lh, true);
+#pragma unroll
+                    for (int k = 0; k < kE; ++k) t[k] = gl::sub(t[k], acc[0][k]);
+                    poly_sync<S::WAVES>();
+#pragma unroll
+                    for (int k = 0; k < kE; ++k) src[phys(idx1<LOGN>(L, k))] = t[k];
                     poly_sync<S::WAVES>();
                 }
 #pragma unroll 1
@@ -126,9 +137,12 @@ br_kernel(BrArgs A) {
                     uint64_t d[kE];
                     if constexpr (METHOD == 0) {
                         const int shift = A.first_shift - j * A.base_log2;
-                        rotate_from<LOGN>(d, src, e0, L, true);
+                        // the lane index re-read per digit: its LDS addresses are formed here, not
+                        // held in registers across the digit loop
+                        int Lj = L;
+                        asm volatile("" : "+v"(Lj));
 #pragma unroll
-                        for (int k = 0; k < kE; ++k) d[k] = (gl::sub(d[k], acc[0][k]) >> shift) & mask;
+                        for (int k = 0; k < kE; ++k) d[k] = (src[phys(idx1<LOGN>(Lj, k))] >> shift) & mask;
                         // the first transform level on the digits themselves: cheap while they are small
                         if (A.base_log2 <= 16) {
 #pragma unroll
