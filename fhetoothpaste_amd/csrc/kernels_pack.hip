@@ -146,8 +146,11 @@ automorphism_kernel(KeySwitchArgs A, const uint64_t* in, uint64_t* out, int coun
     using S = Shape<LOGN>;
     constexpr int N = S::N;
     __shared__ uint64_t lds[S::LDS_WORDS];
-    const int L = threadIdx.x;
     for (int ct = blockIdx.x; ct < count; ct += gridDim.x) {
+        // the lane index re-read per RLWE: its addresses are formed inside the loop, not held in
+        // registers across it
+        int L = threadIdx.x;
+        asm volatile("" : "+v"(L));
         const uint64_t* src = in + (size_t)ct * 2 * N;
         uint64_t a[kE];
 #pragma unroll
@@ -194,8 +197,11 @@ rlwe_keyswitch_kernel(KeySwitchArgs A, const uint64_t* in, uint64_t* out, int co
     using S = Shape<LOGN>;
     constexpr int N = S::N;
     __shared__ uint64_t lds[S::LDS_WORDS];
-    const int L = threadIdx.x;
     for (int ct = blockIdx.x; ct < count; ct += gridDim.x) {
+        // the lane index re-read per RLWE: its addresses are formed inside the loop, not held in
+        // registers across it
+        int L = threadIdx.x;
+        asm volatile("" : "+v"(L));
         const uint64_t* src = in + (size_t)ct * 2 * N;
         uint64_t a[kE];
 #pragma unroll
@@ -283,8 +289,11 @@ auto_ntt_kernel(KeySwitchArgs A, const uint64_t* in, uint64_t* out, int count, u
     using S = Shape<LOGN>;
     constexpr int N = S::N;
     __shared__ uint64_t lds[S::LDS_WORDS];
-    const int L = threadIdx.x;
     for (int ct = blockIdx.x; ct < count; ct += gridDim.x) {
+        // the lane index re-read per RLWE: its addresses are formed inside the loop, not held in
+        // registers across it
+        int L = threadIdx.x;
+        asm volatile("" : "+v"(L));
         uint64_t ya[kE], yb[kE];
         auto_ntt_core<LOGN>(ya, yb, in + (size_t)ct * 2 * N, lds, t, A, ksk, L);
         // coalesced writes (NTT index L + P k)
@@ -318,9 +327,12 @@ sap_tree_kernel(KeySwitchArgs A, const uint64_t* X, uint64_t* out, const uint64_
     using S = Shape<LOGN>;
     constexpr int N = S::N;
     __shared__ uint64_t lds[S::LDS_WORDS];
-    const int L = threadIdx.x;
     const int total = nodes * count;
     for (int r = blockIdx.x; r < total; r += gridDim.x) {
+        // the lane index re-read per RLWE: its addresses are formed inside the loop, not held in
+        // registers across it
+        int L = threadIdx.x;
+        asm volatile("" : "+v"(L));
         const int e = r / count, i = r - e * count;
         const uint64_t* x = X + (size_t)r * 2 * N;
         uint64_t ya[kE], yb[kE];
@@ -366,9 +378,12 @@ __global__ void __launch_bounds__(Shape<LOGN>::P) trace_kernel(KeySwitchArgs A, 
     using S = Shape<LOGN>;
     constexpr int N = S::N;
     __shared__ uint64_t lds[S::LDS_WORDS];
-    const int L = threadIdx.x;
     const size_t key_words = (size_t)A.digits * 2 * N;
     for (int ct = blockIdx.x; ct < count; ct += gridDim.x) {
+        // the lane index re-read per RLWE: its addresses are formed inside the loop, not held in
+        // registers across it
+        int L = threadIdx.x;
+        asm volatile("" : "+v"(L));
         uint64_t o[2][kE];
         const uint64_t* src = in + (size_t)ct * 2 * N;
 #pragma unroll
