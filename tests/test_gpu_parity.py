@@ -135,6 +135,26 @@ def test_keyswitch_batch_vs_oracle(tp, oracle, count):
         assert np.array_equal(got[c], oracle.lwe_keyswitch(lwes[c], ksk, N, n_out, b, d)), c
 
 
+def test_keyswitch_tile_widths_agree(tp, oracle):
+    """A launch large enough for the 16-column tiles (4400 ciphertexts, 1024 -> 630) equals two
+    launches of 2200 on the 8-column tiles, word for word, and sampled rows equal the oracle."""
+    N, n_out, b, d = 1024, 630, 4, 14
+    say("lwe keyswitch tile widths", N, n_out, b, d)
+    src = tp.keygen_secret(N, 0, 12, 2)
+    dst = tp.keygen_secret(n_out, 0, 12, 1)
+    ksk = tp.keygen_lwe_ksk(src, dst, N, n_out, b, d, 3.2, 12)
+    ctx = tp.Context(N, 4, 12, 3, 0, n_out, b, d)
+    ctx.import_lwe_ksk(ksk)
+    count = 4400
+    lwes = np.random.default_rng(13).integers(0, Q, size=(count, N + 1), dtype=np.uint64)
+    lwes[7] = Q - 1
+    whole = ctx.lwe_keyswitch_batch(lwes)
+    halves = np.concatenate([ctx.lwe_keyswitch_batch(lwes[:count // 2]), ctx.lwe_keyswitch_batch(lwes[count // 2:])])
+    assert np.array_equal(whole, halves)
+    for c in (0, 7, 2199, 2200, count - 1):
+        assert np.array_equal(whole[c], oracle.lwe_keyswitch(lwes[c], ksk, N, n_out, b, d)), c
+
+
 @pytest.mark.parametrize("N,n_out,b,d", [(1024, 630, 4, 14), (2048, 800, 5, 12), (1024, 17, 16, 4)])
 def test_keyswitch_single_ciphertext(tp, oracle, N, n_out, b, d):
     """count == 1 runs the row-split kernel (partial sums over key rows meet in LDS)."""
