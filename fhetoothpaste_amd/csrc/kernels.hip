@@ -78,21 +78,72 @@ __global__ void __launch_bounds__(kKsBlock) lwe_keyswitch_kernel(KsArgs A) {
 #pragma unroll
     for (int t = 0; t < COLS; ++t) lo[t] = hi[t] = 0;
     const uint64_t mask = (1ULL << A.base_log2) - 1;
-    // the device key carries kKsColsMax words of zero padding, so a partial last tile reads in
-    // bounds; its extra columns are discarded
+    // The key rows i d + j (a segment of COLS words each) are staged through LDS, a chunk of WC input
+    // words (WC d <= CH rows) at a time, loaded by the whole work-group; the next chunk's key rows and
+    // input words are requested while the current chunk is used, and each wave reads a row as a
+    // broadcast LDS read.  (Read directly, every row waited for its scalar loads.)  The device key
+    // carries kKsColsMax words of zero padding, so a partial last tile reads in bounds; its extra
+    // columns are discarded.
+    constexpr int CH = 64;
+    constexpr int XW = 4;                      // input words per chunk at most
+    constexpr int WPT = CH * COLS / kKsBlock;  // key words each thread stages per chunk
+    static_assert(WPT * kKsBlock == CH * COLS, "chunk divides evenly over the work-group");
+    __shared__ uint64_t kst[2][CH * COLS];
     const uint64_t* krow = A.ksk + col0;
-    for (int i = 0; i < A.n_in; ++i) {
-        const uint64_t x = gl::add(in[i], A.corrector);
-        const uint64_t* kr = krow + (size_t)i * A.digits * W;
-        for (int j = 0; j < A.digits; ++j, kr += W) {
-            const uint32_t u = (uint32_t)((x >> (64 - (j + 1) * A.base_log2)) & mask);
+    const int d = A.digits;
+    const int rows = A.n_in * d;
+    const int wc = d * XW <= CH ? XW : (CH / d > 0 ? CH / d : 1);  // input words per chunk
+    const int nch = (A.n_in + wc - 1) / wc;
+    const int tid = threadIdx.x;
+    uint64_t stage[WPT], xn[XW];
+    auto fetch = [&](int q) __attribute__((always_inline)) {
 #pragma unroll
-            for (int t = 0; t < COLS; ++t) {
-                const uint64_t kv = kr[t];
-                lo[t] += (uint64_t)u * (uint32_t)kv;
-                hi[t] += (uint64_t)u * (uint32_t)(kv >> 32);
+        for (int w = 0; w < WPT; ++w) {
+            const int idx = tid * WPT + w;
+            const int r = q * wc * d + idx / COLS;  // past the last row: the last row again (unused)
+            stage[w] = krow[(size_t)(r < rows ? r : rows - 1) * W + idx % COLS];
+        }
+#pragma unroll
+        for (int w = 0; w < XW; ++w) {
+            const int iw = q * wc + w;
+            xn[w] = in[iw < A.n_in ? iw : A.n_in - 1];
+        }
+    };
+    auto put = [&](int b) __attribute__((always_inline)) {
+#pragma unroll
+        for (int w = 0; w < WPT; ++w) kst[b][tid * WPT + w] = stage[w];
+    };
+    fetch(0);
+    put(0);
+    uint64_t xs[XW];
+#pragma unroll
+    for (int w = 0; w < XW; ++w) xs[w] = xn[w];
+    __syncthreads();
+    for (int q = 0; q < nch; ++q) {
+        if (q + 1 < nch) fetch(q + 1);
+        const uint64_t* ks = kst[q & 1];
+#pragma unroll
+        for (int w = 0; w < XW; ++w) {
+            if (w >= wc || q * wc + w >= A.n_in) break;
+            const uint64_t x = gl::add(xs[w], A.corrector);
+            const uint64_t* kw = ks + w * d * COLS;
+            for (int j = 0; j < d; ++j) {
+                const uint32_t u = (uint32_t)((x >> (64 - (j + 1) * A.base_log2)) & mask);
+#pragma unroll
+                for (int t = 0; t < COLS; ++t) {
+                    const uint64_t kv = kw[j * COLS + t];
+                    lo[t] += (uint64_t)u * (uint32_t)kv;
+                    hi[t] += (uint64_t)u * (uint32_t)(kv >> 32);
+                }
             }
         }
+        // the other buffer was last read in chunk q - 1, before the barrier that ended it
+        if (q + 1 < nch) {
+            put((q + 1) & 1);
+#pragma unroll
+            for (int w = 0; w < XW; ++w) xs[w] = xn[w];
+        }
+        __syncthreads();
     }
     if (!live) return;
 #pragma unroll
