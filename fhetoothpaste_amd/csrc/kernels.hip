@@ -48,12 +48,12 @@ __global__ void __launch_bounds__(256) ntt_batch_kernel(uint64_t* __restrict__ o
 // sum u_ij K - corr[k], corr[k] = sum_ij delta_j K[i][j][k] precomputed per key.  u_ij < 2^b, so
 // u K_lo and u K_hi are accumulated exactly in 64 bits and reduced once at the end.
 // GEMM layout: one ciphertext per lane (its digits are per-lane work), a tile of COLS key
-// columns per block; the key row segment K[i][j][col0 .. col0 + COLS) is wave-uniform and is
-// read through the scalar cache, so each MAC is two v_mad_u64_u32 with a scalar operand.
-// Tile width: 8 columns give twice the resident waves of 16 to cover the scalar key loads' latency
-// (4.5 -> 4.3 ms at 4096 ciphertexts, N = 1024, d = 14), but once the grid holds more waves than a
-// SIMD keeps resident the extra loads cost more than they hide (5180 ciphertexts at N = 2048,
-// d = 12: 11.4 ms at 16, 18.3 ms at 8); the launcher picks by waves per SIMD.  4 columns: 7.9 ms.
+// columns per block; the key row segment K[i][j][col0 .. col0 + COLS) is the same for the whole
+// block, is staged through LDS and read as a broadcast, so each MAC is two v_mad_u64_u32.
+// Tile width: 4 columns while the grid holds at most five waves per SIMD (more waves to hide the
+// row loop's latency: 2048 ciphertexts at N = 1024, d = 14: 1.89 ms at 4, 2.03 ms at 8), else 8
+// (fewer digit extractions per MAC: 4096 ciphertexts 3.18 ms at 8, 3.45 ms at 4; 16 columns was
+// slower at every size measured, 3.88 ms at 4096).
 constexpr int kKsColsMax = 16;
 static_assert(kKsColsMax <= kKsPadWords, "a partial last tile reads into the zero padding");
 // tile width of the row-split kernel below (its threads walk rows, not ciphertexts)
@@ -61,8 +61,9 @@ constexpr int kKsSplitCols = 16;
 static_assert(kKsSplitCols <= kKsPadWords, "a partial last tile reads into the zero padding");
 constexpr int kKsBlock = 256;
 // Launches of at most this many ciphertexts use the row-split kernel below: the batched kernel's
-// time is one lane walking all N d key rows, the same for 1 and for 256 ciphertexts (about 5 ms
-// at N = 1024, d = 14), while the row-split kernel reads the key once per ciphertext (about 60 us).
+// time is one lane walking all N d key rows, the same for 1 and for 256 ciphertexts (about 1.25 ms
+// at N = 1024, d = 14, on 8 columns), while the row-split kernel reads the key once per ciphertext
+// (about 60 us; 0.62 ms for 64, 1.51 ms for 128 against 0.80 ms on 4 columns).
 constexpr int kKsRowSplitMax = 64;
 
 template <int COLS>
@@ -82,7 +83,7 @@ __global__ void __launch_bounds__(kKsBlock) lwe_keyswitch_kernel(KsArgs A) {
     // words (WC d <= CH rows) at a time, loaded by the whole work-group; the next chunk's key rows and
     // input words are requested while the current chunk is used, and each wave reads a row as a
     // broadcast LDS read.  (Read directly, every row waited for its scalar loads.)  The device key
-    // carries kKsColsMax words of zero padding, so a partial last tile reads in bounds; its extra
+    // carries kKsPadWords words of zero padding, so a partial last tile reads in bounds; its extra
     // columns are discarded.
     constexpr int CH = 64;
     constexpr int XW = 4;                      // input words per chunk at most
@@ -274,13 +275,12 @@ hipError_t launch_lwe_keyswitch(const KsArgs& A, hipStream_t s) {
         simds = 4 * cus;
     }
     const int blocks_y = (A.count + kKsBlock - 1) / kKsBlock;
-    const int tiles8 = (A.n_out + 1 + 7) / 8;
-    const long long waves8 = (long long)blocks_y * tiles8 * (kKsBlock / 64);
-    if (waves8 <= 5LL * simds) {
-        hipLaunchKernelGGL(lwe_keyswitch_kernel<8>, dim3(tiles8, blocks_y), dim3(kKsBlock), 0, s, A);
+    const int tiles4 = (A.n_out + 1 + 3) / 4;
+    const long long waves4 = (long long)blocks_y * tiles4 * (kKsBlock / 64);
+    if (waves4 <= 5LL * simds) {
+        hipLaunchKernelGGL(lwe_keyswitch_kernel<4>, dim3(tiles4, blocks_y), dim3(kKsBlock), 0, s, A);
     } else {
-        hipLaunchKernelGGL(lwe_keyswitch_kernel<16>, dim3((A.n_out + 1 + 15) / 16, blocks_y), dim3(kKsBlock), 0, s,
-                           A);
+        hipLaunchKernelGGL(lwe_keyswitch_kernel<8>, dim3((A.n_out + 1 + 7) / 8, blocks_y), dim3(kKsBlock), 0, s, A);
     }
     return hipGetLastError();
 }
