@@ -93,7 +93,8 @@ __global__ void __launch_bounds__(kKsBlock) lwe_keyswitch_kernel(KsArgs A) {
     const uint64_t* krow = A.ksk + col0;
     const int d = A.digits;
     const int rows = A.n_in * d;
-    const int wc = d * XW <= CH ? XW : (CH / d > 0 ? CH / d : 1);  // input words per chunk
+    // input words per chunk; contexts accept d <= ceil(64 / b) <= 64 = CH digits, so wc d <= CH
+    const int wc = d * XW <= CH ? XW : (CH / d > 0 ? CH / d : 1);
     const int nch = (A.n_in + wc - 1) / wc;
     const int tid = threadIdx.x;
     uint64_t stage[WPT], xn[XW];

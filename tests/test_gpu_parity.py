@@ -135,6 +135,27 @@ def test_keyswitch_batch_vs_oracle(tp, oracle, count):
         assert np.array_equal(got[c], oracle.lwe_keyswitch(lwes[c], ksk, N, n_out, b, d)), c
 
 
+@pytest.mark.parametrize("N,n_out,b,d", [(1024, 40, 3, 20), (1024, 33, 2, 32), (1024, 17, 16, 4)],
+                         ids=["partial-last-chunk", "64-row-chunks", "partial-last-tile"])
+@pytest.mark.parametrize("count", [100, 300])
+def test_keyswitch_chunk_shapes(tp, oracle, N, n_out, b, d, count):
+    """The batched kernel stages min(4, 64 // d) input words (their d key rows each) per chunk: d = 20
+    gives 3 words and 1024 = 341 * 3 + 1 leaves a last chunk of one word; d = 32 fills the 64 staged
+    rows exactly; n_out + 1 = 18 ends in a partial 4-column tile.  Every output equals the oracle."""
+    say("lwe keyswitch chunk shapes", N, n_out, b, d, count)
+    src = oracle.secret(N, 0, 21, 2)
+    dst = oracle.secret(n_out, 0, 21, 1)
+    ksk = oracle.lwe_ksk(src, dst, N, n_out, b, d, 3.2, 21)
+    ctx = tp.Context(N, 4, 12, 3, 0, n_out, b, d)
+    ctx.import_lwe_ksk(ksk)
+    lwes = np.random.default_rng(count + d).integers(0, Q, size=(count, N + 1), dtype=np.uint64)
+    lwes[3] = Q - 1
+    lwes[4] = 0
+    got = ctx.lwe_keyswitch_batch(lwes)
+    for c in range(count):
+        assert np.array_equal(got[c], oracle.lwe_keyswitch(lwes[c], ksk, N, n_out, b, d)), c
+
+
 def test_keyswitch_tile_widths_agree(tp, oracle):
     """A launch large enough for the 8-column tiles (4000 ciphertexts, 1024 -> 630, 16 blocks of
     256 lanes) equals two launches of 2000 on the 4-column tiles (8 blocks, the last one partial),
