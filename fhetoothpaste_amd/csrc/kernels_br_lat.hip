@@ -3,11 +3,12 @@
 // launches of at most a few ciphertexts per CU).  br_kernel runs a ciphertext's 2 l digit products
 // one after the other in one wave; here one work-group of 2 l waves runs them side by side: wave
 // (h, j) forms digit j of accumulator half h, its transform and its products with key row h l + j
-// (for ternary and BMMP with the combined key), and writes its two partial products to LDS; waves 0
-// and 1 then add the 2 l partial products of output half 0 and 1, inverse-transform them and update
-// that half of the accumulator, which lives in LDS.  The arithmetic is br_kernel's, exact in Z_Q,
-// so the result is bit-identical.  Work-groups stride over the ciphertexts of the launch with
-// br_kernel's slot, test-vector and chunking rules (BrArgs).
+// (for ternary and BMMP with the combined key), and writes its two partial products to LDS; all
+// waves then add the 2 l partial products, a slice of the words each, and waves 0 and 1
+// inverse-transform the sums of output half 0 and 1 and update that half of the accumulator, which
+// lives in LDS.  The arithmetic is br_kernel's, exact in Z_Q, so the result is bit-identical.
+// Work-groups stride over the ciphertexts of the launch with br_kernel's slot, test-vector and
+// chunking rules (BrArgs).
 #include <hip/hip_runtime.h>
 #include "br_kernel.hpp"
 
@@ -213,7 +214,17 @@ __global__ void __launch_bounds__(2 * LD * 64) br_latency_kernel(BrArgs A) {
             }
             __syncthreads();
 
-            // reduction phase: wave p adds the partial products of output half p and updates it
+            // reduction phase: every wave sums the G partial products of a slice of the 2 N words
+            // into the first wave's region (each word is read and written by one thread only)
+            for (int x = threadIdx.x; x < 2 * N; x += G * 64) {
+                uint64_t s = red[0][x];
+#pragma unroll
+                for (int g = 1; g < G; ++g) s = gl::add_ll(s, red[g][x]);
+                red[0][x] = s;
+            }
+            __syncthreads();
+
+            // wave p inverse-transforms the sum of output half p and updates that half
             if (w < 2) {
                 uint64_t s[kE], prev[kE];
 #pragma unroll
@@ -221,10 +232,6 @@ __global__ void __launch_bounds__(2 * LD * 64) br_latency_kernel(BrArgs A) {
                     s[k] = red[0][w * N + idx3(L, k)];
                     prev[k] = accL[w][phys(idx1<LOGN>(L, k))];
                 }
-#pragma unroll
-                for (int g = 1; g < G; ++g)
-#pragma unroll
-                    for (int k = 0; k < kE; ++k) s[k] = gl::add_ll(s[k], red[g][w * N + idx3(L, k)]);
                 ntt_inverse<LOGN>(s, accL[w], A.itw, L);
                 poly_sync<S::WAVES>();
 #pragma unroll
