@@ -267,14 +267,12 @@ hipError_t launch_lwe_keyswitch(const KsArgs& A, hipStream_t s) {
                            dim3(kKsBlock), 0, s, A);
         return hipGetLastError();
     }
-    static int simds = 0;  // SIMDs of the device (4 per CU), read once
-    if (simds == 0) {
-        int dev = 0, cus = 0;
-        if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
-            cus = 256;
-        simds = 4 * cus;
-    }
+    // SIMDs of the current device (4 per CU); the runtime answers from its cached device properties
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+        cus = 256;
+    const int simds = 4 * cus;
     const int blocks_y = (A.count + kKsBlock - 1) / kKsBlock;
     const int tiles4 = (A.n_out + 1 + 3) / 4;
     const long long waves4 = (long long)blocks_y * tiles4 * (kKsBlock / 64);
