@@ -91,14 +91,24 @@ __global__ void __launch_bounds__(kBlock) sap_leaf_kernel(uint64_t* __restrict__
 }
 
 // Sum over the level-1 nodes of the level-synchronous tree: out[w] (+)= sum_e P[e][w], w < words
-// (the per-node leaf contributions of every input, rows of 2N words).
+// (the per-node leaf contributions of every input, rows of 2N words).  Eight node rows are requested
+// before they are added, so a thread keeps eight loads in flight instead of waiting for each.
 __global__ void __launch_bounds__(kBlock) sap_sum_nodes_kernel(uint64_t* __restrict__ out,
                                                                const uint64_t* __restrict__ P, int nodes,
                                                                size_t words, int accumulate) {
+    constexpr int U = 8;
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += stride) {
         uint64_t acc = accumulate ? out[i] : 0;
-        for (int e = 0; e < nodes; ++e) acc = gl::add(acc, P[(size_t)e * words + i]);
+        int e = 0;
+        for (; e + U <= nodes; e += U) {
+            uint64_t v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) v[u] = P[(size_t)(e + u) * words + i];
+#pragma unroll
+            for (int u = 0; u < U; ++u) acc = gl::add(acc, v[u]);
+        }
+        for (; e < nodes; ++e) acc = gl::add(acc, P[(size_t)e * words + i]);
         out[i] = acc;
     }
 }
