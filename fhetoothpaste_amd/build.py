@@ -54,9 +54,6 @@ def build(verbose=True):
     parity = os.path.join(ROOT, "tests", "cpp", "dropin_parity")
     subprocess.check_call([HIPCC, "-O2", "-std=c++17", "-o", parity, os.path.join(ROOT, "tests", "cpp", "dropin_parity.cpp"),
                            "-L" + HERE, "-ltoothpaste_amd", "-Wl,-rpath,$ORIGIN/../../fhetoothpaste_amd"])
-    probe = os.path.join(HERE, "issue_rate_probe")
-    subprocess.check_call([HIPCC, "--offload-arch=gfx950", "-O3", "-o", probe,
-                           os.path.join(CSRC, "issue_rate_probe.hip")])
     if verbose:
         print("built", OUT)
     return OUT

@@ -107,6 +107,7 @@ br_kernel(BrArgs A) {
                 e2 = (e0 + e1) & (2 * N - 1);
             }
             const uint64_t* key = A.brk + (size_t)i * NC * rgsw_words;
+            // the 2 LD products of each output coefficient, summed in lazy words
             uint64_t out[2][kE];
 #pragma unroll
             for (int k = 0; k < kE; ++k) out[0][k] = out[1][k] = 0;
@@ -194,13 +195,22 @@ br_kernel(BrArgs A) {
                             constexpr int c0 = 2 * G;
                             U64x2 na[2], nb[2];
                             if constexpr (G < 3) load_group(na, nb, c0 + 2);
+                            // coefficients 2 c0 .. 2 c0 + 3 against both key rows, as two groups of 4
+                            const uint64_t dv[4] = {d[2 * c0], d[2 * c0 + 1], d[2 * c0 + 2], d[2 * c0 + 3]};
+                            const uint64_t kav[4] = {va[0].x, va[0].y, va[1].x, va[1].y};
+                            const uint64_t kbv[4] = {vb[0].x, vb[0].y, vb[1].x, vb[1].y};
+                            uint64_t g0[4], g1[4];
 #pragma unroll
-                            for (int q = 0; q < 2; ++q) {
-                                const int c = c0 + q;
-                                out[0][2 * c] = gl::mac_lazy(d[2 * c], va[q].x, out[0][2 * c]);
-                                out[0][2 * c + 1] = gl::mac_lazy(d[2 * c + 1], va[q].y, out[0][2 * c + 1]);
-                                out[1][2 * c] = gl::mac_lazy(d[2 * c], vb[q].x, out[1][2 * c]);
-                                out[1][2 * c + 1] = gl::mac_lazy(d[2 * c + 1], vb[q].y, out[1][2 * c + 1]);
+                            for (int m = 0; m < 4; ++m) {
+                                g0[m] = out[0][2 * c0 + m];
+                                g1[m] = out[1][2 * c0 + m];
+                            }
+                            glv::mac_lazy_g<4>(g0, dv, kav);
+                            glv::mac_lazy_g<4>(g1, dv, kbv);
+#pragma unroll
+                            for (int m = 0; m < 4; ++m) {
+                                out[0][2 * c0 + m] = g0[m];
+                                out[1][2 * c0 + m] = g1[m];
                             }
                             if constexpr (G < 3) {
 #pragma unroll
@@ -286,13 +296,27 @@ br_kernel(BrArgs A) {
             }
 #pragma unroll 1
             for (int h = 0; h < 2; ++h) {
-                ntt_inverse<LOGN>(out[0], buf, A.itw, L);
+                if constexpr (METHOD == 0) {
+                    // inverse transform (canonical out) -> acc + result, canonical
+                    const uint64_t qm1 = glv::qm1_const();
+                    uint64_t t[kE];
+                    ntt_inverse<LOGN>(out[0], buf, A.itw, L);
+                    glv::add_cc_g<kE>(out[0], acc[0], t, qm1);
 #pragma unroll
-                for (int k = 0; k < kE; ++k) {
-                    const uint64_t t = gl::canon(gl::add_lc(out[0][k], acc[0][k]));
-                    acc[0][k] = acc[1][k];
-                    acc[1][k] = t;
-                    out[0][k] = out[1][k];
+                    for (int k = 0; k < kE; ++k) {
+                        acc[0][k] = acc[1][k];
+                        acc[1][k] = t[k];
+                        out[0][k] = out[1][k];
+                    }
+                } else {
+                    ntt_inverse<LOGN>(out[0], buf, A.itw, L);
+#pragma unroll
+                    for (int k = 0; k < kE; ++k) {
+                        const uint64_t t = gl::canon(gl::add_lc(out[0][k], acc[0][k]));
+                        acc[0][k] = acc[1][k];
+                        acc[1][k] = t;
+                        out[0][k] = out[1][k];
+                    }
                 }
             }
         }

@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 #include <utility>
 #include "gl64.hpp"
+#include "gl_valu.hpp"
 
 namespace tp {
 
@@ -89,40 +90,40 @@ __device__ __forceinline__ void transpose(uint64_t (&x)[kE], uint64_t* lds, int 
     for (int k = 0; k < kE; ++k) x[k] = lds[phys(layout_idx<LOGN, B>(L, k))];
 }
 
-// Butterflies on lazy values (any 64-bit representative): the product is made canonical
-// once, then one carry/borrow correction suffices for u + t and u - t.
-__device__ __forceinline__ void ct_bfly_t(uint64_t& u, uint64_t& v, uint64_t t_lazy, bool negate) {
-    const uint64_t t = gl::canon(t_lazy);
+// Butterflies, written with the instruction-level primitives of gl_valu.hpp.
+// Forward (Cooley-Tukey): u and v lazy, the product t = v w canonical, u + t and u - t lazy.
+// Inverse (Gentleman-Sande): every value canonical (inputs made canonical on entry).
+// qm1 = Q - 1, the comparison constant of the canonical forms, lives in an SGPR pair.
+__device__ __forceinline__ void ct_bfly_t(uint64_t& u, uint64_t& v, uint64_t t, bool negate) {
     const uint64_t a = u;
-    const uint64_t s = gl::add_lc(a, t), d = gl::sub_lc(a, t);
+    const uint64_t s = glv::add_lc(a, t), d = glv::sub_lc(a, t);
     u = negate ? d : s;  // u + (-t) = u - t: swap where the results go
     v = negate ? s : d;
 }
 
 __device__ __forceinline__ void ct_bfly_pow2(uint64_t& u, uint64_t& v, int e) {
     bool ng;
-    const uint64_t t = gl::mul_2e_unsigned(v, e, ng);
+    const uint64_t t = glv::mul_pow2(v, e, ng, glv::qm1_const());
     ct_bfly_t(u, v, t, ng);
 }
 
 // First forward level on inputs below 2^16 (unsigned gadget digits): its twiddle is 2^48
-// (gl::kFwdPow2[1]), so u + v 2^48 < 2^64 needs no reduction and u - v 2^48 is represented
-// exactly by u + Q - v 2^48, which lies in (0, 2^64).
+// (gl::kFwdPow2[1]), so u + v 2^48 < 2^64 is the word (u, v << 16) and u - v 2^48 is represented
+// exactly by u + Q - v 2^48, the word (u + 1, ~(v << 16)) (Q = (1, 2^32 - 1) in 32-bit halves).
 __device__ __forceinline__ void ct_bfly_small_first(uint64_t& u, uint64_t& v) {
-    const uint64_t t = v << 48;
-    const uint64_t a = u;
-    u = a + t;
-    v = (a + gl::kQ) - t;
+    const uint32_t a = (uint32_t)u, b = (uint32_t)v << 16;
+    u = glv::pk(a, b);
+    v = glv::pk(a + 1, ~b);
 }
 
 // First forward level on signed gadget digits held as u - delta with 0 <= u < 2^16: the pair
 // (u_a - delta) +- (u_b - delta) 2^48 is formed from the small u's (as above) plus two constants of
 // the digit, kp = -delta (1 + 2^48) and km = delta (2^48 - 1) mod Q (both canonical).
 __device__ __forceinline__ void ct_bfly_signed_first(uint64_t& u, uint64_t& v, uint64_t kp, uint64_t km) {
-    const uint64_t t = v << 48;
-    const uint64_t a = u;
-    u = gl::add_lc(a + t, kp);
-    v = gl::add_lc((a + gl::kQ) - t, km);
+    uint64_t a = u, b = v;
+    ct_bfly_small_first(a, b);
+    u = glv::add_lc(a, kp);
+    v = glv::add_lc(b, km);
 }
 
 // The two constants of ct_bfly_signed_first for a digit offset delta < Q.
@@ -133,23 +134,35 @@ __device__ __forceinline__ void signed_first_consts(uint64_t delta, uint64_t& kp
 }
 
 __device__ __forceinline__ void ct_bfly_mul(uint64_t& u, uint64_t& v, uint64_t w) {
-    ct_bfly_t(u, v, gl::mul_lazy(v, w), false);
+    ct_bfly_t(u, v, glv::mul_canon(v, w, glv::qm1_const()), false);
 }
 
+// canonical u, v: (u + v, (u - v) w)
 __device__ __forceinline__ void gs_bfly_mul(uint64_t& u, uint64_t& v, uint64_t w) {
+    const uint64_t qm1 = glv::qm1_const();
     const uint64_t a = u, b = v;
-    u = gl::add_ll(a, b);
-    v = gl::mul_lazy(gl::sub_ll(a, b), w);
+    u = glv::add_cc(a, b, qm1);
+    v = glv::mul_canon(glv::sub_cc(a, b), w, qm1);
 }
 
-// (u, v) -> (u + v, (u - v) 2^e); a negative 2^e swaps the operands of the difference.
+// Sign of 2^e as mul_pow2 reports it (2^96 = -1; 2^e' = -2^-(96 - e') for 64 < e' < 96).
+constexpr bool pow2_negative(int e) { return (e >= 96) != (e % 96 > 64); }
+
+// canonical u, v: (u + v, (u - v) 2^e); a negative 2^e swaps the operands of the difference.
 __device__ __forceinline__ void gs_bfly_pow2(uint64_t& u, uint64_t& v, int e) {
+    const uint64_t qm1 = glv::qm1_const();
     const uint64_t a = u, b = v;
-    u = gl::add_ll(a, b);
-    const bool neg = (e >= 96) != (e % 96 >= 64);
-    const uint64_t diff = neg ? gl::sub_ll(b, a) : gl::sub_ll(a, b);
+    u = glv::add_cc(a, b, qm1);
+    const bool neg = pow2_negative(e);
+    const uint64_t diff = neg ? glv::sub_cc(b, a) : glv::sub_cc(a, b);
     bool ng;
-    v = gl::mul_2e_unsigned(diff, e, ng);
+    v = glv::mul_pow2(diff, e, ng, qm1);
+}
+
+// Lazy -> canonical (3 VALU): x >= Q becomes x + eps (mod 2^64) = x - Q.
+__device__ __forceinline__ uint64_t canon_v(uint64_t x) {
+    glv::lmask c;
+    return glv::mad_eps(glv::cnd01(glv::ge_q(x, glv::qm1_const())), x, c);
 }
 
 // Twiddle tables are read inside the step loops rather than held in registers across them.
@@ -199,20 +212,128 @@ __device__ __forceinline__ void ld_level(uint64_t (&w)[8], const uint64_t* t, ui
     __builtin_amdgcn_sched_barrier(0);
 }
 
+// ---------------------------------------------------------------- level helpers
+//
+// A transform level holds 8 butterflies per lane; butterfly b pairs x[bf_u(H, b)] with
+// x[bf_u(H, b) + H] (H = half distance).  The butterflies are issued as groups of members of the
+// gl_valu.hpp primitives (op by op across the group), never one after the other.
+
+constexpr int bf_u(int H, int b) { return (b / H) * 2 * H + b % H; }
+
+// Butterflies of one level whose products by 2^e are of one kind (gl_valu.hpp P2), with the
+// shift argument of each.
+struct KindList {
+    int n;
+    int b[8];
+    int arg[8];
+};
+template <class E>
+constexpr KindList kind_list(glv::P2 kind) {
+    KindList kl{0, {}, {}};
+    for (int b = 0; b < 8; ++b) {
+        const glv::Pow2Kind pk = glv::pow2_kind(E::e(b));
+        if (pk.kind == kind) {
+            kl.b[kl.n] = b;
+            kl.arg[kl.n] = pk.arg;
+            ++kl.n;
+        }
+    }
+    return kl;
+}
+
+// t[b] = v[b] 2^(E::e(b)) up to the sign glv::pow2_kind(E::e(b)).neg, canonical, for the 8
+// butterflies of a level, one group per kind.
+template <class E, glv::P2 K>
+__device__ __forceinline__ void pow2_kind_group(const uint64_t (&v)[8], uint64_t (&t)[8], uint64_t qm1) {
+    constexpr KindList kl = kind_list<E>(K);
+    if constexpr (kl.n > 0) {
+        constexpr int n = kl.n;
+        uint64_t xv[n], r[n];
+        int arg[n];
+        static_for<n>([&](auto M_) __attribute__((always_inline)) {
+            constexpr int m = M_;
+            xv[m] = v[kl.b[m]];
+            arg[m] = kl.arg[m];
+        });
+        if constexpr (K == glv::P2::kLo) glv::mul_2e_lo_g<n>(xv, arg, r, qm1);
+        else if constexpr (K == glv::P2::kMid) glv::mul_2e_mid_g<n>(xv, arg, r, qm1);
+        else glv::mul_2e_neg_g<n>(xv, arg, r);
+        static_for<n>([&](auto M_) __attribute__((always_inline)) {
+            constexpr int m = M_;
+            t[kl.b[m]] = r[m];
+        });
+    }
+}
+template <class E>
+__device__ __forceinline__ void pow2_products(const uint64_t (&v)[8], uint64_t (&t)[8], uint64_t qm1) {
+    pow2_kind_group<E, glv::P2::kLo>(v, t, qm1);
+    pow2_kind_group<E, glv::P2::kMid>(v, t, qm1);
+    pow2_kind_group<E, glv::P2::kNeg>(v, t, qm1);
+}
+
+// Cooley-Tukey outputs of the 8 butterflies from their canonical products t (neg(b): the
+// product is -t[b], so the sum and difference trade places); two groups of 4.
+template <int H, class NEG>
+__device__ __forceinline__ void ct_outputs(uint64_t (&x)[kE], const uint64_t (&t)[8]) {
+    static_for<2>([&](auto G) __attribute__((always_inline)) {
+        uint64_t u[4], tt[4], s_[4], d_[4];
+        static_for<4>([&](auto M_) __attribute__((always_inline)) {
+            constexpr int b = 4 * G + M_;
+            u[M_] = x[bf_u(H, b)];
+            tt[M_] = t[b];
+        });
+        glv::add_lc_g<4>(u, tt, s_);
+        glv::sub_lc_g<4>(u, tt, d_);
+        static_for<4>([&](auto M_) __attribute__((always_inline)) {
+            constexpr int b = 4 * G + M_;
+            constexpr bool ng = NEG::neg(b);
+            x[bf_u(H, b)] = ng ? d_[M_] : s_[M_];
+            x[bf_u(H, b) + H] = ng ? s_[M_] : d_[M_];
+        });
+    });
+}
+
 // ---------------------------------------------------------------- forward
+
+template <int LV>
+struct FwdP1 {
+    static constexpr int H = 8 >> LV;
+    static constexpr int e(int b) { return gl::kFwdPow2[(1 << LV) + (bf_u(H, b) >> (4 - LV))]; }
+    static constexpr bool neg(int b) { return glv::pow2_kind(e(b)).neg; }
+};
 
 template <int LOGN, int FIRST_LV = 0>
 __device__ __forceinline__ void fwd_pass1(uint64_t (&x)[kE]) {
-#pragma unroll
-    for (int lv = FIRST_LV; lv < 4; ++lv) {
-        const int half = 8 >> lv;
-#pragma unroll
-        for (int k = 0; k < kE; ++k) {
-            if (k & half) continue;
-            const int blk = k >> (4 - lv);
-            ct_bfly_pow2(x[k], x[k + half], gl::kFwdPow2[(1 << lv) + blk]);
-        }
-    }
+    const uint64_t qm1 = glv::qm1_const();
+    static_for<4 - FIRST_LV>([&](auto I) __attribute__((always_inline)) {
+        constexpr int lv = FIRST_LV + I;
+        using E = FwdP1<lv>;
+        uint64_t v[8], t[8];
+        static_for<8>([&](auto B) __attribute__((always_inline)) { v[B] = x[bf_u(E::H, B) + E::H]; });
+        pow2_products<E>(v, t, qm1);
+        ct_outputs<E::H, E>(x, t);
+    });
+}
+
+// A level with table twiddles: butterfly b uses w[bf_u(H, b) >> SS]; products in two groups of 4.
+struct NoNeg {
+    static constexpr bool neg(int) { return false; }
+};
+template <int H, int SS>
+__device__ __forceinline__ void ct_level_tw(uint64_t (&x)[kE], const uint64_t (&w)[8]) {
+    const uint64_t qm1 = glv::qm1_const();
+    uint64_t t[8];
+    static_for<2>([&](auto G) __attribute__((always_inline)) {
+        uint64_t v[4], ww[4], r[4];
+        static_for<4>([&](auto M_) __attribute__((always_inline)) {
+            constexpr int b = 4 * G + M_;
+            v[M_] = x[bf_u(H, b) + H];
+            ww[M_] = w[bf_u(H, b) >> SS];
+        });
+        glv::mul_canon_g<4>(v, ww, r, qm1);
+        static_for<4>([&](auto M_) __attribute__((always_inline)) { t[4 * G + M_] = r[M_]; });
+    });
+    ct_outputs<H, NoNeg>(x, t);
 }
 
 // Levels 4 .. LOGN - 1 take their twiddles from the table: level lv of pass 2 (lv < 8, layout
@@ -256,11 +377,7 @@ __device__ __forceinline__ void ntt_forward(uint64_t (&x)[kE], uint64_t* lds, co
             ld_level<Level<LOGN, lv + 1>::NB>(wn, opaque_after(tw, x[0]), Level<LOGN, lv + 1>::first(L));
         else
             at_last();
-#pragma unroll
-        for (int k = 0; k < kE; ++k) {
-            if (k & Lv::HALF) continue;
-            ct_bfly_mul(x[k], x[k + Lv::HALF], w[k >> Lv::SLOT_SHIFT]);
-        }
+        ct_level_tw<Lv::HALF, Lv::SLOT_SHIFT>(x, w);
         if constexpr (lv + 1 < LOGN) {
 #pragma unroll
             for (int b = 0; b < 8; ++b) w[b] = wn[b];
@@ -270,37 +387,99 @@ __device__ __forceinline__ void ntt_forward(uint64_t (&x)[kE], uint64_t* lds, co
 
 // ---------------------------------------------------------------- inverse
 
-// Levels m = 8..1 with power-of-two inverse twiddles; N^-1 = 2^(192 - LOGN) is merged into
-// the last level (both of its factors are powers of two).
-template <int LOGN>
-__device__ __forceinline__ void inv_pass1(uint64_t (&x)[kE]) {
-#pragma unroll
-    for (int lv = 3; lv >= 1; --lv) {
-        const int half = 8 >> lv;
-#pragma unroll
-        for (int k = 0; k < kE; ++k) {
-            if (k & half) continue;
-            const int blk = k >> (4 - lv);
-            gs_bfly_pow2(x[k], x[k + half], gl::kInvPow2[(1 << lv) + blk]);
-        }
-    }
-    constexpr int kScale = 192 - LOGN;
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        const uint64_t a = x[k], b = x[k + 8];
-        x[k] = gl::div_2e_lazy(gl::add_ll(a, b), LOGN);
-        const int e = (gl::kInvPow2[1] + kScale) % 192;
-        const bool neg = (e >= 96) != (e % 96 >= 64);
-        const uint64_t diff = neg ? gl::sub_ll(b, a) : gl::sub_ll(a, b);
-        bool ng;
-        x[k + 8] = gl::mul_2e_unsigned(diff, e, ng);
-    }
+// Gentleman-Sande level on canonical values: x[u] = x[u] + x[v], and the difference (operands
+// swapped where the product by the twiddle is negative) goes to d[b] for the caller's products.
+template <int H, class NEG>
+__device__ __forceinline__ void gs_sums_diffs(uint64_t (&x)[kE], uint64_t (&d)[8], uint64_t qm1) {
+    static_for<2>([&](auto G) __attribute__((always_inline)) {
+        uint64_t u[4], v[4], s_[4], a_[4], b_[4], r[4];
+        static_for<4>([&](auto M_) __attribute__((always_inline)) {
+            constexpr int b = 4 * G + M_;
+            u[M_] = x[bf_u(H, b)];
+            v[M_] = x[bf_u(H, b) + H];
+            a_[M_] = NEG::neg(b) ? v[M_] : u[M_];
+            b_[M_] = NEG::neg(b) ? u[M_] : v[M_];
+        });
+        glv::add_cc_g<4>(u, v, s_, qm1);
+        glv::sub_lc_g<4>(a_, b_, r);
+        static_for<4>([&](auto M_) __attribute__((always_inline)) {
+            constexpr int b = 4 * G + M_;
+            x[bf_u(H, b)] = s_[M_];
+            d[b] = r[M_];
+        });
+    });
 }
 
-// Lazy NTT values in pass-3 layout -> lazy natural-order coefficients in pass-1 layout (table
-// levels LOGN - 1 .. 4 with the next level's twiddles requested during the current one).
+template <int LV>
+struct InvP1 {
+    static constexpr int H = 8 >> LV;
+    static constexpr int e(int b) { return gl::kInvPow2[(1 << LV) + (bf_u(H, b) >> (4 - LV))]; }
+    static constexpr bool neg(int b) { return glv::pow2_kind(e(b)).neg; }
+};
+
+// The last inverse level with N^-1 = 2^-LOGN merged in: x[k] = (a + b) 2^-LOGN and
+// x[k + 8] = (a - b) 2^(e1 - LOGN), e1 the level's exponent.
 template <int LOGN>
+struct InvLast {
+    static constexpr int e(int) { return (gl::kInvPow2[1] + 192 - LOGN) % 192; }
+    static constexpr bool neg(int b) { return glv::pow2_kind(e(b)).neg; }
+};
+
+// Levels m = 8..1 with power-of-two inverse twiddles on canonical values; N^-1 = 2^(192 - LOGN)
+// is merged into the last level (both of its factors are powers of two).
+template <int LOGN>
+__device__ __forceinline__ void inv_pass1(uint64_t (&x)[kE]) {
+    const uint64_t qm1 = glv::qm1_const();
+    static_for<3>([&](auto I) __attribute__((always_inline)) {
+        constexpr int lv = 3 - I;
+        using E = InvP1<lv>;
+        uint64_t d[8], t[8];
+        gs_sums_diffs<E::H, E>(x, d, qm1);
+        pow2_products<E>(d, t, qm1);
+        static_for<8>([&](auto B) __attribute__((always_inline)) { x[bf_u(E::H, B) + E::H] = t[B]; });
+    });
+    using E = InvLast<LOGN>;
+    uint64_t d[8], t[8];
+    gs_sums_diffs<8, E>(x, d, qm1);
+    static_for<2>([&](auto G) __attribute__((always_inline)) {
+        uint64_t s_[4], r[4];
+        int sh[4];
+        static_for<4>([&](auto M_) __attribute__((always_inline)) {
+            s_[M_] = x[4 * G + M_];
+            sh[M_] = LOGN;
+        });
+        glv::mul_2e_neg_g<4>(s_, sh, r);
+        static_for<4>([&](auto M_) __attribute__((always_inline)) { x[4 * G + M_] = r[M_]; });
+    });
+    pow2_products<E>(d, t, qm1);
+    static_for<8>([&](auto B) __attribute__((always_inline)) { x[B + 8] = t[B]; });
+}
+
+// A Gentleman-Sande level with table twiddles on canonical values: butterfly b uses
+// w[bf_u(H, b) >> SS].
+template <int H, int SS>
+__device__ __forceinline__ void gs_level_tw(uint64_t (&x)[kE], const uint64_t (&w)[8]) {
+    const uint64_t qm1 = glv::qm1_const();
+    uint64_t d[8];
+    gs_sums_diffs<H, NoNeg>(x, d, qm1);
+    static_for<2>([&](auto G) __attribute__((always_inline)) {
+        uint64_t dd[4], ww[4], r[4];
+        static_for<4>([&](auto M_) __attribute__((always_inline)) {
+            constexpr int b = 4 * G + M_;
+            dd[M_] = d[b];
+            ww[M_] = w[bf_u(H, b) >> SS];
+        });
+        glv::mul_canon_g<4>(dd, ww, r, qm1);
+        static_for<4>([&](auto M_) __attribute__((always_inline)) { x[bf_u(H, 4 * G + M_) + H] = r[M_]; });
+    });
+}
+
+// NTT values in pass-3 layout (lazy, or canonical when CANON_IN) -> canonical natural-order
+// coefficients in pass-1 layout (table levels LOGN - 1 .. 4 with the next level's twiddles
+// requested during the current one).
+template <int LOGN, bool CANON_IN = false>
 __device__ __forceinline__ void ntt_inverse(uint64_t (&x)[kE], uint64_t* lds, const uint64_t* itw, int L) {
+    if constexpr (!CANON_IN) glv::canon_g<kE>(x, glv::qm1_const());
     itw = opaque(itw);
     uint64_t w[8];
     ld_level<Level<LOGN, LOGN - 1>::NB>(w, opaque_after(itw, x[0]), Level<LOGN, LOGN - 1>::first(L));
@@ -311,11 +490,7 @@ __device__ __forceinline__ void ntt_inverse(uint64_t (&x)[kE], uint64_t* lds, co
         uint64_t wn[8];
         if constexpr (lv > 4)
             ld_level<Level<LOGN, lv - 1>::NB>(wn, opaque_after(itw, x[0]), Level<LOGN, lv - 1>::first(L));
-#pragma unroll
-        for (int k = 0; k < kE; ++k) {
-            if (k & Lv::HALF) continue;
-            gs_bfly_mul(x[k], x[k + Lv::HALF], w[k >> Lv::SLOT_SHIFT]);
-        }
+        gs_level_tw<Lv::HALF, Lv::SLOT_SHIFT>(x, w);
         if constexpr (lv > 4) {
 #pragma unroll
             for (int b = 0; b < 8; ++b) w[b] = wn[b];
