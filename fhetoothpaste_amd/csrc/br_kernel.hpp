@@ -138,12 +138,30 @@ br_kernel(BrArgs A) {
                     uint64_t d[kE];
                     if constexpr (METHOD == 0) {
                         const int shift = A.first_shift - j * A.base_log2;
-                        // the lane index re-read per digit: its LDS addresses are formed here, not
-                        // held in registers across the digit loop
-                        int Lj = L;
+                        // the lane index re-read per digit: its LDS address is formed here, not
+                        // held in a register across the digit loop.  Coefficient L + P k sits at
+                        // word (L + L / 16) + k (P + P / 16): one base plus constant offsets.
+                        uint32_t Lj = (uint32_t)L;
                         asm volatile("" : "+v"(Lj));
+                        const uint32_t base = Lj + (Lj >> 4);
+                        constexpr uint32_t kStride = S::P + S::P / 16;
+                        const uint32_t* src32 = (const uint32_t*)src;
+                        if (shift >= 32 && A.base_log2 < 32) {
+                            // the digit lies in the high word: one 32-bit read and a bit-field extract
+                            // (bits past the word's top are zero in both forms)
 #pragma unroll
-                        for (int k = 0; k < kE; ++k) d[k] = (src[phys(idx1<LOGN>(Lj, k))] >> shift) & mask;
+                            for (int k = 0; k < kE; ++k)
+                                d[k] = __builtin_amdgcn_ubfe(src32[2 * (base + k * kStride) + 1], (uint32_t)(shift - 32),
+                                                             (uint32_t)A.base_log2);
+                        } else if (shift + A.base_log2 <= 32 && A.base_log2 < 32) {
+#pragma unroll
+                            for (int k = 0; k < kE; ++k)
+                                d[k] = __builtin_amdgcn_ubfe(src32[2 * (base + k * kStride)], (uint32_t)shift,
+                                                             (uint32_t)A.base_log2);
+                        } else {
+#pragma unroll
+                            for (int k = 0; k < kE; ++k) d[k] = (src[base + k * kStride] >> shift) & mask;
+                        }
                         // the first transform level on the digits themselves: cheap while they are small
                         if (A.base_log2 <= 16) {
 #pragma unroll
