@@ -83,6 +83,8 @@ SIGNATURES = {
     "tp_sap_reset_device": [vp, SZ, U64, vp],
     "tp_lwe_to_rgsw_device": [vp, SZ, U32, U32, vp, vp],
     "tp_keygen_rlwe_ksk": [u64p, u64p, u64p, U32, U32, U32, ctypes.c_double, U64, U64],
+    "tp_sample_gaussian": [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_double, U64, U64],
+    "tp_chacha20_block": [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p],
     "tp_wire_bytes": [SZ, ctypes.POINTER(SZ)],
     "tp_wire_encode": [ctypes.c_char_p, SZ, INT, U32, ctypes.POINTER(Params), u64p, SZ],
     "tp_wire_decode": [ctypes.c_char_p, SZ, ctypes.POINTER(INT), ctypes.POINTER(U32), ctypes.POINTER(Params), u64p,

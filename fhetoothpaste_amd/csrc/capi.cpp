@@ -199,6 +199,9 @@ int tp_context_create(tp_context** out, const tp_params* p, int device) {
         p->br_digits > max_digits(p->br_base_log2))
         return fail(TP_E_INVALID, "bad gadget parameters");
     if (p->method == TP_METHOD_BMMP && (p->lwe_dim & 1)) return fail(TP_E_INVALID, "BMMP needs an even lwe_dim");
+    // blind-rotation kernels are built for 2..4 digits at N = 1024 / 2048 and 3..4 at N = 4096
+    if (p->br_digits < 2 || p->br_digits > 4 || (N == 4096 && p->br_digits < 3))
+        return fail(TP_E_INVALID, "no blind-rotation kernel is built for this (ring_dim, digits)");
     if (p->rlwe_ks_digits) {
         const uint32_t b = p->rlwe_ks_base_log2, d = p->rlwe_ks_digits;
         if (b == 0 || b > 32 || d > max_digits(b)) return fail(TP_E_INVALID, "bad automorphism-key gadget parameters");

@@ -311,7 +311,7 @@ Container rotation_output_container(const Container& input, uint64_t N, uint64_t
 CGGIBlindRotationContext::CGGIBlindRotationContext(KeyDistribution distr, uint64_t modulus, uint64_t ring_dim,
                                                    uint64_t lwe_dim, uint64_t basis, uint64_t digits, double std)
     : m_distribution(distr), m_modulus(modulus), m_N(ring_dim), m_n(lwe_dim), m_digits(digits), m_std(std),
-      m_seed(toothpaste_amd::fresh_seed()) {
+      m_seed(TP_SEED_OS_ENTROPY) {
     SetBlindRotationBasis(basis);
     m_ntt = SelectWorker(modulus, ring_dim);
 }
@@ -319,7 +319,7 @@ CGGIBlindRotationContext::CGGIBlindRotationContext(KeyDistribution distr, uint64
 CGGIBlindRotationContext::CGGIBlindRotationContext(KeyDistribution distr, std::shared_ptr<MathWorker> ntt,
                                                    uint64_t lwe_dim, uint64_t basis, uint64_t digits, double std)
     : m_distribution(distr), m_modulus(ntt->GetModulus()), m_N(ntt->GetDimension()), m_n(lwe_dim), m_digits(digits),
-      m_std(std), m_seed(toothpaste_amd::fresh_seed()), m_ntt(std::move(ntt)) {
+      m_std(std), m_seed(TP_SEED_OS_ENTROPY), m_ntt(std::move(ntt)) {
     SetBlindRotationBasis(basis);
 }
 
@@ -415,7 +415,7 @@ BMMPBlindRotationContext::BMMPBlindRotationContext(KeyDistribution distr, uint64
                                                    uint64_t step_size)
     : m_distribution(distr), m_modulus(modulus), m_N(ring_dim), m_n(lwe_dim), m_basis(basis),
       m_basis_log2(int_log2(basis)), m_digits(digits), m_step_size(step_size), m_std(std),
-      m_seed(toothpaste_amd::fresh_seed()) {
+      m_seed(TP_SEED_OS_ENTROPY) {
     m_ntt = SelectWorker(modulus, ring_dim);
 }
 
@@ -424,7 +424,7 @@ BMMPBlindRotationContext::BMMPBlindRotationContext(KeyDistribution distr, std::s
                                                    uint64_t step_size)
     : m_distribution(distr), m_modulus(ntt->GetModulus()), m_N(ntt->GetDimension()), m_n(lwe_dim), m_basis(basis),
       m_basis_log2(int_log2(basis)), m_digits(digits), m_step_size(step_size), m_std(std),
-      m_seed(toothpaste_amd::fresh_seed()), m_ntt(std::move(ntt)) {}
+      m_seed(TP_SEED_OS_ENTROPY), m_ntt(std::move(ntt)) {}
 
 long double BMMPBlindRotationContext::ComputeOutputVariance(long double) const {
     return 0.0L;  // the reference has no estimate for BMMP (bmmp_blind_rotation.cpp:122-125)
@@ -503,7 +503,7 @@ LWEConversionContext::LWEConversionContext(KeyDistribution source_key_distributi
                                            uint64_t digits, double std)
     : m_source_distribution(source_key_distribution), m_modulus(modulus), m_source_dimension(source_dimension),
       m_target_dimension(target_dimension), m_basis(basis), m_basis_log2(int_log2(basis)), m_digits(digits),
-      m_std(std), m_seed(toothpaste_amd::fresh_seed()) {}
+      m_std(std), m_seed(TP_SEED_OS_ENTROPY) {}
 
 long double LWEConversionContext::ComputeOutputVariance(long double input_variance) const {
     // lwe_conversion.cpp:114-146: n_src * digits * basis^2 std^2 / 12 + E|s|_0 * delta^2/12 * |s_i|^2
@@ -593,7 +593,7 @@ void LWEtoLWEConverter::ConvertBatch(uint64_t* outputs, const uint64_t* inputs, 
 RLWEConversionContext::RLWEConversionContext(KeyDistribution source_key_distribution, uint64_t modulus, uint64_t N,
                                              uint64_t basis, uint64_t digits, double std)
     : m_source_distribution(source_key_distribution), m_modulus(modulus), m_N(N), m_basis(basis),
-      m_basis_log2(int_log2(basis)), m_digits(digits), m_std(std), m_seed(toothpaste_amd::fresh_seed()) {
+      m_basis_log2(int_log2(basis)), m_digits(digits), m_std(std), m_seed(TP_SEED_OS_ENTROPY) {
     m_ntt = SelectWorker(modulus, N);
 }
 
@@ -601,7 +601,7 @@ RLWEConversionContext::RLWEConversionContext(KeyDistribution source_key_distribu
                                              uint64_t basis, uint64_t digits, double std)
     : m_source_distribution(source_key_distribution), m_modulus(ntt->GetModulus()), m_N(ntt->GetDimension()),
       m_basis(basis), m_basis_log2(int_log2(basis)), m_digits(digits), m_std(std),
-      m_seed(toothpaste_amd::fresh_seed()), m_ntt(std::move(ntt)) {}
+      m_seed(TP_SEED_OS_ENTROPY), m_ntt(std::move(ntt)) {}
 
 long double RLWEConversionContext::ComputeOutputVariance(long double input_variance) const {
     return input_variance + product_variance(m_modulus, m_N, m_basis, m_digits, m_std, m_source_distribution);

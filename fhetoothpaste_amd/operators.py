@@ -313,7 +313,20 @@ class Context:
         return d_out.to_host().reshape(count, W)
 
 
-# ---------------------------------------------------------------- key generation (host, seeded)
+# ---------------------------------------------------------------- key generation (host)
+
+# seed value that selects ChaCha20 keyed from the operating system (real keys); any other seed is
+# a reproducible test stream whose outputs reveal its state
+SEED_OS_ENTROPY = 0xFFFFFFFFFFFFFFFF
+
+
+def _words(a, n, name):
+    """a as a contiguous uint64 array of exactly n words (the C keygen reads n words)."""
+    a = np.ascontiguousarray(a, dtype=np.uint64).reshape(-1)
+    if a.size != n:
+        raise ValueError(f"{name} holds {a.size} words, expected {n}")
+    return a
+
 
 def keygen_secret(length, dist, seed, tag):
     out = np.empty(length, dtype=np.uint64)
@@ -322,34 +335,52 @@ def keygen_secret(length, dist, seed, tag):
 
 
 def keygen_brk(lwe_key, rlwe_key, N, n, base_log2, digits, method, gadget_kind, sigma, seed):
+    if method not in (0, 1, 2):
+        raise ValueError(f"unknown blind-rotation method {method}")
     per = {0: 4 * N * digits * n, 1: 8 * N * digits * n, 2: 12 * N * digits * (n // 2)}[method]
+    lk, rk = _words(lwe_key, n, "lwe_key"), _words(rlwe_key, N, "rlwe_key")
     out = np.empty(per, dtype=np.uint64)
-    check(lib().tp_keygen_brk(ptr(out), ptr(np.ascontiguousarray(lwe_key, np.uint64)),
-                              ptr(np.ascontiguousarray(rlwe_key, np.uint64)), N, n, base_log2, digits, method,
-                              gadget_kind, sigma, seed))
+    check(lib().tp_keygen_brk(ptr(out), ptr(lk), ptr(rk), N, n, base_log2, digits, method, gadget_kind, sigma, seed))
     return out
 
 
 def keygen_auto_keys(s, N, base_log2, digits, sigma, seed):
     logN = N.bit_length() - 1
+    sk = _words(s, N, "s")
     out = np.empty(logN * digits * 2 * N, dtype=np.uint64)
-    check(lib().tp_keygen_auto_keys(ptr(out), ptr(np.ascontiguousarray(s, np.uint64)), N, base_log2, digits, sigma,
-                                    seed))
+    check(lib().tp_keygen_auto_keys(ptr(out), ptr(sk), N, base_log2, digits, sigma, seed))
     return out
 
 
 def keygen_rlwe_ksk(src, dst, N, base_log2, digits, sigma, seed, tag=6):
+    sk, dk = _words(src, N, "src"), _words(dst, N, "dst")
     out = np.empty(digits * 2 * N, dtype=np.uint64)
-    check(lib().tp_keygen_rlwe_ksk(ptr(out), ptr(np.ascontiguousarray(src, np.uint64)),
-                                   ptr(np.ascontiguousarray(dst, np.uint64)), N, base_log2, digits, sigma, seed, tag))
+    check(lib().tp_keygen_rlwe_ksk(ptr(out), ptr(sk), ptr(dk), N, base_log2, digits, sigma, seed, tag))
     return out
 
 
 def keygen_lwe_ksk(src, dst, n_in, n_out, base_log2, digits, sigma, seed):
+    sk, dk = _words(src, n_in, "src"), _words(dst, n_out, "dst")
     out = np.empty(n_in * digits * (n_out + 1), dtype=np.uint64)
-    check(lib().tp_keygen_lwe_ksk(ptr(out), ptr(np.ascontiguousarray(src, np.uint64)),
-                                  ptr(np.ascontiguousarray(dst, np.uint64)), n_in, n_out, base_log2, digits,
-                                  sigma, seed))
+    check(lib().tp_keygen_lwe_ksk(ptr(out), ptr(sk), ptr(dk), n_in, n_out, base_log2, digits, sigma, seed))
+    return out
+
+
+def sample_gaussian(count, sigma, seed, tag=0):
+    """count discrete Gaussian samples (the keygen noise sampler) as int64."""
+    out = np.empty(count, dtype=np.int64)
+    check(lib().tp_sample_gaussian(out.ctypes.data, count, sigma, seed, tag))
+    return out
+
+
+def chacha20_block(key_words, counter, nonce_words):
+    """The ChaCha20 block function of the entropy stream (RFC 8439, section 2.3): 16 uint32 words."""
+    key = np.ascontiguousarray(key_words, dtype=np.uint32)
+    nonce = np.ascontiguousarray(nonce_words, dtype=np.uint32)
+    if key.size != 8 or nonce.size != 3:
+        raise ValueError("ChaCha20 takes an 8-word key and a 3-word nonce")
+    out = np.empty(16, dtype=np.uint32)
+    check(lib().tp_chacha20_block(key.ctypes.data, counter, nonce.ctypes.data, out.ctypes.data))
     return out
 
 

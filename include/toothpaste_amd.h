@@ -34,6 +34,7 @@ extern "C" {
 #define TP_E_STATE (-3)     /* keys not imported yet (reference: assert(m_keys_generated)) */
 #define TP_E_NOMEM (-4)     /* device allocation failed */
 #define TP_E_TIMEOUT (-5)   /* tp_server_wait: request not complete in time */
+#define TP_E_ENTROPY (-6)   /* key generation: the operating system's random source failed */
 
 #define TP_Q 0xFFFFFFFF00000001ULL
 
@@ -335,13 +336,20 @@ int tp_wire_save(const char* path, int kind, uint32_t flags, const tp_params* pa
 int tp_wire_load(const char* path, int* kind, uint32_t* flags, tp_params* params, uint64_t* data, size_t cap_words,
                  size_t* words);
 
-/* ------------------------------------------------------------------ seeded key generation
- * Host-side key generation with a seeded sampler (the reference samples with OpenFHE, which
- * has no seed API: lib/src/base_crypto.cpp:34-38,139-142).  Output bytes use the reference
- * layouts, so they can be passed to tp_import_* and to any reference-layout consumer.
+/* ------------------------------------------------------------------ key generation
+ * Host-side key generation (the reference samples with OpenFHE, which has no seed API:
+ * lib/src/base_crypto.cpp:34-38,139-142).  Output bytes use the reference layouts, so they can be
+ * passed to tp_import_* and to any reference-layout consumer.
+ * seed == TP_SEED_OS_ENTROPY: the words come from ChaCha20 (RFC 8439) keyed with 256 bits from the
+ *   operating system (getrandom) on every call; use this for real keys.
+ * any other seed: a reproducible SplitMix64 stream (seed, tag), for tests and parity checks only:
+ *   its outputs reveal its state, and with it the noise of every key drawn from it.
+ * Gaussian noise: a 63-bit inversion table for sigma <= 21 (|e| <= 12 sigma), above that the
+ *   Canonne-Kamath-Steinke rejection sampler (no tail cut).
  * dist: 0 binary, 1 ternary (P(0) = 1/2, P(+-1) = 1/4, -1 stored as Q-1).
  * gadget_kind: 0 = MakeRGSW weights L^(D-1-j) (base_crypto.cpp:68,85-95),
  *              1 = signed-decomposer weights 2^(64-b(j+1)) (gadget_decomp.h:176-193). */
+#define TP_SEED_OS_ENTROPY 0xFFFFFFFFFFFFFFFFULL
 int tp_keygen_secret(uint64_t* out, uint32_t len, int dist, uint64_t seed, uint64_t tag);
 int tp_keygen_brk(uint64_t* brk, const uint64_t* lwe_key, const uint64_t* rlwe_key, uint32_t N, uint32_t n,
                   uint32_t base_log2, uint32_t digits, int method, int gadget_kind, double sigma, uint64_t seed);
@@ -355,6 +363,13 @@ int tp_keygen_auto_keys(uint64_t* keys, const uint64_t* s, uint32_t N, uint32_t 
  * g_i = 2^(64 - b(i+1)) (kept >= 1), out [d][2N] NTT domain; stream (seed, tag). */
 int tp_keygen_rlwe_ksk(uint64_t* key, const uint64_t* src, const uint64_t* dst, uint32_t N, uint32_t base_log2,
                        uint32_t digits, double sigma, uint64_t seed, uint64_t tag);
+/* count discrete Gaussian samples of parameter sigma (the noise of the keygen functions) from the
+ * stream (seed, tag); the sampler behind OpenFHE's DiscreteGaussianGenerator in the reference
+ * (base_crypto.cpp:34-38). */
+int tp_sample_gaussian(int64_t* out, size_t count, double sigma, uint64_t seed, uint64_t tag);
+/* The ChaCha20 block function of the TP_SEED_OS_ENTROPY stream (RFC 8439, section 2.3), exposed
+ * so that its output can be checked against the RFC's test vector. */
+int tp_chacha20_block(const uint32_t key[8], uint32_t counter, const uint32_t nonce[3], uint32_t out[16]);
 
 #ifdef __cplusplus
 }

@@ -17,8 +17,9 @@
 //
 // Differences a caller can observe:
 //   * Keys are generated inside ConstructOperator from the given secret keys with the library's
-//     seeded sampler (SetSeed on the context; by default a fresh seed from std::random_device),
-//     instead of OpenFHE's unseedable one.  Binary CGGI keys use the MakeRGSW gadget weights,
+//     sampler instead of OpenFHE's: by default from ChaCha20 keyed by the operating system's
+//     random source (GetSeed() returns TP_SEED_OS_ENTROPY); SetSeed(s) switches the context to a
+//     reproducible stream, for tests and parity checks only (its outputs reveal its state).  Binary CGGI keys use the MakeRGSW gadget weights,
 //     ternary/BMMP/RLWE keys the signed decomposer's weights 2^(64-b(j+1)).
 //   * Only the Goldilocks modulus Q = 2^64 - 2^32 + 1 and ring dimensions 1024, 2048, 4096 run
 //     here; other values throw std::invalid_argument (the reference falls back to HEXL).

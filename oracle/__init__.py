@@ -81,6 +81,7 @@ def lib():
             "or_sap_decompose": (ctypes.c_int, [ctypes.c_void_p, u64p, u64p, U32, U32, u64p, U32, U32, U32,
                                                 ctypes.c_int, u64p, U32, U32, U64, u64p, U32, U32]),
             "or_modswitch": (None, [u64p, ctypes.c_uint64, U64, U64, ctypes.c_int]),
+            "or_sample_gaussian": (None, [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_double, U64, U64]),
             "or_pack_consecutive": (ctypes.c_int, [ctypes.c_void_p, u64p, u64p, U32, u64p, U32, U32]),
             "or_bootstrap_batch": (ctypes.c_double, [ctypes.c_void_p, u64p, u64p, ctypes.c_size_t, u64p, u64p,
                                                      U32, U32, U32, ctypes.c_int, u64p, U32, U32, U32,
@@ -152,6 +153,12 @@ class Ring:
 def secret(length, dist, seed, tag):
     out = arr(length)
     lib().or_keygen_secret(P(out), length, dist, seed, tag)
+    return out
+
+
+def sample_gaussian(count, sigma, seed, tag):
+    out = np.zeros(count, dtype=np.int64)
+    lib().or_sample_gaussian(out.ctypes.data, count, sigma, seed, tag)
     return out
 
 

@@ -540,13 +540,20 @@ static uint64_t rng_uniform_q(or_rng* g) {
     return x;
 }
 
-/* Discrete Gaussian by inversion of a 63-bit cumulative table over |x| <= 12 sigma. */
+/* Discrete Gaussian.  sigma <= 21: inversion of a 63-bit cumulative table over |x| <= 12 sigma.
+ * sigma > 21: the rejection sampler of Canonne, Kamath and Steinke ("The Discrete Gaussian for
+ * Differential Privacy", 2020, Algorithms 1-3) -- a discrete Laplace proposal of scale
+ * t = floor(sigma) + 1 accepted with probability exp(-(|y| - sigma^2/t)^2 / (2 sigma^2)),
+ * Bernoulli(exp(-g)) as a 53-bit uniform below exp(-g).  Same draws, in the same order, as the
+ * library's keygen (both consume the seeded stream above). */
 typedef struct { double sigma; uint32_t K; uint64_t T[256]; } or_gauss;
+#define OR_GAUSS_TABLE_MAX 21.0
 static void gauss_init(or_gauss* G, double sigma) {
     G->sigma = sigma;
-    if (sigma <= 0) { G->K = 0; G->T[0] = 1ULL << 63; return; }
+    G->K = 0;
+    if (sigma <= 0) { G->T[0] = 1ULL << 63; return; }
+    if (sigma > OR_GAUSS_TABLE_MAX) return;
     uint32_t K = (uint32_t)ceil(sigma * 12.0);
-    if (K > 255) K = 255;
     double w[256], tot = 0;
     for (uint32_t k = 0; k <= K; ++k) {
         w[k] = exp(-(double)(k * k) / (2.0 * sigma * sigma));
@@ -560,12 +567,52 @@ static void gauss_init(or_gauss* G, double sigma) {
     }
     G->K = K;
 }
-static uint64_t gauss_sample(const or_gauss* G, or_rng* g) {
+static double rng_unit(or_rng* g) { return (double)(rng_next(g) >> 11) * 0x1.0p-53; }
+static uint64_t rng_below(or_rng* g, uint64_t n) {
+    if (n <= 1) return 0;
+    const uint64_t lim = ~0ULL - (~0ULL % n);
+    uint64_t x;
+    do { x = rng_next(g); } while (x >= lim);
+    return x % n;
+}
+static int bern_exp(or_rng* g, double gamma) { return rng_unit(g) < exp(-gamma); }
+static int64_t dlaplace(or_rng* g, uint64_t t) {
+    for (;;) {
+        const uint64_t u = rng_below(g, t);
+        if (!bern_exp(g, (double)u / (double)t)) continue;
+        uint64_t v = 0;
+        while (bern_exp(g, 1.0)) ++v;
+        const int b = (int)(rng_next(g) >> 63);
+        if (b && u == 0 && v == 0) continue;
+        const int64_t z = (int64_t)(u + t * v);
+        return b ? -z : z;
+    }
+}
+static int64_t gauss_int(const or_gauss* G, or_rng* g) {
+    if (G->sigma > OR_GAUSS_TABLE_MAX) {
+        const uint64_t t = (uint64_t)floor(G->sigma) + 1;
+        const double s2 = G->sigma * G->sigma;
+        for (;;) {
+            const int64_t y = dlaplace(g, t);
+            const double d = fabs((double)y) - s2 / (double)t;
+            if (bern_exp(g, d * d / (2.0 * s2))) return y;
+        }
+    }
     const uint64_t x = rng_next(g);
     const uint64_t u = x >> 1;
     uint32_t k = 0;
     while (k < G->K && u >= G->T[k]) ++k;
-    return (x & 1) ? neg(k) : k;
+    return (x & 1) ? -(int64_t)k : (int64_t)k;
+}
+static uint64_t gauss_sample(const or_gauss* G, or_rng* g) {
+    const int64_t v = gauss_int(G, g);
+    return v >= 0 ? (uint64_t)v : neg((uint64_t)(-v));
+}
+OR_EXPORT void or_sample_gaussian(int64_t* out, size_t count, double sigma, uint64_t seed, uint64_t tag) {
+    or_rng g = rng_make(seed, tag);
+    or_gauss G;
+    gauss_init(&G, sigma);
+    for (size_t i = 0; i < count; ++i) out[i] = gauss_int(&G, &g);
 }
 
 /* Secret keys.  dist 0: binary {0,1}; 1: ternary P(0)=1/2, P(1)=P(-1)=1/4 with -1 stored as Q-1. */

@@ -1,6 +1,7 @@
 """Parameter checks of tp_context_create, which run before any device call.
 
-The checks guard what the kernels assume: ring dimensions with a built transform, gadgets whose
+The checks guard what the kernels assume: ring dimensions with a built transform, digit counts with a
+built blind-rotation kernel (2..4 at N = 1024 / 2048, 3..4 at N = 4096), gadgets whose
 digits cover at most 64 bits (so a key-switch chunk of at most 64 key rows holds at least one input
 word), and an even LWE dimension for BMMP's pairs.  Rejected parameters return TP_E_INVALID; accepted
 ones go on to the device query (TP_E_DEVICE without a GPU, a context with one).
@@ -26,6 +27,9 @@ def create(**kw):
     dict(base_log2=0), dict(base_log2=33), dict(digits=0),
     dict(base_log2=16, digits=5),                       # 5 digits of 16 bits exceed 64
     dict(method=2, lwe_dim=15),                         # BMMP pairs the LWE coefficients
+    dict(base_log2=16, digits=1),                       # no blind-rotation kernel for 1 digit
+    dict(base_log2=12, digits=5),                       # ... nor for 5
+    dict(ring_dim=4096, base_log2=16, digits=2),        # ... nor for 2 digits at N = 4096
     dict(ks_out_dim=40, ks_base_log2=0, ks_digits=4),
     dict(ks_out_dim=40, ks_base_log2=17, ks_digits=2),
     dict(ks_out_dim=40, ks_base_log2=4, ks_digits=0),
