@@ -55,6 +55,7 @@ SIGNATURES = {
     "tp_eval_func_batch_device": [vp, SZ, vp, vp, SZ, vp, vp, INT],
     "tp_lwe_keyswitch_batch_device": [vp, SZ, vp, SZ, vp],
     "tp_bootstrap_batch_device": [vp, SZ, vp, vp, SZ, vp],
+    "tp_bootstrap_batch_device_noalloc": [vp, SZ, vp, vp, SZ, vp],
     "tp_malloc": [vp, ctypes.POINTER(vp), SZ],
     "tp_free": [vp, vp],
     "tp_copy_to_device": [vp, vp, u64p, SZ],

@@ -27,6 +27,7 @@ SOURCES = [
     ("kernels_eltwise.hip", DEVICE_FLAGS),
     ("kernels_sap.hip", DEVICE_FLAGS),
     ("kernels_rgsw.hip", DEVICE_FLAGS),
+    ("noalloc.hip", DEVICE_FLAGS),
     ("capi.cpp", HOST_FLAGS),
     ("keygen.cpp", HOST_FLAGS),
     ("serve.cpp", HOST_FLAGS),

@@ -227,6 +227,9 @@ int tp_context_create(tp_context** out, const tp_params* p, int device) {
     if (split_built(c)) c->split_max = 2 * (size_t)(prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 1);
     int rc = set_device(c);
     if (rc) { delete c; return rc; }
+    // device address of the no-allocation entry point's scratch array, once per device (its own
+    // error is reported by that entry point; this context works without it)
+    tp::noalloc_setup(device);
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
         return fail(TP_E_DEVICE, "stream creation failed");
@@ -632,6 +635,15 @@ int launch_chunk(tp_context* c, size_t count, const uint32_t* active, uint32_t* 
 }
 
 int set_error(int code, const char* msg) { return fail(code, msg); }
+int context_device(tp_context* c) { return c->device; }
+
+int bootstrap_into(tp_context* c, size_t count, const uint64_t* lwes, const uint64_t* tv, size_t tv_stride,
+                   uint64_t* ext, uint64_t* out) {
+    if (!c->d_ksk) return fail(TP_E_STATE, "key-switching key not imported");
+    int rc = launch_eval(c, count, lwes, tv, tv_stride, ext, nullptr, 0, 0);
+    if (rc) return rc;
+    return launch_ks_rows(c, count, ext, c->N + 1, nullptr, out);
+}
 hipStream_t context_stream(tp_context* c) { return c->stream; }
 void context_params(tp_context* c, tp_params* p) { *p = c->p; }
 int context_ready(tp_context* c) { return c->d_brk && c->d_ksk; }
@@ -649,9 +661,7 @@ int tp_bootstrap_batch_device(tp_context* c, size_t count, const uint64_t* lwes,
     if (rc) return rc;
     rc = ensure_scratch(c, count * (c->N + 1));
     if (rc) return rc;
-    rc = launch_eval(c, count, lwes, tv, tv_stride, c->d_scratch, nullptr, 0, 0);
-    if (rc) return rc;
-    return tp_lwe_keyswitch_batch_device(c, count, c->d_scratch, c->N + 1, out);
+    return tp::bootstrap_into(c, count, lwes, tv, tv_stride, c->d_scratch, out);
 }
 
 int tp_set_option(tp_context* c, int option, int value) {

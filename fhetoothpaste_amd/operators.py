@@ -312,6 +312,10 @@ class Context:
         check(lib().tp_bootstrap_batch_device(self.h, count, d_in.p, d_tv.p, stride, d_out.p))
         return d_out.to_host().reshape(count, W)
 
+    def bootstrap_batch_noalloc_device(self, count, d_lwes, d_tv, tv_stride, d_out):
+        """tp_bootstrap_batch_device_noalloc on device buffers (allocates nothing itself)."""
+        check(lib().tp_bootstrap_batch_device_noalloc(self.h, count, d_lwes.p, d_tv.p, tv_stride, d_out.p))
+
 
 # ---------------------------------------------------------------- key generation (host)
 

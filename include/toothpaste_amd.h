@@ -230,6 +230,16 @@ int tp_lwe_keyswitch_batch_device(tp_context* ctx, size_t count, const uint64_t*
 int tp_bootstrap_batch_device(tp_context* ctx, size_t count, const uint64_t* lwes, const uint64_t* tv,
                               size_t tv_stride, uint64_t* out);
 
+/* tp_bootstrap_batch_device without any device allocation.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph capture, allocation-free serving
+ * loops).  The extracted LWEs between the blind rotation and the key switch live in a 36 MiB array
+ * that every GPU the library runs on holds from load until the process ends; count * (N + 1) must
+ * not exceed 4,718,592 words (4,603 ciphertexts at N = 1024, 2,302 at 2048, 1,151 at 4096), else
+ * TP_E_INVALID.  All calls on one GPU share the array: calls that overlap on one GPU race.
+ * Output identical to tp_bootstrap_batch_device. */
+int tp_bootstrap_batch_device_noalloc(tp_context* ctx, size_t count, const uint64_t* lwes, const uint64_t* tv,
+                                      size_t tv_stride, uint64_t* out);
+
 /* Tuning (no counterpart in the reference).
  * TP_OPT_BR_WAVES selects the one-wave-per-ciphertext blind-rotation kernel: 2 (default; 256
  * registers per lane, two ciphertexts' waves share a SIMD) or 1 (512 registers, no spills in the
