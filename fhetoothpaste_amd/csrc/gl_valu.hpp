@@ -65,7 +65,7 @@ GV_FN uint64_t mad0(uint32_t a, uint32_t b) {
 }
 GV_FN uint64_t mad_eps_x(uint32_t a, uint64_t c) {  // a * eps + c, carry known to be zero
     uint64_t d;
-    asm volatile("v_mad_u64_u32 %0, vcc, %1, -1, %2" : "=v"(d) : "v"(a), "v"(c) : "vcc");
+    asm volatile("v_mad_u64_u32 %0, vcc, %1, -1, %2" : "=&v"(d) : "v"(a), "v"(c) : "vcc");
     return d;
 }
 // a * eps + c (eps = 2^32 - 1, the inline constant -1 as an unsigned 32-bit operand)
