@@ -361,9 +361,11 @@ def main():
     ctx.import_brk(brk)
     ctx.import_lwe_ksk(ksk)
     tune = ctx.autotune(args.batch)
-    log("blind-rotation variant: %d wave(s)/SIMD (two: %.1f ms, one: %.1f ms, digit-parallel: %.1f ms); "
-        "digit-parallel up to %d ciphertexts" % (tune["waves"], tune["ms_two_waves"], tune["ms_one_wave"],
-                                                 tune["ms_digit_parallel"], tune["split_max"]))
+    log("blind-rotation variant: engine %s, %d wave(s)/SIMD (NTT two: %.1f ms, one: %.1f ms, digit-parallel: %.1f ms; "
+        "floating point one: %.1f ms, two: %.1f ms); digit-parallel up to %d ciphertexts"
+        % ("floating point" if tune["engine"] else "NTT", tune["fft_waves"] if tune["engine"] else tune["waves"],
+           tune["ms_two_waves"], tune["ms_one_wave"], tune["ms_digit_parallel"], tune["ms_fft_one_wave"],
+           tune["ms_fft_two_waves"], tune["split_max"]))
 
     B = args.batch
     rng = np.random.default_rng(SEED)

@@ -22,6 +22,7 @@ HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall"]
 SOURCES = [
     ("kernels.hip", DEVICE_FLAGS),
     ("kernels_br1.hip", DEVICE_FLAGS),
+    ("kernels_br_fft.hip", DEVICE_FLAGS),
     ("kernels_br_lat.hip", DEVICE_FLAGS),
     ("kernels_pack.hip", DEVICE_FLAGS),
     ("kernels_eltwise.hip", DEVICE_FLAGS),
@@ -29,6 +30,7 @@ SOURCES = [
     ("kernels_rgsw.hip", DEVICE_FLAGS),
     ("noalloc.hip", DEVICE_FLAGS),
     ("capi.cpp", HOST_FLAGS),
+    ("fft_key.cpp", HOST_FLAGS),
     ("keygen.cpp", HOST_FLAGS),
     ("serve.cpp", HOST_FLAGS),
     ("wire.cpp", HOST_FLAGS),

@@ -11,6 +11,7 @@
 #include "../../include/toothpaste_amd.h"
 #include "gl64.hpp"
 #include "context_internal.hpp"
+#include "fft_key.hpp"
 #include "host_ntt.hpp"
 #include "kernels.hpp"
 
@@ -75,6 +76,13 @@ struct tp_context {
     uint64_t* d_rgsw = nullptr;       // LWE -> RGSW workspace
     size_t rgsw_words = 0;
     hipEvent_t ev[8] = {};
+    // exact floating-point external product (binary CGGI at N = 1024, kernels_br_fft.hip)
+    double* d_fkey = nullptr;    // transform-domain key limbs
+    double* d_zeta = nullptr;    // twiddle table
+    bool fft_ready = false;      // key imported and its rounding bound below 1/2
+    double fft_bound = -1.0;     // that bound (-1: not computed)
+    int br_engine = 1;           // 1: floating-point product where ready, 0: Goldilocks NTT
+    int fft_waves = 1;           // waves per SIMD of the floating-point kernel (1 or 2)
 };
 
 namespace {
@@ -150,9 +158,19 @@ void fill_br_args(tp_context* c, tp::BrArgs& A, const uint64_t* lwes, const uint
 // The digit-parallel kernel (2 l waves per ciphertext) is built for N = 1024 with 2..4 digits.
 bool split_built(const tp_context* c) { return c->logN == 10 && c->p.br_digits >= 2 && c->p.br_digits <= 4; }
 
+// Parameter sets the floating-point kernel is built for: binary CGGI, N = 1024, 2..4 digits (it is
+// used only when the rounding bound of the imported key is below 1/2 as well).
+bool fft_built(const tp_context* c) {
+    return c->logN == 10 && c->p.method == TP_METHOD_CGGI_BINARY && c->p.br_digits >= 2 && c->p.br_digits <= 4;
+}
+bool fft_active(const tp_context* c) { return c->br_engine == 1 && c->fft_ready; }
+
 hipError_t launch_br(tp_context* c, const tp::BrArgs& A, size_t count, int grid) {
     if (split_built(c) && count <= c->split_max)
         return tp::launch_blind_rotate_latency((int)c->p.method, (int)c->p.br_digits, A, grid, c->stream);
+    if (fft_active(c))
+        return tp::launch_blind_rotate_fft((int)c->p.br_digits, A, c->d_fkey, c->d_zeta, grid, c->fft_waves,
+                                           c->stream);
     return tp::launch_blind_rotate(c->logN, (int)c->p.method, (int)c->p.br_digits, A, grid, c->br_waves, c->stream);
 }
 
@@ -293,6 +311,8 @@ int tp_context_destroy(tp_context* c) {
     for (uint64_t* p : {c->d_tw, c->d_itw, c->d_brk, c->d_ksk, c->d_corr, c->d_scratch, c->d_io, c->d_mono,
                         c->d_auto, c->d_nodes, c->d_rksk, c->d_sapw, c->d_sapt, c->d_sap_polys, c->d_rgsw})
         if (p) (void)hipFree(p);
+    for (double* p : {c->d_fkey, c->d_zeta})
+        if (p) (void)hipFree(p);
     for (auto& ev : c->ev)
         if (ev) (void)hipEventDestroy(ev);
     if (c->d_perm) (void)hipFree(c->d_perm);
@@ -333,6 +353,27 @@ int tp_import_brk(tp_context* c, const uint64_t* brk, size_t words) {
     if (hipMalloc(&c->d_brk, words * 8) != hipSuccess) return fail(TP_E_NOMEM, "brk allocation failed");
     TP_HIP(hipMemcpy(c->d_brk, dev.data(), words * 8, hipMemcpyHostToDevice));
     c->brk_words = words;
+    // transform-domain key of the floating-point kernel, kept when its rounding bound certifies
+    // exact results for this key (any digits of the gadget)
+    c->fft_ready = false;
+    c->fft_bound = -1.0;
+    if (c->d_fkey) { TP_HIP(hipFree(c->d_fkey)); c->d_fkey = nullptr; }
+    if (fft_built(c)) {
+        double key_inf = 0.0;
+        const std::vector<double> fk = tp::fftk::device_key(brk, (int)c->p.lwe_dim, (int)c->p.br_digits, &key_inf);
+        c->fft_bound = tp::fftk::rounding_bound((int)c->p.br_base_log2, (int)c->p.br_digits, key_inf);
+        if (c->fft_bound < 0.45) {
+            if (!c->d_zeta) {
+                const std::vector<double> z = tp::fftk::zeta_table();
+                if (hipMalloc(&c->d_zeta, z.size() * 8) != hipSuccess) return fail(TP_E_NOMEM, "twiddle allocation failed");
+                TP_HIP(hipMemcpy(c->d_zeta, z.data(), z.size() * 8, hipMemcpyHostToDevice));
+            }
+            if (hipMalloc(&c->d_fkey, fk.size() * 8) != hipSuccess)
+                return fail(TP_E_NOMEM, "transform-domain key allocation failed");
+            TP_HIP(hipMemcpy(c->d_fkey, fk.data(), fk.size() * 8, hipMemcpyHostToDevice));
+            c->fft_ready = true;
+        }
+    }
     return TP_OK;
 }
 
@@ -680,6 +721,18 @@ int tp_set_option(tp_context* c, int option, int value) {
         c->split_max = (size_t)value;
         return TP_OK;
     }
+    if (option == TP_OPT_BR_ENGINE) {
+        if (value != 0 && value != 1) return fail(TP_E_INVALID, "engine must be 0 (NTT) or 1 (floating point)");
+        if (value == 1 && !fft_built(c))
+            return fail(TP_E_INVALID, "the floating-point product is built for binary CGGI at N = 1024 with 2..4 digits");
+        c->br_engine = value;
+        return TP_OK;
+    }
+    if (option == TP_OPT_BR_FFT_WAVES) {
+        if (value != 1 && value != 2) return fail(TP_E_INVALID, "waves per SIMD must be 1 or 2");
+        c->fft_waves = value;
+        return TP_OK;
+    }
     return fail(TP_E_INVALID, "unknown option");
 }
 
@@ -691,6 +744,14 @@ int tp_get_option(tp_context* c, int option, int* value) {
     }
     if (option == TP_OPT_BR_SPLIT_MAX) {
         *value = (int)std::min<size_t>(c->split_max, 0x7fffffff);
+        return TP_OK;
+    }
+    if (option == TP_OPT_BR_ENGINE) {
+        *value = fft_active(c) ? 1 : 0;
+        return TP_OK;
+    }
+    if (option == TP_OPT_BR_FFT_WAVES) {
+        *value = c->fft_waves;
         return TP_OK;
     }
     return fail(TP_E_INVALID, "unknown option");
@@ -725,13 +786,18 @@ int tp_autotune(tp_context* c, size_t count, float* ms) {
     if (!rc && (hipMemcpyAsync(d_lwe, lwes.data(), lwes.size() * 8, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
                 hipMemsetAsync(d_tv, 0, 2 * (size_t)N * 8, c->stream) != hipSuccess))
         rc = fail(TP_E_DEVICE, "autotune upload");
-    // candidates: br_kernel at two and at one wave per SIMD, then the digit-parallel kernel
-    float t_var[3] = {-1.0f, -1.0f, -1.0f};
-    for (int v = 0; v < 3 && !rc; ++v) {
+    // candidates: br_kernel at two and at one wave per SIMD, the digit-parallel kernel, then the
+    // floating-point kernel at one and at two waves per SIMD
+    const int saved_engine = c->br_engine, saved_fw = c->fft_waves;
+    float t_var[5] = {-1.0f, -1.0f, -1.0f, -1.0f, -1.0f};
+    for (int v = 0; v < 5 && !rc; ++v) {
         if (v == 1 && c->p.br_digits != 3 && c->p.br_digits != 4) continue;
         if (v == 2 && !split_built(c)) continue;
+        if (v >= 3 && !c->fft_ready) continue;
         c->br_waves = v == 1 ? 1 : 2;
         c->split_max = v == 2 ? count : 0;
+        c->br_engine = v >= 3 ? 1 : 0;
+        c->fft_waves = v == 4 ? 2 : 1;
         float t = -1.0f;
         rc = launch_eval(c, count, d_lwe, d_tv, 0, d_out, nullptr, 0, 0);  // warm-up
         if (!rc && (hipEventRecord(e0, c->stream) != hipSuccess))
@@ -745,15 +811,24 @@ int tp_autotune(tp_context* c, size_t count, float* ms) {
     if (rc) {
         c->br_waves = saved_w;
         c->split_max = saved_split;
+        c->br_engine = saved_engine;
+        c->fft_waves = saved_fw;
     } else {
         c->br_waves = (t_var[1] >= 0 && t_var[1] < t_var[0]) ? 1 : 2;
-        const float t_one = c->br_waves == 1 ? t_var[1] : t_var[0];
+        float t_one = c->br_waves == 1 ? t_var[1] : t_var[0];
+        c->br_engine = 0;
+        c->fft_waves = (t_var[4] >= 0 && t_var[4] < t_var[3]) ? 2 : 1;
+        const float t_fft = c->fft_waves == 2 ? t_var[4] : t_var[3];
+        if (t_fft >= 0 && t_fft < t_one) {
+            c->br_engine = 1;
+            t_one = t_fft;
+        }
         // the digit-parallel kernel wins at small counts: move the threshold to this count's side
         if (t_var[2] >= 0 && t_var[2] < t_one) c->split_max = std::max(saved_split, count);
         else c->split_max = std::min(saved_split, count - 1);
     }
     if (ms)
-        for (int v = 0; v < 3; ++v) ms[v] = t_var[v];
+        for (int v = 0; v < 5; ++v) ms[v] = t_var[v];
     (void)hipStreamSynchronize(c->stream);
     for (void* p : {(void*)d_lwe, (void*)d_tv, (void*)d_out})
         if (p) (void)hipFree(p);

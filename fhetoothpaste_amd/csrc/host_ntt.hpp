@@ -46,6 +46,23 @@ struct HostNtt {
                 }
             }
     }
+
+    // Inverse of forward(): bit-reversed order in, natural order out (Gentleman-Sande with
+    // psi_inv_rev, then the factor N^-1).
+    void inverse(uint64_t* a) const {
+        for (uint32_t m = N / 2, t = 1; m >= 1; m >>= 1, t <<= 1)
+            for (uint32_t i = 0; i < m; ++i) {
+                const uint64_t w = psi_inv_rev[m + i];
+                uint64_t* p = a + 2 * i * t;
+                for (uint32_t j = 0; j < t; ++j) {
+                    const uint64_t u = p[j], v = p[j + t];
+                    p[j] = gl::add(u, v);
+                    p[j + t] = gl::mul(gl::sub(u, v), w);
+                }
+            }
+        const uint64_t ninv = gl::inv(N);
+        for (uint32_t i = 0; i < N; ++i) a[i] = gl::mul(a[i], ninv);
+    }
 };
 
 }  // namespace tp

@@ -89,6 +89,10 @@ hipError_t launch_ntt_batch(int logN, uint64_t* out, const uint64_t* in, size_t 
 hipError_t launch_blind_rotate(int logN, int method, int digits, const BrArgs& A, int grid, int waves,
                                hipStream_t s);
 hipError_t launch_blind_rotate_w1(int logN, int method, int digits, const BrArgs& A, int grid, hipStream_t s);
+// Binary CGGI at N = 1024 with the exact floating-point external product (kernels_br_fft.hip):
+// fkey the transform-domain key limbs, zt the twiddle table (fft_key.hpp); waves 1 or 2 per SIMD.
+hipError_t launch_blind_rotate_fft(int digits, const BrArgs& A, const void* fkey, const void* zt, int grid, int waves,
+                                   hipStream_t s);
 // Digit-parallel variant at N = 1024 (digits 2..4): the 2 l digit products of a step run on 2 l
 // waves of one work-group per ciphertext; `grid` work-groups stride over A.count ciphertexts.
 hipError_t launch_blind_rotate_latency(int method, int digits, const BrArgs& A, int grid, hipStream_t s);

@@ -173,13 +173,31 @@ class Context:
 
     def autotune(self, count):
         """Time the blind-rotation variants on `count` synthetic ciphertexts (tp_autotune): keeps the
-        faster one-wave-per-ciphertext variant and moves the digit-parallel threshold to the side of
-        `count` that won; returns {"waves", "split_max", "ms_two_waves", "ms_one_wave",
-        "ms_digit_parallel"} (-1 for a variant that is not built)."""
-        ms = (ctypes.c_float * 3)()
+        fastest one-wave-per-ciphertext variant (engine and waves) and moves the digit-parallel
+        threshold to the side of `count` that won; returns {"waves", "split_max", "engine",
+        "fft_waves", "ms_two_waves", "ms_one_wave", "ms_digit_parallel", "ms_fft_one_wave",
+        "ms_fft_two_waves"} (-1 for a variant that is not built or not usable)."""
+        ms = (ctypes.c_float * 5)()
         check(lib().tp_autotune(self.h, count, ms))
-        return {"waves": self.br_waves(), "split_max": self.br_split_max(), "ms_two_waves": ms[0],
-                "ms_one_wave": ms[1], "ms_digit_parallel": ms[2]}
+        return {"waves": self.br_waves(), "split_max": self.br_split_max(), "engine": self.br_engine(),
+                "fft_waves": self._get(T.TP_OPT_BR_FFT_WAVES), "ms_two_waves": ms[0], "ms_one_wave": ms[1],
+                "ms_digit_parallel": ms[2], "ms_fft_one_wave": ms[3], "ms_fft_two_waves": ms[4]}
+
+    def _get(self, option):
+        v = ctypes.c_int()
+        check(lib().tp_get_option(self.h, option, ctypes.byref(v)))
+        return v.value
+
+    def set_br_engine(self, engine):
+        """1: exact floating-point external product (binary CGGI, N = 1024), 0: Goldilocks NTT."""
+        check(lib().tp_set_option(self.h, T.TP_OPT_BR_ENGINE, engine))
+
+    def br_engine(self):
+        """Engine the next launches use (1 only when the key's rounding bound certifies it)."""
+        return self._get(T.TP_OPT_BR_ENGINE)
+
+    def set_fft_waves(self, waves):
+        check(lib().tp_set_option(self.h, T.TP_OPT_BR_FFT_WAVES, waves))
 
     def set_br_waves(self, waves):
         check(lib().tp_set_option(self.h, T.TP_OPT_BR_WAVES, waves))
