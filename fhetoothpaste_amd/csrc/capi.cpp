@@ -81,7 +81,7 @@ struct tp_context {
     double* d_zeta = nullptr;    // twiddle table
     bool fft_ready = false;      // key imported and its rounding bound below 1/2
     double fft_bound = -1.0;     // that bound (-1: not computed)
-    int br_engine = 1;           // 1: floating-point product where ready, 0: Goldilocks NTT
+    int br_engine = 0;           // 0: Goldilocks NTT (default), 1: floating-point product where ready (opt-in)
     int fft_waves = 1;           // waves per SIMD of the floating-point kernel (1 or 2)
 };
 
@@ -788,12 +788,14 @@ int tp_autotune(tp_context* c, size_t count, float* ms) {
         rc = fail(TP_E_DEVICE, "autotune upload");
     // candidates: br_kernel at two and at one wave per SIMD, the digit-parallel kernel, then the
     // floating-point kernel at one and at two waves per SIMD
+    // the floating-point engine is a candidate only for a context that opted in to it
+    const bool fft_candidate = c->br_engine == 1 && c->fft_ready;
     const int saved_engine = c->br_engine, saved_fw = c->fft_waves;
     float t_var[5] = {-1.0f, -1.0f, -1.0f, -1.0f, -1.0f};
     for (int v = 0; v < 5 && !rc; ++v) {
         if (v == 1 && c->p.br_digits != 3 && c->p.br_digits != 4) continue;
         if (v == 2 && !split_built(c)) continue;
-        if (v >= 3 && !c->fft_ready) continue;
+        if (v >= 3 && !fft_candidate) continue;
         c->br_waves = v == 1 ? 1 : 2;
         c->split_max = v == 2 ? count : 0;
         c->br_engine = v >= 3 ? 1 : 0;

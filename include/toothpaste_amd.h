@@ -248,14 +248,16 @@ int tp_bootstrap_batch_device_noalloc(tp_context* ctx, size_t count, const uint6
  * instead (2 l waves per ciphertext, one work-group of them per CU; N = 1024 with 2..4 digits
  * only, 0 switches it off).  Default: twice the device's CU count for contexts it is built for,
  * else 0.  Both kernels give bit-identical results.
- * TP_OPT_BR_ENGINE (binary CGGI at N = 1024 with 2..4 digits): 1 (default) computes each step's
- * external product as an exact integer convolution in double-precision complex arithmetic (the
- * key split into four 16-bit limbs, kernels_br_fft.hip), 0 with the Goldilocks NTT.  Engine 1 is
- * used only when the rounding bound of the imported key certifies exact results (below 1/2 for
- * any digits of the gadget; otherwise tp_get_option reports 0); both engines give bit-identical
- * results.  TP_OPT_BR_FFT_WAVES: waves per SIMD of engine 1 (1: 512 registers, default; 2: 256).
+ * TP_OPT_BR_ENGINE (binary CGGI at N = 1024 with 2..4 digits): 0 (default) computes each step's
+ * external product with the Goldilocks NTT; 1 (opt-in) as an exact integer convolution in
+ * double-precision complex arithmetic (the key split into four 16-bit limbs, kernels_br_fft.hip).
+ * Engine 1 is used only when the rounding bound of the imported key certifies exact results
+ * (below 1/2 for any digits of the gadget; otherwise tp_get_option reports 0); both engines give
+ * bit-identical results.  TP_OPT_BR_FFT_WAVES: waves per SIMD of engine 1 (1: 512 registers,
+ * default; 2: 256).
  * tp_autotune times every variant built for the context on `count` synthetic ciphertexts with
- * the imported key, keeps the fastest one-wave-per-ciphertext variant (engine and waves), moves
+ * the imported key (the engine-1 variants only when the context opted in to engine 1), keeps
+ * the fastest one-wave-per-ciphertext variant (engine and waves), moves
  * the digit-parallel threshold to the side of `count` that won, and reports ms[0] (NTT engine,
  * two waves per SIMD), ms[1] (one wave), ms[2] (digit-parallel), ms[3] (floating-point engine,
  * one wave) and ms[4] (two waves), -1 where a variant is not built or not usable; ms must hold 5

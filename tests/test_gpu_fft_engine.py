@@ -44,6 +44,8 @@ def test_fft_engine_matches_oracle_and_ntt(tp, oracle, n, b, l):
     N = 1024
     say("fft engine", n, b, l)
     ctx, brk, lk, rk = _ctx(tp, n, b, l, 300 + n + b)
+    assert ctx.br_engine() == 0   # the Goldilocks NTT unless the caller opts in
+    ctx.set_br_engine(1)
     assert ctx.br_engine() == 1
     rng = np.random.default_rng(n * b)
     count = 7
@@ -88,6 +90,7 @@ def test_fft_engine_bootstrap_with_keyswitch(tp, oracle):
     ctx.import_brk(brk)
     ctx.import_lwe_ksk(ksk)
     ctx.set_br_split_max(0)
+    ctx.set_br_engine(1)
     assert ctx.br_engine() == 1
     lwes = oracle.random_mod_pow2(5 * (n + 1), 2 * N, 17, 18).reshape(5, n + 1)
     tv = np.zeros(2 * N, dtype=np.uint64)
