@@ -83,6 +83,7 @@ struct tp_context {
     double fft_bound = -1.0;     // that bound (-1: not computed)
     int br_engine = 0;           // 0: Goldilocks NTT (default), 1: floating-point product where ready (opt-in)
     int fft_waves = 1;           // waves per SIMD of the floating-point kernel (1 or 2)
+    int br_grid = 0;             // work-groups of a br_kernel launch: 0 one per ciphertext, else at most this many
 };
 
 namespace {
@@ -182,7 +183,11 @@ int launch_eval(tp_context* c, size_t count, const uint64_t* lwes, const uint64_
     if (tv_stride != 0 && tv_stride != 2 * (size_t)c->N) return fail(TP_E_INVALID, "tv_stride must be 0 or 2N");
     tp::BrArgs A{};
     fill_br_args(c, A, lwes, tv, tv_stride, out_lwe, out_rlwe, rlwe_ntt, count);
-    if (grid <= 0) grid = (int)count;
+    if (grid <= 0) {
+        grid = (int)count;
+        // a fixed grid (work-groups stride over the ciphertexts) for the one-work-group-per-ciphertext kernels
+        if (c->br_grid > 0 && !(split_built(c) && count <= c->split_max)) grid = std::min(grid, c->br_grid);
+    }
     hipError_t e = launch_br(c, A, count, grid);
     if (e != hipSuccess) return fail(TP_E_DEVICE, std::string("blind rotation launch: ") + hipGetErrorString(e));
     return TP_OK;
@@ -733,6 +738,11 @@ int tp_set_option(tp_context* c, int option, int value) {
         c->fft_waves = value;
         return TP_OK;
     }
+    if (option == TP_OPT_BR_GRID) {
+        if (value < 0) return fail(TP_E_INVALID, "the work-group count must be >= 0");
+        c->br_grid = value;
+        return TP_OK;
+    }
     return fail(TP_E_INVALID, "unknown option");
 }
 
@@ -752,6 +762,10 @@ int tp_get_option(tp_context* c, int option, int* value) {
     }
     if (option == TP_OPT_BR_FFT_WAVES) {
         *value = c->fft_waves;
+        return TP_OK;
+    }
+    if (option == TP_OPT_BR_GRID) {
+        *value = c->br_grid;
         return TP_OK;
     }
     return fail(TP_E_INVALID, "unknown option");

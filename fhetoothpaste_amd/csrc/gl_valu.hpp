@@ -56,16 +56,27 @@ GV_FN uint64_t mad(uint32_t a, uint32_t b, uint64_t c, lmask& co) {
     asm volatile("v_mad_u64_u32 %0, %1, %2, %3, %4" : "=v"(d), "=s"(co) : "v"(a), "v"(b), "v"(c));
     return d;
 }
-// dead carry-outs go to vcc (a clobber: consecutive writes need no wait state, unlike an SGPR
-// pair the compiler would hand out again and again)
-GV_FN uint64_t mad0(uint32_t a, uint32_t b) {
+// Dead carry-outs rotate over four sink registers (sink(m) for member m): consecutive writes of
+// one register by inline asm make the compiler insert a wait state between them.
+#define GV_SINK_SWITCH(m, BODY) \
+    switch ((m) & 3) { \
+        case 0: BODY("s[96:97]", "s96", "s97"); break; \
+        case 1: BODY("s[98:99]", "s98", "s99"); break; \
+        case 2: BODY("s[100:101]", "s100", "s101"); break; \
+        default: BODY("vcc", "vcc_lo", "vcc_hi"); break; \
+    }
+GV_FN uint64_t mad0(uint32_t a, uint32_t b, int m = 3) {
     uint64_t d;
-    asm volatile("v_mad_u64_u32 %0, vcc, %1, %2, 0" : "=v"(d) : "v"(a), "v"(b) : "vcc");
+#define B_(R, C1, C2) asm volatile("v_mad_u64_u32 %0, " R ", %1, %2, 0" : "=v"(d) : "v"(a), "v"(b) : C1, C2)
+    GV_SINK_SWITCH(m, B_)
+#undef B_
     return d;
 }
-GV_FN uint64_t mad_eps_x(uint32_t a, uint64_t c) {  // a * eps + c, carry known to be zero
+GV_FN uint64_t mad_eps_x(uint32_t a, uint64_t c, int m = 3) {  // a * eps + c, carry known to be zero
     uint64_t d;
-    asm volatile("v_mad_u64_u32 %0, vcc, %1, -1, %2" : "=&v"(d) : "v"(a), "v"(c) : "vcc");
+#define B_(R, C1, C2) asm volatile("v_mad_u64_u32 %0, " R ", %1, -1, %2" : "=&v"(d) : "v"(a), "v"(c) : C1, C2)
+    GV_SINK_SWITCH(m, B_)
+#undef B_
     return d;
 }
 // a * eps + c (eps = 2^32 - 1, the inline constant -1 as an unsigned 32-bit operand)
@@ -74,9 +85,11 @@ GV_FN uint64_t mad_eps(uint32_t a, uint64_t c, lmask& co) {
     asm volatile("v_mad_u64_u32 %0, %1, %2, -1, %3" : "=v"(d), "=s"(co) : "v"(a), "v"(c));
     return d;
 }
-GV_FN uint64_t mad_eps0(uint32_t a) {
+GV_FN uint64_t mad_eps0(uint32_t a, int m = 3) {
     uint64_t d;
-    asm volatile("v_mad_u64_u32 %0, vcc, %1, -1, 0" : "=v"(d) : "v"(a) : "vcc");
+#define B_(R, C1, C2) asm volatile("v_mad_u64_u32 %0, " R ", %1, -1, 0" : "=v"(d) : "v"(a) : C1, C2)
+    GV_SINK_SWITCH(m, B_)
+#undef B_
     return d;
 }
 GV_FN uint32_t add_co(uint32_t a, uint32_t b, lmask& co) {
@@ -89,9 +102,11 @@ GV_FN uint32_t addc(uint32_t a, uint32_t b, lmask ci, lmask& co) {
     asm volatile("v_addc_co_u32 %0, %1, %2, %3, %4" : "=v"(d), "=s"(co) : "v"(a), "v"(b), "s"(ci));
     return d;
 }
-GV_FN uint32_t addc0(uint32_t a, lmask ci) {  // a + ci
+GV_FN uint32_t addc0(uint32_t a, lmask ci, int m = 3) {  // a + ci
     uint32_t d;
-    asm volatile("v_addc_co_u32 %0, vcc, %1, 0, %2" : "=v"(d) : "v"(a), "s"(ci) : "vcc");
+#define B_(R, C1, C2) asm volatile("v_addc_co_u32 %0, " R ", %1, 0, %2" : "=v"(d) : "v"(a), "s"(ci) : C1, C2)
+    GV_SINK_SWITCH(m, B_)
+#undef B_
     return d;
 }
 GV_FN uint32_t addc0c(uint32_t a, lmask ci, lmask& co) {  // a + ci, with its carry-out
@@ -99,9 +114,11 @@ GV_FN uint32_t addc0c(uint32_t a, lmask ci, lmask& co) {  // a + ci, with its ca
     asm volatile("v_addc_co_u32 %0, %1, %2, 0, %3" : "=v"(d), "=s"(co) : "v"(a), "s"(ci));
     return d;
 }
-GV_FN uint32_t subb0x(uint32_t a, lmask bi) {  // a - bi, borrow-out dead
+GV_FN uint32_t subb0x(uint32_t a, lmask bi, int m = 3) {  // a - bi, borrow-out dead
     uint32_t d;
-    asm volatile("v_subb_co_u32 %0, vcc, %1, 0, %2" : "=v"(d) : "v"(a), "s"(bi) : "vcc");
+#define B_(R, C1, C2) asm volatile("v_subb_co_u32 %0, " R ", %1, 0, %2" : "=v"(d) : "v"(a), "s"(bi) : C1, C2)
+    GV_SINK_SWITCH(m, B_)
+#undef B_
     return d;
 }
 GV_FN uint32_t sub_co(uint32_t a, uint32_t b, lmask& bo) {
@@ -183,10 +200,10 @@ GV_FN uint64_t mad(uint32_t a, uint32_t b, uint64_t c, lmask& co) {
     co = (bool)(t >> 64);
     return (uint64_t)t;
 }
-GV_FN uint64_t mad0(uint32_t a, uint32_t b) { return (uint64_t)a * b; }
+GV_FN uint64_t mad0(uint32_t a, uint32_t b, int = 0) { return (uint64_t)a * b; }
 GV_FN uint64_t mad_eps(uint32_t a, uint64_t c, lmask& co) { return mad(a, 0xFFFFFFFFu, c, co); }
-GV_FN uint64_t mad_eps0(uint32_t a) { return (uint64_t)a * 0xFFFFFFFFu; }
-GV_FN uint64_t mad_eps_x(uint32_t a, uint64_t c) { return (uint64_t)a * 0xFFFFFFFFu + c; }
+GV_FN uint64_t mad_eps0(uint32_t a, int = 0) { return (uint64_t)a * 0xFFFFFFFFu; }
+GV_FN uint64_t mad_eps_x(uint32_t a, uint64_t c, int = 0) { return (uint64_t)a * 0xFFFFFFFFu + c; }
 GV_FN uint32_t add_co(uint32_t a, uint32_t b, lmask& co) {
     const uint64_t t = (uint64_t)a + b;
     co = (bool)(t >> 32);
@@ -197,7 +214,7 @@ GV_FN uint32_t addc(uint32_t a, uint32_t b, lmask ci, lmask& co) {
     co = (bool)(t >> 32);
     return (uint32_t)t;
 }
-GV_FN uint32_t addc0(uint32_t a, lmask ci) { return a + (ci ? 1u : 0u); }
+GV_FN uint32_t addc0(uint32_t a, lmask ci, int = 0) { return a + (ci ? 1u : 0u); }
 GV_FN uint32_t addc0c(uint32_t a, lmask ci, lmask& co) { return addc(a, 0, ci, co); }
 GV_FN uint32_t sub_co(uint32_t a, uint32_t b, lmask& bo) {
     bo = a < b;
@@ -209,7 +226,7 @@ GV_FN uint32_t subb(uint32_t a, uint32_t b, lmask bi, lmask& bo) {
     return (uint32_t)((uint64_t)a - s);
 }
 GV_FN uint32_t subb0(uint32_t a, lmask bi, lmask& bo) { return subb(a, 0, bi, bo); }
-GV_FN uint32_t subb0x(uint32_t a, lmask bi) { return a - (bi ? 1u : 0u); }
+GV_FN uint32_t subb0x(uint32_t a, lmask bi, int = 0) { return a - (bi ? 1u : 0u); }
 GV_FN uint32_t cnd(uint32_t f, uint32_t t, lmask m) { return m ? t : f; }
 GV_FN uint32_t cnd01(lmask m) { return m ? 1u : 0u; }
 GV_FN uint32_t cnd0f(lmask m) { return m ? 0xFFFFFFFFu : 0u; }
@@ -244,15 +261,15 @@ template <int M>
 GV_FN void mul128_g(const uint64_t (&a)[M], const uint64_t (&b)[M], uint64_t (&lo)[M], uint64_t (&hi)[M]) {
     uint64_t X[M], L[M], H[M];
     lmask cX[M], k[M];
-    GV_FOR(m) X[m] = mad0(hi32(a[m]), lo32(b[m]));
-    GV_FOR(m) L[m] = mad0(lo32(a[m]), lo32(b[m]));
+    GV_FOR(m) X[m] = mad0(hi32(a[m]), lo32(b[m]), m);
+    GV_FOR(m) L[m] = mad0(lo32(a[m]), lo32(b[m]), m);
     GV_FOR(m) X[m] = mad(lo32(a[m]), hi32(b[m]), X[m], cX[m]);
-    GV_FOR(m) H[m] = mad0(hi32(a[m]), hi32(b[m]));
+    GV_FOR(m) H[m] = mad0(hi32(a[m]), hi32(b[m]), m);
     uint32_t l1[M], h0[M], h1[M];
     GV_FOR(m) l1[m] = add_co(hi32(L[m]), lo32(X[m]), k[m]);
     GV_FOR(m) h0[m] = addc(lo32(H[m]), hi32(X[m]), k[m], k[m]);
-    GV_FOR(m) h1[m] = addc0(hi32(H[m]), k[m]);
-    GV_FOR(m) h1[m] = addc0(h1[m], cX[m]);
+    GV_FOR(m) h1[m] = addc0(hi32(H[m]), k[m], m);
+    GV_FOR(m) h1[m] = addc0(h1[m], cX[m], m);
     GV_FOR(m) {
         lo[m] = pk(lo32(L[m]), l1[m]);
         hi[m] = pk(h0[m], h1[m]);
@@ -310,7 +327,7 @@ GV_FN void add_lc_g(const uint64_t (&u)[M], const uint64_t (&t)[M], uint64_t (&r
     GV_FOR(m) s0[m] = add_co(lo32(u[m]), lo32(t[m]), k[m]);
     GV_FOR(m) s1[m] = addc(hi32(u[m]), hi32(t[m]), k[m], c[m]);
     GV_FOR(m) e[m] = cnd01(c[m]);
-    GV_FOR(m) r[m] = mad_eps_x(e[m], pk(s0[m], s1[m]));
+    GV_FOR(m) r[m] = mad_eps_x(e[m], pk(s0[m], s1[m]), m);
 }
 
 // lazy u - canonical t -> lazy (5 VALU).  A borrow means d = u - t + 2^64 >= 2^64 - Q + 1 = eps,
@@ -324,7 +341,7 @@ GV_FN void sub_lc_g(const uint64_t (&u)[M], const uint64_t (&t)[M], uint64_t (&r
     GV_FOR(m) d1[m] = subb(hi32(u[m]), hi32(t[m]), k[m], b[m]);
     GV_FOR(m) e[m] = cnd0f(b[m]);
     GV_FOR(m) d0[m] = sub_co(d0[m], e[m], x[m]);
-    GV_FOR(m) d1[m] = subb0x(d1[m], x[m]);
+    GV_FOR(m) d1[m] = subb0x(d1[m], x[m], m);
     GV_FOR(m) r[m] = pk(d0[m], d1[m]);
 }
 
@@ -338,7 +355,7 @@ GV_FN void add_cc_g(const uint64_t (&u)[M], const uint64_t (&t)[M], uint64_t (&r
     GV_FOR(m) s1[m] = addc(hi32(u[m]), hi32(t[m]), k[m], c[m]);
     GV_FOR(m) ge[m] = ge_q(pk(s0[m], s1[m]), qm1);
     GV_FOR(m) e[m] = cnd01(m_or(c[m], ge[m]));
-    GV_FOR(m) r[m] = mad_eps_x(e[m], pk(s0[m], s1[m]));
+    GV_FOR(m) r[m] = mad_eps_x(e[m], pk(s0[m], s1[m]), m);
 }
 
 // Lazy -> canonical (3 VALU): x >= Q becomes x + eps (mod 2^64) = x - Q.
@@ -348,7 +365,7 @@ GV_FN void canon_g(uint64_t (&x)[M], uint64_t qm1) {
     uint32_t e[M];
     GV_FOR(m) ge[m] = ge_q(x[m], qm1);
     GV_FOR(m) e[m] = cnd01(ge[m]);
-    GV_FOR(m) x[m] = mad_eps_x(e[m], x[m]);
+    GV_FOR(m) x[m] = mad_eps_x(e[m], x[m], m);
 }
 
 // ------------------------------------------------------------------ products by 2^e
@@ -370,7 +387,7 @@ GV_FN void mul_2e_lo_g(const uint64_t (&x)[M], const int (&e)[M], uint64_t (&r)[
     GV_FOR(m) R[m] = mad_eps(h[m], lo[m], c[m]);
     GV_FOR(m) ge[m] = ge_q(R[m], qm1);
     GV_FOR(m) f[m] = cnd01(m_or(c[m], ge[m]));
-    GV_FOR(m) r[m] = mad_eps_x(f[m], R[m]);
+    GV_FOR(m) r[m] = mad_eps_x(f[m], R[m], m);
 }
 
 // 32 < e < 64 (10 VALU): the 128-bit x << e, reduced.
@@ -393,12 +410,12 @@ GV_FN void mul_2e_neg_g(const uint64_t (&x)[M], const int (&s)[M], uint64_t (&r)
     lmask k[M], b[M], z[M];
     GV_FOR(m) q[m] = shr64(x[m], s[m]);
     GV_FOR(m) v[m] = shl32(lo32(x[m]), 32 - s[m]);
-    GV_FOR(m) ve[m] = mad_eps0(v[m]);
+    GV_FOR(m) ve[m] = mad_eps0(v[m], m);
     GV_FOR(m) d0[m] = sub_co(lo32(q[m]), lo32(ve[m]), k[m]);
     GV_FOR(m) d1[m] = subb(hi32(q[m]), hi32(ve[m]), k[m], b[m]);
     GV_FOR(m) e[m] = cnd0f(b[m]);
     GV_FOR(m) d0[m] = sub_co(d0[m], e[m], z[m]);
-    GV_FOR(m) d1[m] = subb0x(d1[m], z[m]);
+    GV_FOR(m) d1[m] = subb0x(d1[m], z[m], m);
     GV_FOR(m) r[m] = pk(d0[m], d1[m]);
 }
 
@@ -496,20 +513,20 @@ GV_FN void acc_mac_g(Acc (&acc)[M], const uint64_t (&a)[M], const uint64_t (&b)[
     uint64_t X[M], L[M], H[M], R[M];
     lmask cX[M], cL[M], k[M], c1[M], bb[M];
     uint32_t l1[M], h0[M], h1[M], r0[M], r1[M], C[M];
-    GV_FOR(m) X[m] = mad0(hi32(a[m]), lo32(b[m]));
+    GV_FOR(m) X[m] = mad0(hi32(a[m]), lo32(b[m]), m);
     GV_FOR(m) L[m] = mad(lo32(a[m]), lo32(b[m]), acc[m].A, cL[m]);
     GV_FOR(m) X[m] = mad(lo32(a[m]), hi32(b[m]), X[m], cX[m]);
-    GV_FOR(m) H[m] = mad0(hi32(a[m]), hi32(b[m]));
+    GV_FOR(m) H[m] = mad0(hi32(a[m]), hi32(b[m]), m);
     GV_FOR(m) l1[m] = add_co(hi32(L[m]), lo32(X[m]), k[m]);
     GV_FOR(m) h0[m] = addc(lo32(H[m]), hi32(X[m]), k[m], k[m]);
-    GV_FOR(m) h1[m] = addc0(hi32(H[m]), k[m]);
-    GV_FOR(m) h1[m] = addc0(h1[m], cX[m]);
+    GV_FOR(m) h1[m] = addc0(hi32(H[m]), k[m], m);
+    GV_FOR(m) h1[m] = addc0(h1[m], cX[m], m);
     GV_FOR(m) R[m] = mad_eps(h0[m], pk(lo32(L[m]), l1[m]), c1[m]);
     GV_FOR(m) r0[m] = sub_co(lo32(R[m]), h1[m], bb[m]);
     GV_FOR(m) r1[m] = subb0(hi32(R[m]), bb[m], bb[m]);
-    GV_FOR(m) C[m] = addc0(acc[m].C, cL[m]);
-    GV_FOR(m) C[m] = addc0(C[m], c1[m]);
-    GV_FOR(m) C[m] = subb0x(C[m], bb[m]);
+    GV_FOR(m) C[m] = addc0(acc[m].C, cL[m], m);
+    GV_FOR(m) C[m] = addc0(C[m], c1[m], m);
+    GV_FOR(m) C[m] = subb0x(C[m], bb[m], m);
     GV_FOR(m) {
         acc[m].A = pk(r0[m], r1[m]);
         acc[m].C = C[m];
@@ -525,7 +542,7 @@ GV_FN void acc_value_g(const Acc (&acc)[M], uint64_t (&v)[M], uint64_t qm1) {
     GV_FOR(m) R[m] = mad_eps(acc[m].C, acc[m].A, c[m]);
     GV_FOR(m) ge[m] = ge_q(R[m], qm1);
     GV_FOR(m) f[m] = cnd01(m_or(c[m], ge[m]));
-    GV_FOR(m) v[m] = mad_eps_x(f[m], R[m]);
+    GV_FOR(m) v[m] = mad_eps_x(f[m], R[m], m);
 }
 
 // Two-word accumulator (16 VALU per term): acc (lazy) + a b -> lazy.  The addend rides in the
@@ -538,15 +555,15 @@ GV_FN void mac_lazy_g(uint64_t (&acc)[M], const uint64_t (&a)[M], const uint64_t
     uint64_t X[M], L[M], H[M], R[M];
     lmask cX[M], cL[M], c2[M], k[M], c1[M], bb[M];
     uint32_t x1[M], l1[M], h0[M], h1[M], r0[M], r1[M], t[M], dlo[M], dhi[M];
-    GV_FOR(m) X[m] = mad0(hi32(a[m]), lo32(b[m]));
+    GV_FOR(m) X[m] = mad0(hi32(a[m]), lo32(b[m]), m);
     GV_FOR(m) L[m] = mad(lo32(a[m]), lo32(b[m]), acc[m], cL[m]);
     GV_FOR(m) X[m] = mad(lo32(a[m]), hi32(b[m]), X[m], cX[m]);
-    GV_FOR(m) H[m] = mad0(hi32(a[m]), hi32(b[m]));
+    GV_FOR(m) H[m] = mad0(hi32(a[m]), hi32(b[m]), m);
     GV_FOR(m) x1[m] = addc0c(hi32(X[m]), cL[m], c2[m]);
     GV_FOR(m) l1[m] = add_co(hi32(L[m]), lo32(X[m]), k[m]);
     GV_FOR(m) h0[m] = addc(lo32(H[m]), x1[m], k[m], k[m]);
-    GV_FOR(m) h1[m] = addc0(hi32(H[m]), k[m]);
-    GV_FOR(m) h1[m] = addc0(h1[m], m_or(cX[m], c2[m]));
+    GV_FOR(m) h1[m] = addc0(hi32(H[m]), k[m], m);
+    GV_FOR(m) h1[m] = addc0(h1[m], m_or(cX[m], c2[m]), m);
     GV_FOR(m) R[m] = mad_eps(h0[m], pk(lo32(L[m]), l1[m]), c1[m]);
     GV_FOR(m) r0[m] = sub_co(lo32(R[m]), h1[m], bb[m]);
     GV_FOR(m) r1[m] = subb0(hi32(R[m]), bb[m], bb[m]);

@@ -266,6 +266,10 @@ int tp_bootstrap_batch_device_noalloc(tp_context* ctx, size_t count, const uint6
 #define TP_OPT_BR_SPLIT_MAX 2
 #define TP_OPT_BR_ENGINE 3
 #define TP_OPT_BR_FFT_WAVES 4
+/* TP_OPT_BR_GRID: work-groups per launch of the one-work-group-per-ciphertext blind-rotation
+ * kernels; 0 (default) launches one per ciphertext, a positive value at most that many, each
+ * striding over the launch's ciphertexts (results are the same either way). */
+#define TP_OPT_BR_GRID 5
 int tp_set_option(tp_context* ctx, int option, int value);
 int tp_get_option(tp_context* ctx, int option, int* value);
 int tp_autotune(tp_context* ctx, size_t count, float* ms);
