@@ -329,3 +329,61 @@ def sap_decompose(N, lwe, radix, beta, brk, n, bl, ll, method, keys, b, d, max_d
                                  bl, ll, method, P(keys), b, d, r, P(k), kb, kd)
     assert cnt >= 0
     return out[:cnt * (N + 1)].reshape(cnt, N + 1)
+
+
+# ---------------------------------------------------------------- vectorised CPU port (cpu_port.cpp)
+_port = None
+_PORT_PATH = os.path.join(_HERE, "libcpuport.so")
+
+
+def port():
+    """ctypes view of libcpuport.so, the AVX2 CPU port timed as the CPU baseline (None if the CPU
+    lacks AVX2)."""
+    global _port
+    if _port is None:
+        if not os.path.exists(_PORT_PATH):
+            build()
+        L = ctypes.CDLL(_PORT_PATH)
+        L.cp_available.restype = ctypes.c_int
+        L.cp_available.argtypes = []
+        L.cp_ntt.restype = None
+        L.cp_ntt.argtypes = [U32, u64p, ctypes.c_size_t, ctypes.c_int]
+        L.cp_ntt_time.restype = None
+        L.cp_ntt_time.argtypes = [U32, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+        L.cp_bootstrap_batch.restype = ctypes.c_double
+        L.cp_bootstrap_batch.argtypes = [U32, u64p, u64p, ctypes.c_size_t, u64p, u64p, U32, U32, U32, u64p, U32, U32,
+                                         U32, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+        _port = L
+    return _port if _port.cp_available() else None
+
+
+def port_ntt(N, polys, inverse=False):
+    """Forward (or inverse) transforms of the rows of polys (count x N) with the CPU port."""
+    a = np.ascontiguousarray(polys, dtype=np.uint64).reshape(-1, N).copy()
+    port().cp_ntt(N, P(a), a.shape[0], 1 if inverse else 0)
+    return a
+
+
+def port_ntt_time(N, reps):
+    """Seconds per forward and per inverse transform of N words (one thread)."""
+    f, i = ctypes.c_double(), ctypes.c_double()
+    port().cp_ntt_time(N, reps, ctypes.byref(f), ctypes.byref(i))
+    return f.value, i.value
+
+
+def port_bootstrap_batch(N, lwes, tv, brk_bytes, n, b, l, ksk=None, n_out=0, ks_b=0, ks_d=0, threads=1,
+                         per_ct=False):
+    """Binary-key bootstrap batch on the CPU port: (outputs, seconds[, per-ciphertext seconds])."""
+    lwes = np.ascontiguousarray(lwes, dtype=np.uint64).reshape(-1)
+    count = lwes.size // (n + 1)
+    width = (n_out + 1) if ksk is not None else (N + 1)
+    out = arr(count * width)
+    times = np.zeros(count, dtype=np.float64) if per_ct else None
+    tp = times.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) if per_ct else None
+    secs = port().cp_bootstrap_batch(N, P(out), P(lwes), count, P(np.ascontiguousarray(tv, dtype=np.uint64)),
+                                     P(brk_bytes), n, b, l, P(ksk), n_out, ks_b, ks_d, threads, tp)
+    if secs < 0:
+        raise ValueError("cp_bootstrap_batch rejected its arguments")
+    if per_ct:
+        return out.reshape(count, width), secs, times
+    return out.reshape(count, width), secs
