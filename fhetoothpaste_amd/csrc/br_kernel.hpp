@@ -223,8 +223,25 @@ br_kernel(BrArgs A) {
                                 g0[m] = out[0][2 * c0 + m];
                                 g1[m] = out[1][2 * c0 + m];
                             }
-                            glv::mac_lazy_g<4>(g0, dv, kav);
-                            glv::mac_lazy_g<4>(g1, dv, kbv);
+                            {
+                                // both key rows as one group of eight members (each carry is read
+                                // eight instructions after it is written)
+                                uint64_t g8[8], d8[8], k8[8];
+#pragma unroll
+                                for (int m = 0; m < 4; ++m) {
+                                    g8[m] = g0[m];
+                                    g8[m + 4] = g1[m];
+                                    d8[m] = d8[m + 4] = dv[m];
+                                    k8[m] = kav[m];
+                                    k8[m + 4] = kbv[m];
+                                }
+                                glv::mac_lazy_g<8>(g8, d8, k8);
+#pragma unroll
+                                for (int m = 0; m < 4; ++m) {
+                                    g0[m] = g8[m];
+                                    g1[m] = g8[m + 4];
+                                }
+                            }
 #pragma unroll
                             for (int m = 0; m < 4; ++m) {
                                 out[0][2 * c0 + m] = g0[m];

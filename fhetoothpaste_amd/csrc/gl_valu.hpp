@@ -60,9 +60,9 @@ GV_FN uint64_t mad(uint32_t a, uint32_t b, uint64_t c, lmask& co) {
 // one register by inline asm make the compiler insert a wait state between them.
 #define GV_SINK_SWITCH(m, BODY) \
     switch ((m) & 3) { \
-        case 0: BODY("s[96:97]", "s96", "s97"); break; \
-        case 1: BODY("s[98:99]", "s98", "s99"); break; \
-        case 2: BODY("s[100:101]", "s100", "s101"); break; \
+        case 0: BODY("s[94:95]", "s94", "s95"); break; \
+        case 1: BODY("s[96:97]", "s96", "s97"); break; \
+        case 2: BODY("s[98:99]", "s98", "s99"); break; \
         default: BODY("vcc", "vcc_lo", "vcc_hi"); break; \
     }
 GV_FN uint64_t mad0(uint32_t a, uint32_t b, int m = 3) {

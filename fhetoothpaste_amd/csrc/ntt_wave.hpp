@@ -34,6 +34,10 @@ __device__ __forceinline__ void static_for(F&& f) {
 }
 
 constexpr int kE = 16;  // coefficients per lane
+// Butterflies per instruction group: the eight of a level's lane share each op, so a carry is read
+// eight instructions after it is written (the compiler adds a wait state between an inline-asm
+// carry write and a read fewer than five instructions later).
+constexpr int kGW = 8;
 
 template <int LOGN>
 struct Shape {
@@ -275,17 +279,17 @@ __device__ __forceinline__ void pow2_products(const uint64_t (&v)[8], uint64_t (
 // product is -t[b], so the sum and difference trade places); two groups of 4.
 template <int H, class NEG>
 __device__ __forceinline__ void ct_outputs(uint64_t (&x)[kE], const uint64_t (&t)[8]) {
-    static_for<2>([&](auto G) __attribute__((always_inline)) {
-        uint64_t u[4], tt[4], s_[4], d_[4];
-        static_for<4>([&](auto M_) __attribute__((always_inline)) {
-            constexpr int b = 4 * G + M_;
+    static_for<8 / kGW>([&](auto G) __attribute__((always_inline)) {
+        uint64_t u[kGW], tt[kGW], s_[kGW], d_[kGW];
+        static_for<kGW>([&](auto M_) __attribute__((always_inline)) {
+            constexpr int b = kGW * G + M_;
             u[M_] = x[bf_u(H, b)];
             tt[M_] = t[b];
         });
-        glv::add_lc_g<4>(u, tt, s_);
-        glv::sub_lc_g<4>(u, tt, d_);
-        static_for<4>([&](auto M_) __attribute__((always_inline)) {
-            constexpr int b = 4 * G + M_;
+        glv::add_lc_g<kGW>(u, tt, s_);
+        glv::sub_lc_g<kGW>(u, tt, d_);
+        static_for<kGW>([&](auto M_) __attribute__((always_inline)) {
+            constexpr int b = kGW * G + M_;
             constexpr bool ng = NEG::neg(b);
             x[bf_u(H, b)] = ng ? d_[M_] : s_[M_];
             x[bf_u(H, b) + H] = ng ? s_[M_] : d_[M_];
@@ -323,15 +327,15 @@ template <int H, int SS>
 __device__ __forceinline__ void ct_level_tw(uint64_t (&x)[kE], const uint64_t (&w)[8]) {
     const uint64_t qm1 = glv::qm1_const();
     uint64_t t[8];
-    static_for<2>([&](auto G) __attribute__((always_inline)) {
-        uint64_t v[4], ww[4], r[4];
-        static_for<4>([&](auto M_) __attribute__((always_inline)) {
-            constexpr int b = 4 * G + M_;
+    static_for<8 / kGW>([&](auto G) __attribute__((always_inline)) {
+        uint64_t v[kGW], ww[kGW], r[kGW];
+        static_for<kGW>([&](auto M_) __attribute__((always_inline)) {
+            constexpr int b = kGW * G + M_;
             v[M_] = x[bf_u(H, b) + H];
             ww[M_] = w[bf_u(H, b) >> SS];
         });
-        glv::mul_canon_g<4>(v, ww, r, qm1);
-        static_for<4>([&](auto M_) __attribute__((always_inline)) { t[4 * G + M_] = r[M_]; });
+        glv::mul_canon_g<kGW>(v, ww, r, qm1);
+        static_for<kGW>([&](auto M_) __attribute__((always_inline)) { t[kGW * G + M_] = r[M_]; });
     });
     ct_outputs<H, NoNeg>(x, t);
 }
@@ -391,19 +395,19 @@ __device__ __forceinline__ void ntt_forward(uint64_t (&x)[kE], uint64_t* lds, co
 // swapped where the product by the twiddle is negative) goes to d[b] for the caller's products.
 template <int H, class NEG>
 __device__ __forceinline__ void gs_sums_diffs(uint64_t (&x)[kE], uint64_t (&d)[8], uint64_t qm1) {
-    static_for<2>([&](auto G) __attribute__((always_inline)) {
-        uint64_t u[4], v[4], s_[4], a_[4], b_[4], r[4];
-        static_for<4>([&](auto M_) __attribute__((always_inline)) {
-            constexpr int b = 4 * G + M_;
+    static_for<8 / kGW>([&](auto G) __attribute__((always_inline)) {
+        uint64_t u[kGW], v[kGW], s_[kGW], a_[kGW], b_[kGW], r[kGW];
+        static_for<kGW>([&](auto M_) __attribute__((always_inline)) {
+            constexpr int b = kGW * G + M_;
             u[M_] = x[bf_u(H, b)];
             v[M_] = x[bf_u(H, b) + H];
             a_[M_] = NEG::neg(b) ? v[M_] : u[M_];
             b_[M_] = NEG::neg(b) ? u[M_] : v[M_];
         });
-        glv::add_cc_g<4>(u, v, s_, qm1);
-        glv::sub_lc_g<4>(a_, b_, r);
-        static_for<4>([&](auto M_) __attribute__((always_inline)) {
-            constexpr int b = 4 * G + M_;
+        glv::add_cc_g<kGW>(u, v, s_, qm1);
+        glv::sub_lc_g<kGW>(a_, b_, r);
+        static_for<kGW>([&](auto M_) __attribute__((always_inline)) {
+            constexpr int b = kGW * G + M_;
             x[bf_u(H, b)] = s_[M_];
             d[b] = r[M_];
         });
@@ -462,15 +466,15 @@ __device__ __forceinline__ void gs_level_tw(uint64_t (&x)[kE], const uint64_t (&
     const uint64_t qm1 = glv::qm1_const();
     uint64_t d[8];
     gs_sums_diffs<H, NoNeg>(x, d, qm1);
-    static_for<2>([&](auto G) __attribute__((always_inline)) {
-        uint64_t dd[4], ww[4], r[4];
-        static_for<4>([&](auto M_) __attribute__((always_inline)) {
-            constexpr int b = 4 * G + M_;
+    static_for<8 / kGW>([&](auto G) __attribute__((always_inline)) {
+        uint64_t dd[kGW], ww[kGW], r[kGW];
+        static_for<kGW>([&](auto M_) __attribute__((always_inline)) {
+            constexpr int b = kGW * G + M_;
             dd[M_] = d[b];
             ww[M_] = w[bf_u(H, b) >> SS];
         });
-        glv::mul_canon_g<4>(dd, ww, r, qm1);
-        static_for<4>([&](auto M_) __attribute__((always_inline)) { x[bf_u(H, 4 * G + M_) + H] = r[M_]; });
+        glv::mul_canon_g<kGW>(dd, ww, r, qm1);
+        static_for<kGW>([&](auto M_) __attribute__((always_inline)) { x[bf_u(H, kGW * G + M_) + H] = r[M_]; });
     });
 }
 
