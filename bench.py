@@ -322,6 +322,54 @@ def extra_workloads(tp, reps=2):
     return out
 
 
+def cpu_baseline_leg(args, lwes, tv, brk, ksk):
+    """The headline workload on the host cores: the AVX2 CPU port (oracle/cpu_port.cpp, equal word for
+    word to the scalar oracle) on one thread and on every thread this process may use, per-bootstrap
+    latency percentiles from the one-thread run, and NTT calibration; the scalar oracle's one-thread
+    rate beside it.  Bounded samples (about args.cpu_seconds in all)."""
+    import oracle as O
+    nproc = os.cpu_count()
+    affinity = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else nproc
+    threads = affinity
+    out = {"unit": "bootstraps/s", "kind": "port", "nproc": nproc, "affinity": affinity, "cpu_model": cpu_model()}
+    port = O.port()
+    if port is None:
+        # no AVX2 here: the scalar oracle on every thread
+        cnt = threads
+        _, secs = O.bootstrap_batch(N, np.ascontiguousarray(lwes[:cnt]).reshape(-1), tv, brk, n, BB, L, O.BINARY,
+                                    ksk, KS_OUT, KS_B, KS_D, threads)
+        out.update({"value": round(cnt / secs, 3), "cores": threads, "isa": "scalar",
+                    "sample": "%d bootstraps on %d threads (scalar oracle), %.1f s" % (cnt, threads, secs)})
+        return out
+    budget = max(args.cpu_seconds, 3.0)
+    # one thread: per-bootstrap latencies (p50/p99) and the single-core rate
+    probe, secs1, _ = O.port_bootstrap_batch(N, lwes[:1], tv, brk, n, BB, L, ksk, KS_OUT, KS_B, KS_D, 1, per_ct=True)
+    cnt1 = int(min(len(lwes), max(4, 0.3 * budget / max(secs1, 1e-3))))
+    _, secs1, per = O.port_bootstrap_batch(N, lwes[:cnt1], tv, brk, n, BB, L, ksk, KS_OUT, KS_B, KS_D, 1, per_ct=True)
+    # every thread: the aggregate rate
+    cntn = int(min(len(lwes), max(threads, threads * 0.5 * budget / max(secs1 / cnt1, 1e-3))))
+    cntn -= cntn % threads
+    _, secsn = O.port_bootstrap_batch(N, lwes[:cntn], tv, brk, n, BB, L, ksk, KS_OUT, KS_B, KS_D, threads)
+    f1, i1 = O.port_ntt_time(1024, 2000)
+    f2, i2 = O.port_ntt_time(2048, 1000)
+    # the scalar oracle (the plain C restatement), one bootstrap on one thread
+    _, secs_or = O.bootstrap_batch(N, np.ascontiguousarray(lwes[:1]).reshape(-1), tv, brk, n, BB, L, O.BINARY, ksk,
+                                   KS_OUT, KS_B, KS_D, 1)
+    out.update({
+        "value": round(cntn / secsn, 3), "cores": threads, "isa": "avx2",
+        "one_thread_bootstraps_per_s": round(cnt1 / secs1, 3),
+        "p50_ms": round(1e3 * float(np.percentile(per, 50)), 3),
+        "p99_ms": round(1e3 * float(np.percentile(per, 99)), 3),
+        "ntt_us": {"fwd_1024": round(f1 * 1e6, 2), "inv_1024": round(i1 * 1e6, 2),
+                   "fwd_2048": round(f2 * 1e6, 2), "inv_2048": round(i2 * 1e6, 2)},
+        "scalar_oracle_one_thread_bootstraps_per_s": round(1.0 / secs_or, 3),
+        "sample": "AVX2 port (oracle/cpu_port.cpp): %d bootstraps on 1 thread in %.1f s, %d on %d threads in %.1f s "
+                  "(%s; nproc %d, affinity %d)" % (cnt1, secs1, cntn, threads, secsn, cpu_model(), nproc, affinity)})
+    log("CPU baseline: %.1f bootstraps/s on %d threads, %.2f on one (p50 %.1f ms, p99 %.1f ms)"
+        % (out["value"], threads, out["one_thread_bootstraps_per_s"], out["p50_ms"], out["p99_ms"]))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -512,25 +560,7 @@ def main():
         result["latency_ms"]["serve_p99"] = result["serving"]["p99_ms"]
 
     if not args.no_cpu:
-        import oracle as O
-        # bounded CPU sample of the same workload: a few bootstraps per thread
-        threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-        threads = max(1, min(threads, 16))
-        per = 1
-        O_lwes = np.ascontiguousarray(lwes[: threads * per])
-        _, secs = O.bootstrap_batch(N, O_lwes.reshape(-1), tv, brk, n, BB, L, O.BINARY, ksk, KS_OUT, KS_B, KS_D,
-                                    threads)
-        cnt = threads * per
-        if secs < args.cpu_seconds / 3:
-            reps = max(1, int(args.cpu_seconds / max(secs, 1e-3)) - 1)
-            cnt2 = min(len(lwes), threads * per * reps)
-            _, secs = O.bootstrap_batch(N, np.ascontiguousarray(lwes[:cnt2]).reshape(-1), tv, brk, n, BB, L,
-                                        O.BINARY, ksk, KS_OUT, KS_B, KS_D, threads)
-            cnt = cnt2
-        result["cpu_baseline"] = {"value": round(cnt / secs, 3), "unit": "bootstraps/s", "cores": threads,
-                                  "kind": "port",
-                                  "sample": "%d bootstraps of the same workload on %d threads (%s), %.1f s" % (
-                                      cnt, threads, cpu_model(), secs)}
+        result["cpu_baseline"] = cpu_baseline_leg(args, lwes, tv, brk, ksk)
     print(json.dumps(result), flush=True)
 
 
