@@ -9,7 +9,9 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libtoothpaste_amd.so")
+# TOOTHPASTE_AMD_LIB selects another build of the same library (the host-sanitised build of
+# fhetoothpaste_amd.build.build_sanitized, which the CPU sanitizer test loads).
+LIB_PATH = os.environ.get("TOOTHPASTE_AMD_LIB") or os.path.join(HERE, "libtoothpaste_amd.so")
 
 Q = 0xFFFFFFFF00000001
 TP_OK, TP_E_INVALID, TP_E_DEVICE, TP_E_STATE, TP_E_NOMEM, TP_E_TIMEOUT = 0, -1, -2, -3, -4, -5

@@ -62,6 +62,47 @@ def build(verbose=True):
     return OUT
 
 
+# Host-sanitised variant for the CPU tests: the host sources (C-ABI, key generation, wire format,
+# serving scheduler, drop-in classes) with AddressSanitizer + UndefinedBehaviorSanitizer, the device
+# objects of the regular build unchanged.  -Xarch_host keeps every sanitizer off the GPU code.
+SAN_DIR = os.path.join(HERE, "_san")
+SAN_OUT = os.path.join(SAN_DIR, "libtoothpaste_amd.so")
+SAN_FLAGS = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+             "-Xarch_host", "-fno-sanitize-recover=undefined", "-Xarch_host", "-shared-libsan",
+             "-fno-omit-frame-pointer", "-g", "-O1", "-std=c++17", "-fPIC"]
+
+
+def build_sanitized(verbose=True):
+    os.makedirs(SAN_DIR, exist_ok=True)
+    objs, todo = [], []
+    for src, flags in SOURCES:
+        if not os.path.exists(os.path.join(CSRC, src)):
+            continue
+        if src.endswith(".hip"):
+            obj = os.path.join(BUILD_DIR, os.path.splitext(src)[0] + ".o")
+            if not os.path.exists(obj):
+                todo.append((src, flags, obj))
+        else:
+            obj = os.path.join(SAN_DIR, os.path.splitext(src)[0] + ".o")
+            todo.append((src, SAN_FLAGS, obj))
+        objs.append(obj)
+    os.makedirs(BUILD_DIR, exist_ok=True)
+
+    def comp(t):
+        src, flags, obj = t
+        subprocess.check_call([HIPCC] + flags + ["-c", os.path.join(CSRC, src), "-o", obj])
+    with ThreadPoolExecutor(max_workers=6) as ex:
+        list(ex.map(comp, todo))
+    subprocess.check_call([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-Xarch_host", "-fsanitize=address",
+                           "-Xarch_host", "-fsanitize=undefined", "-Xarch_host", "-shared-libsan", "-o", SAN_OUT] + objs)
+    if verbose:
+        print("built", SAN_OUT)
+    return SAN_OUT
+
+
 if __name__ == "__main__":
-    build()
+    if "--sanitize" in sys.argv:
+        build_sanitized()
+    else:
+        build()
     sys.exit(0)

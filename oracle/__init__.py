@@ -13,7 +13,10 @@ Q = 0xFFFFFFFF00000001
 BINARY, TERNARY, BMMP = 0, 1, 2
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+# TOOTHPASTE_ORACLE_LIB_DIR selects another build of the oracle libraries (oracle/_san: the
+# ASan/UBSan build of `make -C oracle sanitize`, which the CPU sanitizer test loads).
+_LIB_DIR = os.environ.get("TOOTHPASTE_ORACLE_LIB_DIR") or _HERE
+_LIB_PATH = os.path.join(_LIB_DIR, "liboracle.so")
 _lib = None
 
 u64p = ctypes.POINTER(ctypes.c_uint64)
@@ -333,7 +336,7 @@ def sap_decompose(N, lwe, radix, beta, brk, n, bl, ll, method, keys, b, d, max_d
 
 # ---------------------------------------------------------------- vectorised CPU port (cpu_port.cpp)
 _port = None
-_PORT_PATH = os.path.join(_HERE, "libcpuport.so")
+_PORT_PATH = os.path.join(_LIB_DIR, "libcpuport.so")
 
 
 def port():
