@@ -107,10 +107,18 @@ br_kernel(BrArgs A) {
                 e2 = (e0 + e1) & (2 * N - 1);
             }
             const uint64_t* key = A.brk + (size_t)i * NC * rgsw_words;
-            // the 2 LD products of each output coefficient, summed in lazy words
+            // binary: the 2 LD products of each output coefficient, summed in lazy words; ternary and
+            // BMMP: one such sum per control (RGSW_c), weighted once the 2 LD rows are summed
             uint64_t out[2][kE];
+            uint64_t outc[METHOD == 0 ? 1 : NC][2][kE];
 #pragma unroll
             for (int k = 0; k < kE; ++k) out[0][k] = out[1][k] = 0;
+            if constexpr (METHOD != 0) {
+#pragma unroll
+                for (int r = 0; r < NC; ++r)
+#pragma unroll
+                    for (int k = 0; k < kE; ++k) outc[r][0][k] = outc[r][1][k] = 0;
+            }
 
 #pragma unroll 1
             for (int h = 0; h < 2; ++h) {
@@ -256,30 +264,21 @@ br_kernel(BrArgs A) {
                             }
                         });
                     } else {
-                        // combined key sum_c w_c RGSW_c[row], formed pair by pair next to its use; the
-                        // monomial and key pairs of pair c + 1 are requested while pair c is used (the
-                        // first ones during the transform's last level)
+                        // the reference's MultiMux order (mux_operator.cpp:156-202,264-291): every control
+                        // c sums d_row * RGSW_c[row] over the 2 LD rows, and its weight multiplies the
+                        // sum once per step (below); key pairs of pair c + 1 are requested while pair c
+                        // is used (the first ones during the transform's last level)
                         const uint64_t* krow = opaque(key + row);
-                        const uint64_t* mono = opaque_after(A.mono, d[0]);  // weights re-read per digit
                         struct PairLoads {
-                            U64x2 m[3];      // NTT(X^e) at e0, e1, e2 (raw table rows)
                             U64x2 k[NC][2];  // key pairs of RGSW_c, output halves 0 and 1
                         };
                         auto load_pair = [&](PairLoads& pl, int c) __attribute__((always_inline)) {
-                            const uint32_t es[3] = {e0, e1, e2};
-#pragma unroll
-                            for (int q = 0; q < (NC == 3 ? 3 : 2); ++q)
-                                pl.m[q] = ld_pair(mono + (size_t)(es[q] & (N - 1)) * N, c * S::P + L);
 #pragma unroll
                             for (int r = 0; r < NC; ++r)
 #pragma unroll
                                 for (int p = 0; p < 2; ++p)
                                     pl.k[r][p] = ld_pair(krow + r * rgsw_words + p * N, c * S::P + L);
                             __builtin_amdgcn_sched_barrier(0);
-                        };
-                        // NTT(X^e) for e in [0, 2N): X^e = -X^(e-N) for e >= N
-                        auto mono_val = [&](const U64x2& v, uint32_t e) __attribute__((always_inline)) {
-                            return e >= (uint32_t)N ? U64x2{gl::neg(v.x), gl::neg(v.y)} : v;
                         };
                         PairLoads cur;
                         ntt_forward<LOGN, 1>(d, buf, A.tw, L, [&]() __attribute__((always_inline)) {
@@ -289,34 +288,26 @@ br_kernel(BrArgs A) {
                             constexpr int c = C;
                             PairLoads nxt;
                             if constexpr (c + 1 < kE / 2) load_pair(nxt, c + 1);
-                            const U64x2 m0 = mono_val(cur.m[0], e0), m1 = mono_val(cur.m[1], e1);
-                            const U64x2 w0{gl::sub(m0.x, 1), gl::sub(m0.y, 1)};  // X^e0 - 1
-                            const U64x2 w1{gl::sub(m1.x, 1), gl::sub(m1.y, 1)};  // X^e1 - 1
-                            U64x2 w2{0, 0};
-                            if constexpr (NC == 3) {
-                                // X^(e0+e1) - X^e0 - X^e1 + 1 (bmmp_blind_rotation.cpp:280-323)
-                                const U64x2 m2 = mono_val(cur.m[2], e2);
-                                w2 = U64x2{gl::add(gl::sub(gl::sub(m2.x, m0.x), m1.x), 1),
-                                           gl::add(gl::sub(gl::sub(m2.y, m0.y), m1.y), 1)};
-                            }
-#pragma unroll
-                            for (int p = 0; p < 2; ++p) {
-                                const U64x2 a0 = cur.k[0][p], a1 = cur.k[1][p];
-                                uint64_t cx = gl::mac_lazy(a1.x, w1.x, gl::mul_lazy(a0.x, w0.x));
-                                uint64_t cy = gl::mac_lazy(a1.y, w1.y, gl::mul_lazy(a0.y, w0.y));
-                                if constexpr (NC == 3) {
-                                    const U64x2 a2 = cur.k[NC - 1][p];
-                                    cx = gl::mac_lazy(a2.x, w2.x, cx);
-                                    cy = gl::mac_lazy(a2.y, w2.y, cy);
-                                }
-                                if (p == 0) {
-                                    out[0][2 * c] = gl::mac_lazy(d[2 * c], cx, out[0][2 * c]);
-                                    out[0][2 * c + 1] = gl::mac_lazy(d[2 * c + 1], cy, out[0][2 * c + 1]);
-                                } else {
-                                    out[1][2 * c] = gl::mac_lazy(d[2 * c], cx, out[1][2 * c]);
-                                    out[1][2 * c + 1] = gl::mac_lazy(d[2 * c + 1], cy, out[1][2 * c + 1]);
-                                }
-                            }
+                            // members (control, half, coefficient 2c or 2c + 1), in groups of at most 8
+                            static_for<(NC + 1) / 2>([&](auto G) __attribute__((always_inline)) {
+                                constexpr int r0 = 2 * G;
+                                constexpr int nr = (r0 + 2 <= NC) ? 2 : 1;
+                                constexpr int M = 4 * nr;
+                                uint64_t g[M], dd[M], kk[M];
+                                static_for<M>([&](auto Mi) __attribute__((always_inline)) {
+                                    constexpr int m = Mi;
+                                    constexpr int r = r0 + m / 4, p = (m / 2) % 2, q = m % 2;
+                                    g[m] = outc[r][p][2 * c + q];
+                                    dd[m] = d[2 * c + q];
+                                    kk[m] = q ? cur.k[r][p].y : cur.k[r][p].x;
+                                });
+                                glv::mac_lazy_g<M>(g, dd, kk);
+                                static_for<M>([&](auto Mi) __attribute__((always_inline)) {
+                                    constexpr int m = Mi;
+                                    constexpr int r = r0 + m / 4, p = (m / 2) % 2, q = m % 2;
+                                    outc[r][p][2 * c + q] = g[m];
+                                });
+                            });
                             if constexpr (c + 1 < kE / 2) cur = nxt;
                         });
                     }
@@ -328,6 +319,61 @@ br_kernel(BrArgs A) {
                     acc[0][k] = acc[1][k];
                     acc[1][k] = t;
                 }
+            }
+            if constexpr (METHOD != 0) {
+                // out = sum_c w_c sum_rows: w = (X^-a - 1, X^a - 1) for ternary
+                // (cggi_blind_rotation.cpp:388-472), (X^a0 - 1, X^a1 - 1, X^(a0+a1) - X^a0 - X^a1 + 1)
+                // for BMMP (bmmp_blind_rotation.cpp:280-323); NTT(X^e) rows come from the monomial
+                // table, X^e = -X^(e-N) for e >= N; the rows of pair c + 1 are requested while pair c
+                // is used
+                constexpr int NQ = NC == 3 ? 3 : 2;
+                const uint64_t* mono = opaque_after(A.mono, outc[0][0][0]);
+                const uint32_t es[3] = {e0, e1, e2};
+                auto load_m = [&](U64x2 (&m)[NQ], int c) __attribute__((always_inline)) {
+#pragma unroll
+                    for (int q = 0; q < NQ; ++q) m[q] = ld_pair(mono + (size_t)(es[q] & (N - 1)) * N, c * S::P + L);
+                    __builtin_amdgcn_sched_barrier(0);
+                };
+                auto mono_val = [&](const U64x2& v, uint32_t e) __attribute__((always_inline)) {
+                    return e >= (uint32_t)N ? U64x2{gl::neg(v.x), gl::neg(v.y)} : v;
+                };
+                U64x2 mc[NQ];
+                load_m(mc, 0);
+                static_for<kE / 2>([&](auto C) __attribute__((always_inline)) {
+                    constexpr int c = C;
+                    U64x2 mn[NQ];
+                    if constexpr (c + 1 < kE / 2) load_m(mn, c + 1);
+                    const U64x2 m0 = mono_val(mc[0], e0), m1 = mono_val(mc[1], e1);
+                    U64x2 w[NC];
+                    w[0] = U64x2{gl::sub(m0.x, 1), gl::sub(m0.y, 1)};  // X^e0 - 1
+                    w[1] = U64x2{gl::sub(m1.x, 1), gl::sub(m1.y, 1)};  // X^e1 - 1
+                    if constexpr (NC == 3) {
+                        const U64x2 m2 = mono_val(mc[NQ - 1], e2);
+                        w[NC - 1] = U64x2{gl::add(gl::sub(gl::sub(m2.x, m0.x), m1.x), 1),
+                                          gl::add(gl::sub(gl::sub(m2.y, m0.y), m1.y), 1)};
+                    }
+                    // members (half, coefficient 2c or 2c + 1), one multiply-accumulate per control
+                    uint64_t o[4] = {0, 0, 0, 0};
+                    static_for<NC>([&](auto R) __attribute__((always_inline)) {
+                        constexpr int r = R;
+                        uint64_t ww[4], vv[4];
+                        static_for<4>([&](auto Mi) __attribute__((always_inline)) {
+                            constexpr int m = Mi;
+                            constexpr int p = m / 2, q = m % 2;
+                            ww[m] = q ? w[r].y : w[r].x;
+                            vv[m] = outc[r][p][2 * c + q];
+                        });
+                        glv::mac_lazy_g<4>(o, ww, vv);
+                    });
+                    static_for<4>([&](auto Mi) __attribute__((always_inline)) {
+                        constexpr int m = Mi;
+                        out[m / 2][2 * c + m % 2] = o[m];
+                    });
+                    if constexpr (c + 1 < kE / 2) {
+#pragma unroll
+                        for (int q = 0; q < NQ; ++q) mc[q] = mn[q];
+                    }
+                });
             }
 #pragma unroll 1
             for (int h = 0; h < 2; ++h) {
