@@ -72,6 +72,13 @@ br_kernel(BrArgs A) {
     const uint64_t mask = (1ULL << A.base_log2) - 1;
     const size_t rgsw_words = (size_t)4 * N * LD;
 
+    // first-level constants of the signed digits: offsets 2^(b-1) and `least` (the last digit)
+    uint64_t kp_n = 0, km_n = 0, kp_l = 0, km_l = 0;
+    if constexpr (METHOD != 0) {
+        signed_first_consts(1ULL << (A.base_log2 - 1), kp_n, km_n);
+        signed_first_consts(A.least, kp_l, km_l);
+    }
+
     for (int ct = blockIdx.x; ct < A.count; ct += gridDim.x) {
         const int slot = A.active ? (int)A.active[ct] : ct;
         const uint64_t* lwe = A.lwe + (size_t)slot * (A.n + 1);
@@ -141,6 +148,13 @@ br_kernel(BrArgs A) {
                     for (int k = 0; k < kE; ++k) src[phys(idx1<LOGN>(L, k))] = t[k];
                     poly_sync<S::WAVES>();
                 }
+                uint64_t xc[METHOD == 0 ? 1 : kE];  // x + C mod Q of the half (signed digits)
+                if constexpr (METHOD != 0) {
+                    uint64_t cv[kE];
+#pragma unroll
+                    for (int k = 0; k < kE; ++k) cv[k] = A.corrector;
+                    glv::add_cc_g<kE>(acc[0], cv, xc, glv::qm1_const());
+                }
 #pragma unroll 1
                 for (int j = 0; j < LD; ++j) {
                     uint64_t d[kE];
@@ -179,17 +193,22 @@ br_kernel(BrArgs A) {
                             for (int k = 0; k < kE / 2; ++k) ct_bfly_pow2(d[k], d[k + kE / 2], gl::kFwdPow2[1]);
                         }
                     } else {
-                        // signed digits of x + C mod Q (gadget_decomp.h:169-198)
+                        // signed digits of x + C mod Q (gadget_decomp.h:169-198), x + C formed once per half
                         const bool last = (j == A.max_digits - 1);
                         const int shift = last ? 0 : 64 - (j + 1) * A.base_log2;
                         const uint64_t dm = last ? (2 * A.least - 1) : mask;
                         const uint64_t delta = last ? A.least : (1ULL << (A.base_log2 - 1));
+                        if (shift >= 32 && A.base_log2 < 32) {
 #pragma unroll
-                        for (int k = 0; k < kE; ++k) d[k] = (gl::add(acc[0][k], A.corrector) >> shift) & dm;
+                            for (int k = 0; k < kE; ++k)
+                                d[k] = __builtin_amdgcn_ubfe(glv::hi32(xc[k]), (uint32_t)(shift - 32), (uint32_t)A.base_log2);
+                        } else {
+#pragma unroll
+                            for (int k = 0; k < kE; ++k) d[k] = (xc[k] >> shift) & dm;
+                        }
                         // the first transform level folds the offset in while the digits are small
                         if (dm < (1ULL << 16)) {
-                            uint64_t kp, km;
-                            signed_first_consts(delta, kp, km);
+                            const uint64_t kp = last ? kp_l : kp_n, km = last ? km_l : km_n;
 #pragma unroll
                             for (int k = 0; k < kE / 2; ++k) ct_bfly_signed_first(d[k], d[k + kE / 2], kp, km);
                         } else {
