@@ -409,11 +409,11 @@ def main():
     ctx.import_brk(brk)
     ctx.import_lwe_ksk(ksk)
     tune = ctx.autotune(args.batch)
-    log("blind-rotation variant: engine %s, %d wave(s)/SIMD (NTT two: %.1f ms, one: %.1f ms, digit-parallel: %.1f ms; "
-        "floating point one: %.1f ms, two: %.1f ms); digit-parallel up to %d ciphertexts"
+    log("blind-rotation variant: engine %s, %d wave(s)/SIMD, grid %s (NTT two: %.1f ms, one: %.1f ms, digit-parallel: "
+        "%.1f ms, persistent grid: %.1f ms); digit-parallel up to %d ciphertexts"
         % ("floating point" if tune["engine"] else "NTT", tune["fft_waves"] if tune["engine"] else tune["waves"],
-           tune["ms_two_waves"], tune["ms_one_wave"], tune["ms_digit_parallel"], tune["ms_fft_one_wave"],
-           tune["ms_fft_two_waves"], tune["split_max"]))
+           tune["grid"] or "one work-group per ciphertext", tune["ms_two_waves"], tune["ms_one_wave"],
+           tune["ms_digit_parallel"], tune["ms_persistent_grid"], tune["split_max"]))
 
     B = args.batch
     rng = np.random.default_rng(SEED)
@@ -547,6 +547,7 @@ def main():
         "tuning": {"br_waves_per_simd": tune["waves"], "br_ms_two_waves": round(tune["ms_two_waves"], 2),
                    "br_ms_one_wave": round(tune["ms_one_wave"], 2),
                    "br_ms_digit_parallel": round(tune["ms_digit_parallel"], 2),
+                   "br_ms_persistent_grid": round(tune["ms_persistent_grid"], 2), "br_grid": tune["grid"],
                    "digit_parallel_up_to": split_tuned},
         "batch_sweep": sweep,
     }

@@ -61,6 +61,9 @@ SIGNATURES = {
     "tp_lwe_keyswitch_batch_device": [vp, SZ, vp, SZ, vp],
     "tp_bootstrap_batch_device": [vp, SZ, vp, vp, SZ, vp],
     "tp_bootstrap_batch_device_noalloc": [vp, SZ, vp, vp, SZ, vp],
+    "tp_graph_bootstrap_create": [vp, SZ, vp, vp, SZ, vp, ctypes.POINTER(vp)],
+    "tp_graph_launch": [vp],
+    "tp_graph_destroy": [vp],
     "tp_malloc": [vp, ctypes.POINTER(vp), SZ],
     "tp_free": [vp, vp],
     "tp_copy_to_device": [vp, vp, u64p, SZ],
@@ -80,6 +83,7 @@ SIGNATURES = {
     "tp_set_option": [vp, INT, INT],
     "tp_get_option": [vp, INT, ctypes.POINTER(INT)],
     "tp_autotune": [vp, SZ, ctypes.POINTER(ctypes.c_float)],
+    "tp_autotune_cached": [vp, SZ, ctypes.c_char_p, ctypes.POINTER(INT), ctypes.POINTER(ctypes.c_float)],
     "tp_rlwe_keyswitch_device": [vp, SZ, U32, INT, INT, vp, vp],
     "tp_pack_consecutive_device": [vp, SZ, vp, vp],
     "tp_eltwise_device": [vp, INT, vp, vp, vp, vp, U64, SZ],

@@ -30,6 +30,7 @@ SOURCES = [
     ("kernels_rgsw.hip", DEVICE_FLAGS),
     ("noalloc.hip", DEVICE_FLAGS),
     ("capi.cpp", HOST_FLAGS),
+    ("tuning_cache.cpp", HOST_FLAGS),
     ("fft_key.cpp", HOST_FLAGS),
     ("keygen.cpp", HOST_FLAGS),
     ("serve.cpp", HOST_FLAGS),

@@ -14,6 +14,7 @@
 #include "fft_key.hpp"
 #include "host_ntt.hpp"
 #include "kernels.hpp"
+#include "tuning_cache.hpp"
 
 namespace {
 
@@ -84,6 +85,8 @@ struct tp_context {
     int br_engine = 0;           // 0: Goldilocks NTT (default), 1: floating-point product where ready (opt-in)
     int fft_waves = 1;           // waves per SIMD of the floating-point kernel (1 or 2)
     int br_grid = 0;             // work-groups of a br_kernel launch: 0 one per ciphertext, else at most this many
+    int cu_count = 256;          // compute units of the context's device
+    char arch[32] = {0};         // gcnArchName of the context's device
 };
 
 namespace {
@@ -247,7 +250,11 @@ int tp_context_create(tp_context** out, const tp_params* p, int device) {
     c->logN = ilog2(N);
     // up to two ciphertexts per CU: two rounds of the digit-parallel kernel (one work-group per CU)
     // still beat one round of br_kernel at one wave per SIMD
-    if (split_built(c)) c->split_max = 2 * (size_t)(prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 1);
+    if (prop.multiProcessorCount > 0) c->cu_count = prop.multiProcessorCount;
+    std::snprintf(c->arch, sizeof(c->arch), "%s", prop.gcnArchName);
+    for (char* q = c->arch; *q; ++q)
+        if (*q == ':' || *q == '"' || *q == '\\') *q = '\0';  // "gfx950:sramecc+:xnack-" -> "gfx950"
+    if (split_built(c)) c->split_max = 2 * (size_t)c->cu_count;
     int rc = set_device(c);
     if (rc) { delete c; return rc; }
     // device address of the no-allocation entry point's scratch array, once per device (its own
@@ -710,6 +717,70 @@ int tp_bootstrap_batch_device(tp_context* c, size_t count, const uint64_t* lwes,
     return tp::bootstrap_into(c, count, lwes, tv, tv_stride, c->d_scratch, out);
 }
 
+struct tp_graph {
+    tp_context* ctx = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    uint64_t* d_ext = nullptr;  // extracted LWEs between the two launches, owned by the graph
+};
+
+int tp_graph_destroy(tp_graph* g) {
+    if (!g) return TP_OK;
+    if (g->ctx) (void)set_device(g->ctx);
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    if (g->d_ext) (void)hipFree(g->d_ext);
+    delete g;
+    return TP_OK;
+}
+
+int tp_graph_bootstrap_create(tp_context* c, size_t count, const uint64_t* lwes, const uint64_t* tv,
+                              size_t tv_stride, uint64_t* out, tp_graph** graph) {
+    if (!c || !lwes || !tv || !out || !graph) return fail(TP_E_INVALID, "null argument");
+    *graph = nullptr;
+    if (!c->d_brk || !c->d_ksk) return fail(TP_E_STATE, "blind-rotation and key-switching keys must be imported");
+    if (count == 0 || count > 0x7fffffff) return fail(TP_E_INVALID, "count must lie in [1, 2^31)");
+    int rc = set_device(c);
+    if (rc) return rc;
+    tp_graph* g = new tp_graph;
+    g->ctx = c;
+    if (hipMalloc(&g->d_ext, count * (c->N + 1) * 8) != hipSuccess) {
+        tp_graph_destroy(g);
+        return fail(TP_E_NOMEM, "graph scratch allocation failed");
+    }
+    // capture on the context's stream: the same launches tp_bootstrap_batch_device issues (kernels
+    // only, no memset or copy nodes)
+    if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        tp_graph_destroy(g);
+        return fail(TP_E_DEVICE, "stream capture could not begin");
+    }
+    rc = tp::bootstrap_into(c, count, lwes, tv, tv_stride, g->d_ext, out);
+    const hipError_t e = hipStreamEndCapture(c->stream, &g->graph);
+    if (rc) {
+        tp_graph_destroy(g);
+        return rc;
+    }
+    if (e != hipSuccess || !g->graph) {
+        tp_graph_destroy(g);
+        return fail(TP_E_DEVICE, std::string("stream capture: ") + hipGetErrorString(e));
+    }
+    if (hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0) != hipSuccess) {
+        tp_graph_destroy(g);
+        return fail(TP_E_DEVICE, "graph instantiation failed");
+    }
+    *graph = g;
+    return TP_OK;
+}
+
+int tp_graph_launch(tp_graph* g) {
+    if (!g || !g->exec) return fail(TP_E_INVALID, "null graph");
+    int rc = set_device(g->ctx);
+    if (rc) return rc;
+    const hipError_t e = hipGraphLaunch(g->exec, g->ctx->stream);
+    if (e != hipSuccess) return fail(TP_E_DEVICE, std::string("graph launch: ") + hipGetErrorString(e));
+    return TP_OK;
+}
+
 int tp_set_option(tp_context* c, int option, int value) {
     if (!c) return fail(TP_E_INVALID, "null context");
     if (option == TP_OPT_BR_WAVES) {
@@ -805,7 +876,18 @@ int tp_autotune(tp_context* c, size_t count, float* ms) {
     // the floating-point engine is a candidate only for a context that opted in to it
     const bool fft_candidate = c->br_engine == 1 && c->fft_ready;
     const int saved_engine = c->br_engine, saved_fw = c->fft_waves;
-    float t_var[5] = {-1.0f, -1.0f, -1.0f, -1.0f, -1.0f};
+    const int saved_grid = c->br_grid;
+    float t_var[6] = {-1.0f, -1.0f, -1.0f, -1.0f, -1.0f, -1.0f};
+    auto time_launch = [&](float& t) {
+        rc = launch_eval(c, count, d_lwe, d_tv, 0, d_out, nullptr, 0, 0);  // warm-up
+        if (!rc && (hipEventRecord(e0, c->stream) != hipSuccess))
+            rc = fail(TP_E_DEVICE, "event record");
+        if (!rc) rc = launch_eval(c, count, d_lwe, d_tv, 0, d_out, nullptr, 0, 0);
+        if (!rc && (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                    hipEventElapsedTime(&t, e0, e1) != hipSuccess))
+            rc = fail(TP_E_DEVICE, "autotune timing");
+    };
+    c->br_grid = 0;
     for (int v = 0; v < 5 && !rc; ++v) {
         if (v == 1 && c->p.br_digits != 3 && c->p.br_digits != 4) continue;
         if (v == 2 && !split_built(c)) continue;
@@ -815,13 +897,7 @@ int tp_autotune(tp_context* c, size_t count, float* ms) {
         c->br_engine = v >= 3 ? 1 : 0;
         c->fft_waves = v == 4 ? 2 : 1;
         float t = -1.0f;
-        rc = launch_eval(c, count, d_lwe, d_tv, 0, d_out, nullptr, 0, 0);  // warm-up
-        if (!rc && (hipEventRecord(e0, c->stream) != hipSuccess))
-            rc = fail(TP_E_DEVICE, "event record");
-        if (!rc) rc = launch_eval(c, count, d_lwe, d_tv, 0, d_out, nullptr, 0, 0);
-        if (!rc && (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-                    hipEventElapsedTime(&t, e0, e1) != hipSuccess))
-            rc = fail(TP_E_DEVICE, "autotune timing");
+        time_launch(t);
         t_var[v] = t;
     }
     if (rc) {
@@ -829,6 +905,7 @@ int tp_autotune(tp_context* c, size_t count, float* ms) {
         c->split_max = saved_split;
         c->br_engine = saved_engine;
         c->fft_waves = saved_fw;
+        c->br_grid = saved_grid;
     } else {
         c->br_waves = (t_var[1] >= 0 && t_var[1] < t_var[0]) ? 1 : 2;
         float t_one = c->br_waves == 1 ? t_var[1] : t_var[0];
@@ -839,18 +916,90 @@ int tp_autotune(tp_context* c, size_t count, float* ms) {
             c->br_engine = 1;
             t_one = t_fft;
         }
-        // the digit-parallel kernel wins at small counts: move the threshold to this count's side
-        if (t_var[2] >= 0 && t_var[2] < t_one) c->split_max = std::max(saved_split, count);
-        else c->split_max = std::min(saved_split, count - 1);
+        // the winner on a persistent grid (work-groups = the ciphertexts resident at once, each
+        // striding over the launch) when the launch holds more ciphertexts than that
+        const int slots = c->cu_count * 4 * (c->br_engine ? c->fft_waves : c->br_waves);
+        if (count > (size_t)slots) {
+            c->split_max = 0;
+            c->br_grid = slots;
+            float t = -1.0f;
+            time_launch(t);
+            t_var[5] = t;
+            c->br_grid = 0;
+        }
+        if (rc) {
+            c->br_waves = saved_w;
+            c->split_max = saved_split;
+            c->br_engine = saved_engine;
+            c->fft_waves = saved_fw;
+            c->br_grid = saved_grid;
+        } else {
+            if (t_var[5] >= 0 && t_var[5] < t_one) {
+                c->br_grid = slots;
+                t_one = t_var[5];
+            }
+            // the digit-parallel kernel wins at small counts: move the threshold to this count's side
+            if (t_var[2] >= 0 && t_var[2] < t_one) c->split_max = std::max(saved_split, count);
+            else c->split_max = std::min(saved_split, count - 1);
+        }
     }
     if (ms)
-        for (int v = 0; v < 5; ++v) ms[v] = t_var[v];
+        for (int v = 0; v < 6; ++v) ms[v] = t_var[v];
     (void)hipStreamSynchronize(c->stream);
     for (void* p : {(void*)d_lwe, (void*)d_tv, (void*)d_out})
         if (p) (void)hipFree(p);
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
     return rc;
+}
+
+int tp_autotune_cached(tp_context* c, size_t count, const char* path, int* hit, float* ms) {
+    if (!c || !path) return fail(TP_E_INVALID, "null argument");
+    if (hit) *hit = 0;
+    std::vector<tp::TuneEntry> entries;
+    std::string err;
+    if (tp::read_tuning_cache(path, entries, err)) return fail(TP_E_INVALID, err);
+    tp::TuneEntry key;
+    key.arch = c->arch;
+    key.cus = c->cu_count;
+    key.N = c->N;
+    key.n = c->p.lwe_dim;
+    key.b = c->p.br_base_log2;
+    key.l = c->p.br_digits;
+    key.method = c->p.method;
+    key.count = count;
+    key.fft_opt_in = (c->br_engine == 1 && c->fft_ready) ? 1 : 0;
+    for (const tp::TuneEntry& e : entries) {
+        if (!e.same_key(key)) continue;
+        // stored settings, checked against what this build and context allow
+        if ((e.waves != 1 && e.waves != 2) || (e.waves == 1 && c->p.br_digits != 3 && c->p.br_digits != 4) ||
+            (e.split_max > 0 && !split_built(c)) || (e.engine == 1 && !c->fft_ready) || e.grid < 0 ||
+            (e.fft_waves != 1 && e.fft_waves != 2))
+            return fail(TP_E_INVALID, "tuning cache entry does not fit this context");
+        c->br_waves = e.waves;
+        c->split_max = (size_t)e.split_max;
+        c->br_engine = e.engine;
+        c->fft_waves = e.fft_waves;
+        c->br_grid = e.grid;
+        if (ms)
+            for (int v = 0; v < 6; ++v) ms[v] = v < (int)e.ms.size() ? e.ms[v] : -1.0f;
+        if (hit) *hit = 1;
+        return TP_OK;
+    }
+    float t[6];
+    int rc = tp_autotune(c, count, t);
+    if (rc) return rc;
+    key.waves = c->br_waves;
+    key.split_max = c->split_max;
+    key.engine = c->br_engine;
+    key.fft_waves = c->fft_waves;
+    key.grid = c->br_grid;
+    key.ms.assign(t, t + 6);
+    if (ms)
+        for (int v = 0; v < 6; ++v) ms[v] = t[v];
+    entries.push_back(key);
+    if (tp::write_tuning_cache(path, entries, err)) return fail(TP_E_INVALID, err);
+    return TP_OK;
 }
 
 int tp_malloc(tp_context* c, uint64_t** ptr, size_t words) {

@@ -10,6 +10,7 @@ Every call runs on the GPU through libtoothpaste_amd.so; there is no CPU path.
 Reference asserts and std::invalid_argument become ValueError / ToothpasteError.
 """
 import ctypes
+import os
 
 import numpy as np
 
@@ -39,6 +40,13 @@ class DeviceBuffer:
         b = cls(ctx, a.size)
         check(lib().tp_copy_to_device(ctx.h, b.p, ptr(a), a.size))
         return b
+
+    def copy_from_host(self, a):
+        """Overwrites the buffer's leading words with the host array a."""
+        a = np.ascontiguousarray(a, dtype=np.uint64).reshape(-1)
+        if a.size > self.words:
+            raise ValueError("host array larger than the device buffer")
+        check(lib().tp_copy_to_device(self.ctx.h, self.p, ptr(a), a.size))
 
     def to_host(self, words=None):
         words = self.words if words is None else words
@@ -171,17 +179,26 @@ class Context:
         check(lib().tp_pack_device(self.h, x.shape[0], d_in.p, d_perm.p if d_perm else None, d_out.p))
         return d_out.to_host()
 
-    def autotune(self, count):
+    def autotune(self, count, cache=None):
         """Time the blind-rotation variants on `count` synthetic ciphertexts (tp_autotune): keeps the
-        fastest one-wave-per-ciphertext variant (engine and waves) and moves the digit-parallel
-        threshold to the side of `count` that won; returns {"waves", "split_max", "engine",
-        "fft_waves", "ms_two_waves", "ms_one_wave", "ms_digit_parallel", "ms_fft_one_wave",
-        "ms_fft_two_waves"} (-1 for a variant that is not built or not usable)."""
-        ms = (ctypes.c_float * 5)()
-        check(lib().tp_autotune(self.h, count, ms))
+        fastest one-wave-per-ciphertext variant (engine and waves) and grid, and moves the
+        digit-parallel threshold to the side of `count` that won; with `cache` (a JSON file path,
+        tp_autotune_cached) a stored result for this device, parameter set and count is applied
+        instead of timing.  Returns {"waves", "split_max", "engine", "fft_waves", "grid",
+        "cache_hit", "ms_two_waves", "ms_one_wave", "ms_digit_parallel", "ms_fft_one_wave",
+        "ms_fft_two_waves", "ms_persistent_grid"} (-1 for a variant that is not built, not usable
+        or not timed)."""
+        ms = (ctypes.c_float * 6)()
+        hit = ctypes.c_int(0)
+        if cache is None:
+            check(lib().tp_autotune(self.h, count, ms))
+        else:
+            check(lib().tp_autotune_cached(self.h, count, os.fsencode(cache), ctypes.byref(hit), ms))
         return {"waves": self.br_waves(), "split_max": self.br_split_max(), "engine": self.br_engine(),
-                "fft_waves": self._get(T.TP_OPT_BR_FFT_WAVES), "ms_two_waves": ms[0], "ms_one_wave": ms[1],
-                "ms_digit_parallel": ms[2], "ms_fft_one_wave": ms[3], "ms_fft_two_waves": ms[4]}
+                "fft_waves": self._get(T.TP_OPT_BR_FFT_WAVES), "grid": self._get(T.TP_OPT_BR_GRID),
+                "cache_hit": bool(hit.value), "ms_two_waves": ms[0], "ms_one_wave": ms[1],
+                "ms_digit_parallel": ms[2], "ms_fft_one_wave": ms[3], "ms_fft_two_waves": ms[4],
+                "ms_persistent_grid": ms[5]}
 
     def _get(self, option):
         v = ctypes.c_int()
@@ -333,6 +350,29 @@ class Context:
     def bootstrap_batch_noalloc_device(self, count, d_lwes, d_tv, tv_stride, d_out):
         """tp_bootstrap_batch_device_noalloc on device buffers (allocates nothing itself)."""
         check(lib().tp_bootstrap_batch_device_noalloc(self.h, count, d_lwes.p, d_tv.p, tv_stride, d_out.p))
+
+    def bootstrap_graph(self, count, d_lwes, d_tv, tv_stride, d_out):
+        """A BootstrapGraph replaying tp_bootstrap_batch_device's launches on these device buffers."""
+        return BootstrapGraph(self, count, d_lwes, d_tv, tv_stride, d_out)
+
+
+class BootstrapGraph:
+    """tp_graph_bootstrap_create / tp_graph_launch: one captured full bootstrap per launch() (on the
+    context's stream, asynchronous), reading the buffers' current contents."""
+
+    def __init__(self, ctx, count, d_lwes, d_tv, tv_stride, d_out):
+        self.ctx = ctx
+        self.h = ctypes.c_void_p()
+        check(lib().tp_graph_bootstrap_create(ctx.h, count, d_lwes.p, d_tv.p, tv_stride, d_out.p,
+                                              ctypes.byref(self.h)))
+
+    def launch(self):
+        check(lib().tp_graph_launch(self.h))
+
+    def close(self):
+        if self.h:
+            lib().tp_graph_destroy(self.h)
+            self.h = ctypes.c_void_p()
 
 
 # ---------------------------------------------------------------- key generation (host)

@@ -240,6 +240,22 @@ int tp_bootstrap_batch_device(tp_context* ctx, size_t count, const uint64_t* lwe
 int tp_bootstrap_batch_device_noalloc(tp_context* ctx, size_t count, const uint64_t* lwes, const uint64_t* tv,
                                       size_t tv_stride, uint64_t* out);
 
+/* A full bootstrap (tp_bootstrap_batch_device's launches: blind rotation + sample extraction, then
+ * the LWE key switch) captured once into a HIP graph and replayed with one call.  No counterpart in
+ * the reference (its bootstraps are host loops): for callers that repeat one batch shape on fixed
+ * device buffers (a serving loop with staging buffers, a benchmark), where a replay issues the
+ * whole launch sequence as one submission.  The graph holds the pointers it was created with and
+ * its own scratch for the extracted LWEs (count * (N + 1) words, allocated at creation): every
+ * replay bootstraps the current contents of lwes/tv into out, asynchronously on the context's
+ * stream; replays and other calls on the same context are ordered by that stream.  The context's
+ * settings at creation (kernel variants, thresholds) are baked in.  Destroy the graph before its
+ * context. */
+typedef struct tp_graph tp_graph;
+int tp_graph_bootstrap_create(tp_context* ctx, size_t count, const uint64_t* lwes, const uint64_t* tv,
+                              size_t tv_stride, uint64_t* out, tp_graph** graph);
+int tp_graph_launch(tp_graph* graph);
+int tp_graph_destroy(tp_graph* graph);
+
 /* Tuning (no counterpart in the reference).
  * TP_OPT_BR_WAVES selects the one-wave-per-ciphertext blind-rotation kernel: 2 (default; 256
  * registers per lane, two ciphertexts' waves share a SIMD) or 1 (512 registers, no spills in the
@@ -257,11 +273,19 @@ int tp_bootstrap_batch_device_noalloc(tp_context* ctx, size_t count, const uint6
  * default; 2: 256).
  * tp_autotune times every variant built for the context on `count` synthetic ciphertexts with
  * the imported key (the engine-1 variants only when the context opted in to engine 1), keeps
- * the fastest one-wave-per-ciphertext variant (engine and waves), moves
- * the digit-parallel threshold to the side of `count` that won, and reports ms[0] (NTT engine,
- * two waves per SIMD), ms[1] (one wave), ms[2] (digit-parallel), ms[3] (floating-point engine,
- * one wave) and ms[4] (two waves), -1 where a variant is not built or not usable; ms must hold 5
- * floats.  The context's later launches, including the server's chunks, follow these settings. */
+ * the fastest one-wave-per-ciphertext variant (engine and waves), times that variant on a
+ * persistent grid (TP_OPT_BR_GRID = the ciphertexts resident at once) when `count` exceeds it and
+ * keeps the faster grid, moves the digit-parallel threshold to the side of `count` that won, and
+ * reports ms[0] (NTT engine, two waves per SIMD), ms[1] (one wave), ms[2] (digit-parallel), ms[3]
+ * (floating-point engine, one wave), ms[4] (two waves) and ms[5] (the winner on the persistent
+ * grid), -1 where a variant is not built, not usable or not timed; ms must hold 6 floats.  The
+ * context's later launches, including the server's chunks, follow these settings.
+ * tp_autotune_cached(ctx, count, path, &hit, ms) keeps those choices in a JSON file at `path`,
+ * keyed by device architecture, CU count, (N, n, b, l, method), count and the engine-1 opt-in: a
+ * hit applies the stored settings (hit = 1, ms = the stored times) without launching anything, a
+ * miss runs tp_autotune and adds its result to the file (created if absent; rewritten through
+ * path.tmp and a rename, so processes tuning into one file at the same time may lose each other's
+ * entries).  A malformed file or an entry this context cannot use gives TP_E_INVALID. */
 #define TP_OPT_BR_WAVES 1
 #define TP_OPT_BR_SPLIT_MAX 2
 #define TP_OPT_BR_ENGINE 3
@@ -273,6 +297,7 @@ int tp_bootstrap_batch_device_noalloc(tp_context* ctx, size_t count, const uint6
 int tp_set_option(tp_context* ctx, int option, int value);
 int tp_get_option(tp_context* ctx, int option, int* value);
 int tp_autotune(tp_context* ctx, size_t count, float* ms);
+int tp_autotune_cached(tp_context* ctx, size_t count, const char* path, int* hit, float* ms);
 
 /* Device memory helpers (so that callers need no HIP headers). */
 int tp_malloc(tp_context* ctx, uint64_t** ptr, size_t words);
