@@ -66,8 +66,11 @@ br_kernel(BrArgs A) {
     using S = Shape<LOGN>;
     constexpr int N = S::N;
     constexpr int NC = METHOD == 0 ? 1 : (METHOD == 1 ? 2 : 3);  // RGSWs per step
-    __shared__ uint64_t buf[S::LDS_WORDS];
-    __shared__ uint64_t src[METHOD == 0 ? S::LDS_WORDS : 1];
+    // binary: transpose buffer and parked digit source side by side, so that together they hold
+    // the 2N-word negacyclic extension the rotation reads (below)
+    __shared__ uint64_t lds[METHOD == 0 ? 2 * S::LDS_WORDS : S::LDS_WORDS];
+    uint64_t* const buf = lds;
+    uint64_t* const src = METHOD == 0 ? lds + S::LDS_WORDS : lds;
     const int L = threadIdx.x;
     const uint64_t mask = (1ULL << A.base_log2) - 1;
     const size_t rgsw_words = (size_t)4 * N * LD;
@@ -131,21 +134,38 @@ br_kernel(BrArgs A) {
             for (int h = 0; h < 2; ++h) {
                 if constexpr (METHOD == 0) {
                     // the polynomial the digits are taken from, X^e0 acc_h - acc_h, is formed once
-                    // per half (through the LDS copy of acc_h for the rotation) and parked in LDS,
-                    // where every digit re-reads it (kept out of registers)
-                    poly_sync<S::WAVES>();
+                    // per half and parked in LDS, where every digit re-reads it (kept out of
+                    // registers).  With e' = e0 mod N and E = (-acc_h | acc_h) (e0 < N) or
+                    // (acc_h | -acc_h) (e0 >= N, X^e0 = -X^e'), coefficient i of X^e0 acc_h is
+                    // E[i + N - e']: lane L reads one base address plus constant offsets.
+                    constexpr uint32_t kStride = S::P + S::P / 16;  // phys(idx1(L, k + 1)) - phys(idx1(L, k))
+                    constexpr uint32_t kHi = N + N / 16;             // phys(j + N) - phys(j)
+                    uint64_t na[kE], zero[kE];
 #pragma unroll
-                    for (int k = 0; k < kE; ++k) src[phys(idx1<LOGN>(L, k))] = acc[0][k];
-                    poly_sync<S::WAVES>();
-                    uint64_t t[kE];
-                    int Lh = L;  // re-read per half: the rotation's addresses are formed here
+                    for (int k = 0; k < kE; ++k) zero[k] = 0;
+                    glv::sub_lc_g<kE>(zero, acc[0], na);  // -acc_h, canonical
+                    int Lh = L;  // re-read per half: the addresses are formed here
                     asm volatile("" : "+v"(Lh));
-                    rotate_from<LOGN>(t, src, e0, Lh, true);
-#pragma unroll
-                    for (int k = 0; k < kE; ++k) t[k] = gl::sub(t[k], acc[0][k]);
+                    const uint32_t pl = (uint32_t)Lh + ((uint32_t)Lh >> 4);  // phys(idx1(L, 0))
+                    // which half of E receives acc_h and which -acc_h: a wave-uniform choice of
+                    // base address, not a per-lane select of values
+                    const uint32_t off_a = e0 >= (uint32_t)N ? 0u : kHi, off_n = kHi - off_a;
                     poly_sync<S::WAVES>();
 #pragma unroll
-                    for (int k = 0; k < kE; ++k) src[phys(idx1<LOGN>(L, k))] = t[k];
+                    for (int k = 0; k < kE; ++k) {
+                        lds[pl + off_a + k * kStride] = acc[0][k];
+                        lds[pl + off_n + k * kStride] = na[k];
+                    }
+                    poly_sync<S::WAVES>();
+                    const uint32_t j0 = (uint32_t)Lh + (uint32_t)N - (e0 & (uint32_t)(N - 1));
+                    const uint32_t p0 = j0 + (j0 >> 4);
+                    uint64_t u[kE], t[kE];
+#pragma unroll
+                    for (int k = 0; k < kE; ++k) u[k] = lds[p0 + k * kStride];
+                    glv::sub_lc_g<kE>(u, acc[0], t);  // both canonical: canonical
+                    poly_sync<S::WAVES>();
+#pragma unroll
+                    for (int k = 0; k < kE; ++k) src[pl + k * kStride] = t[k];
                     poly_sync<S::WAVES>();
                 }
                 uint64_t xc[METHOD == 0 ? 1 : kE];  // x + C mod Q of the half (signed digits)
