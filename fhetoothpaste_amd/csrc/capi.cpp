@@ -878,14 +878,19 @@ int tp_autotune(tp_context* c, size_t count, float* ms) {
     const int saved_engine = c->br_engine, saved_fw = c->fft_waves;
     const int saved_grid = c->br_grid;
     float t_var[6] = {-1.0f, -1.0f, -1.0f, -1.0f, -1.0f, -1.0f};
+    // a warm-up launch, then the faster of two timed launches (one timing flips close variants)
     auto time_launch = [&](float& t) {
-        rc = launch_eval(c, count, d_lwe, d_tv, 0, d_out, nullptr, 0, 0);  // warm-up
-        if (!rc && (hipEventRecord(e0, c->stream) != hipSuccess))
-            rc = fail(TP_E_DEVICE, "event record");
-        if (!rc) rc = launch_eval(c, count, d_lwe, d_tv, 0, d_out, nullptr, 0, 0);
-        if (!rc && (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-                    hipEventElapsedTime(&t, e0, e1) != hipSuccess))
-            rc = fail(TP_E_DEVICE, "autotune timing");
+        rc = launch_eval(c, count, d_lwe, d_tv, 0, d_out, nullptr, 0, 0);
+        t = -1.0f;
+        for (int r = 0; r < 2 && !rc; ++r) {
+            float tr = -1.0f;
+            if (hipEventRecord(e0, c->stream) != hipSuccess) rc = fail(TP_E_DEVICE, "event record");
+            if (!rc) rc = launch_eval(c, count, d_lwe, d_tv, 0, d_out, nullptr, 0, 0);
+            if (!rc && (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                        hipEventElapsedTime(&tr, e0, e1) != hipSuccess))
+                rc = fail(TP_E_DEVICE, "autotune timing");
+            if (!rc && (t < 0 || tr < t)) t = tr;
+        }
     };
     c->br_grid = 0;
     for (int v = 0; v < 5 && !rc; ++v) {

@@ -26,12 +26,15 @@ __global__ void __launch_bounds__(2 * LD * 64) br_latency_kernel(BrArgs A) {
     static_assert(S::P == 64, "one wave per polynomial");
     constexpr int NC = METHOD == 2 ? 3 : (METHOD == 1 ? 2 : 1);
     constexpr int G = 2 * LD;
-    // accumulator halves (coefficients, phys(idx1) positions; also waves 0/1's transpose buffers
-    // in the reduction phase) and the partial products of every wave (NTT domain, idx3 positions;
-    // the first words of a wave's region are its transpose buffer in the product phase)
+    // accL: ternary/BMMP: the accumulator halves (coefficients, phys(idx1) positions); binary: the
+    // halves' digit sources X^e0 acc_h - acc_h (the accumulator itself stays in the registers of
+    // waves 0 and 1).  red: the partial products of every wave (NTT domain, idx3 positions; the
+    // first words of a wave's region are its transpose buffer in the product phase, and in the
+    // binary step prologue red[2 + p] holds wave p's negacyclic extension of acc_p).
     __shared__ uint64_t accL[2][S::LDS_WORDS];
     __shared__ uint64_t red[G][2 * N];
     static_assert(S::LDS_WORDS <= 2 * N, "transpose buffer inside the product region");
+    static_assert(G >= 4, "binary prologue uses red[2] and red[3]");
 
     const int w = threadIdx.x >> 6;
     const int h = w / LD, j = w % LD;
@@ -50,16 +53,18 @@ __global__ void __launch_bounds__(2 * LD * 64) br_latency_kernel(BrArgs A) {
             A.tv + (A.tv_index ? (size_t)A.tv_index[slot] * 2 * N : (size_t)slot * A.tv_stride);
         const int i0 = A.slot_step ? (int)A.slot_step[slot] : 0;
         const int i1 = (A.chunk_steps > 0 && i0 + A.chunk_steps < total) ? i0 + A.chunk_steps : total;
+        uint64_t accR[kE];  // binary: wave p < 2 holds accumulator half p (coefficients idx1(L, k))
         if (w < 2) {
             if (i0 == 0) {
-                uint64_t x[kE];
-                rotate_from<LOGN>(x, tv + w * N, (uint32_t)lwe[A.n], L, false);
-#pragma unroll
-                for (int k = 0; k < kE; ++k) accL[w][phys(idx1<LOGN>(L, k))] = x[k];
+                rotate_from<LOGN>(accR, tv + w * N, (uint32_t)lwe[A.n], L, false);
             } else {
                 const uint64_t* st = A.state + (size_t)slot * 2 * N + (size_t)w * N;
 #pragma unroll
-                for (int k = 0; k < kE; ++k) accL[w][phys(idx1<LOGN>(L, k))] = st[idx1<LOGN>(L, k)];
+                for (int k = 0; k < kE; ++k) accR[k] = st[idx1<LOGN>(L, k)];
+            }
+            if constexpr (METHOD != 0) {
+#pragma unroll
+                for (int k = 0; k < kE; ++k) accL[w][phys(idx1<LOGN>(L, k))] = accR[k];
             }
         }
         __syncthreads();
@@ -79,14 +84,55 @@ __global__ void __launch_bounds__(2 * LD * 64) br_latency_kernel(BrArgs A) {
             const uint64_t* key = A.brk + (size_t)i * NC * rgsw_words;
             uint64_t* tb = red[w];
 
+            if constexpr (METHOD == 0) {
+                // step prologue: wave p parks X^e0 acc_p - acc_p in accL[p].  With e' = e0 mod N and
+                // E = (-acc_p | acc_p) (e0 < N) or (acc_p | -acc_p) (e0 >= N) in red[2 + p] (unpadded:
+                // consecutive lanes touch consecutive words), coefficient i of X^e0 acc_p is
+                // E[i + N - e'].
+                if (w < 2) {
+                    uint64_t na[kE], zero[kE], u[kE], t[kE];
+#pragma unroll
+                    for (int k = 0; k < kE; ++k) zero[k] = 0;
+                    glv::sub_lc_g<kE>(zero, accR, na);  // -acc_p, canonical
+                    uint64_t* E = red[2 + w];
+                    const uint32_t off_a = e0 >= (uint32_t)N ? 0u : (uint32_t)N, off_n = (uint32_t)N - off_a;
+#pragma unroll
+                    for (int k = 0; k < kE; ++k) {
+                        E[off_a + (uint32_t)idx1<LOGN>(L, k)] = accR[k];
+                        E[off_n + (uint32_t)idx1<LOGN>(L, k)] = na[k];
+                    }
+                    poly_sync<S::WAVES>();
+                    const uint32_t j0 = (uint32_t)L + (uint32_t)N - (e0 & (uint32_t)(N - 1));
+#pragma unroll
+                    for (int k = 0; k < kE; ++k) u[k] = E[j0 + (uint32_t)(k * S::P)];
+                    glv::sub_lc_g<kE>(u, accR, t);  // both canonical: canonical
+#pragma unroll
+                    for (int k = 0; k < kE; ++k) accL[w][phys(idx1<LOGN>(L, k))] = t[k];
+                }
+                __syncthreads();
+            }
+
             // product phase: digit j of half h
             uint64_t d[kE];
             if constexpr (METHOD == 0) {
                 const int shift = A.first_shift - j * A.base_log2;
-                rotate_from<LOGN>(d, accL[h], e0, L, true);
+                const uint32_t* src32 = (const uint32_t*)accL[h];
+                const uint32_t base = (uint32_t)L + ((uint32_t)L >> 4);
+                constexpr uint32_t kStride = S::P + S::P / 16;
+                if (shift >= 32 && A.base_log2 < 32) {
 #pragma unroll
-                for (int k = 0; k < kE; ++k)
-                    d[k] = (gl::sub(d[k], accL[h][phys(idx1<LOGN>(L, k))]) >> shift) & mask;
+                    for (int k = 0; k < kE; ++k)
+                        d[k] = __builtin_amdgcn_ubfe(src32[2 * (base + k * kStride) + 1], (uint32_t)(shift - 32),
+                                                     (uint32_t)A.base_log2);
+                } else if (shift + A.base_log2 <= 32 && A.base_log2 < 32) {
+#pragma unroll
+                    for (int k = 0; k < kE; ++k)
+                        d[k] = __builtin_amdgcn_ubfe(src32[2 * (base + k * kStride)], (uint32_t)shift,
+                                                     (uint32_t)A.base_log2);
+                } else {
+#pragma unroll
+                    for (int k = 0; k < kE; ++k) d[k] = (accL[h][base + k * kStride] >> shift) & mask;
+                }
                 if (A.base_log2 <= 16) {
 #pragma unroll
                     for (int k = 0; k < kE / 2; ++k) ct_bfly_small_first(d[k], d[k + kE / 2]);
@@ -135,13 +181,20 @@ __global__ void __launch_bounds__(2 * LD * 64) br_latency_kernel(BrArgs A) {
                     constexpr int c0 = 2 * G;
                     U64x2 na[2], nb[2];
                     if constexpr (G < 3) load_group(na, nb, c0 + 2);
+                    // coefficients 2 c0 .. 2 c0 + 3 against both key rows, one group of eight
+                    uint64_t g8[8], d8[8], k8[8];
 #pragma unroll
-                    for (int q = 0; q < 2; ++q) {
-                        const int c = c0 + q;
-                        out[0][2 * c] = gl::mul_lazy(d[2 * c], va[q].x);
-                        out[0][2 * c + 1] = gl::mul_lazy(d[2 * c + 1], va[q].y);
-                        out[1][2 * c] = gl::mul_lazy(d[2 * c], vb[q].x);
-                        out[1][2 * c + 1] = gl::mul_lazy(d[2 * c + 1], vb[q].y);
+                    for (int m = 0; m < 4; ++m) {
+                        g8[m] = g8[m + 4] = 0;
+                        d8[m] = d8[m + 4] = d[2 * c0 + m];
+                    }
+                    k8[0] = va[0].x; k8[1] = va[0].y; k8[2] = va[1].x; k8[3] = va[1].y;
+                    k8[4] = vb[0].x; k8[5] = vb[0].y; k8[6] = vb[1].x; k8[7] = vb[1].y;
+                    glv::mac_lazy_g<8>(g8, d8, k8);
+#pragma unroll
+                    for (int m = 0; m < 4; ++m) {
+                        out[0][2 * c0 + m] = g8[m];
+                        out[1][2 * c0 + m] = g8[m + 4];
                     }
                     if constexpr (G < 3) {
 #pragma unroll
@@ -226,16 +279,25 @@ __global__ void __launch_bounds__(2 * LD * 64) br_latency_kernel(BrArgs A) {
 
             // wave p inverse-transforms the sum of output half p and updates that half
             if (w < 2) {
-                uint64_t s[kE], prev[kE];
+                uint64_t s[kE];
 #pragma unroll
-                for (int k = 0; k < kE; ++k) {
-                    s[k] = red[0][w * N + idx3(L, k)];
-                    prev[k] = accL[w][phys(idx1<LOGN>(L, k))];
+                for (int k = 0; k < kE; ++k) s[k] = red[0][w * N + idx3(L, k)];
+                if constexpr (METHOD == 0) {
+                    uint64_t t[kE];
+                    ntt_inverse<LOGN>(s, red[2 + w], A.itw, L);  // canonical out
+                    glv::add_cc_g<kE>(s, accR, t, glv::qm1_const());
+#pragma unroll
+                    for (int k = 0; k < kE; ++k) accR[k] = t[k];
+                } else {
+                    uint64_t prev[kE];
+#pragma unroll
+                    for (int k = 0; k < kE; ++k) prev[k] = accL[w][phys(idx1<LOGN>(L, k))];
+                    ntt_inverse<LOGN>(s, accL[w], A.itw, L);
+                    poly_sync<S::WAVES>();
+#pragma unroll
+                    for (int k = 0; k < kE; ++k)
+                        accL[w][phys(idx1<LOGN>(L, k))] = gl::canon(gl::add_lc(s[k], prev[k]));
                 }
-                ntt_inverse<LOGN>(s, accL[w], A.itw, L);
-                poly_sync<S::WAVES>();
-#pragma unroll
-                for (int k = 0; k < kE; ++k) accL[w][phys(idx1<LOGN>(L, k))] = gl::canon(gl::add_lc(s[k], prev[k]));
             }
             __syncthreads();
         }
@@ -244,7 +306,7 @@ __global__ void __launch_bounds__(2 * LD * 64) br_latency_kernel(BrArgs A) {
         if (w < 2) {
             uint64_t x[kE];
 #pragma unroll
-            for (int k = 0; k < kE; ++k) x[k] = accL[w][phys(idx1<LOGN>(L, k))];
+            for (int k = 0; k < kE; ++k) x[k] = METHOD == 0 ? accR[k] : accL[w][phys(idx1<LOGN>(L, k))];
             if (i1 < total) {
                 // chunk boundary: park the accumulator half for the next launch
                 uint64_t* st = A.state + (size_t)slot * 2 * N + (size_t)w * N;
