@@ -59,6 +59,20 @@ __device__ __forceinline__ void rlwe_ks_core(uint64_t (&out_a)[kE], uint64_t (&o
     const uint64_t mask = (1ULL << A.base_log2) - 1;
 #pragma unroll
     for (int k = 0; k < kE; ++k) out_a[k] = out_b[k] = 0;
+    // x + C mod Q (canonical a: the corrector sum is formed once, not per digit) and the first-level
+    // constants of the signed digits (offsets 2^(b-1) and `least`)
+    uint64_t xc[kE], cv[kE];
+#pragma unroll
+    for (int k = 0; k < kE; ++k) cv[k] = A.corrector;
+    {
+        uint64_t ac[kE];
+#pragma unroll
+        for (int k = 0; k < kE; ++k) ac[k] = gl::canon(a[k]);
+        glv::add_cc_g<kE>(ac, cv, xc, glv::qm1_const());
+    }
+    uint64_t kp_n, km_n, kp_l, km_l;
+    signed_first_consts(1ULL << (A.base_log2 - 1), kp_n, km_n);
+    signed_first_consts(A.least, kp_l, km_l);
 #pragma unroll 1
     for (int j = 0; j < A.digits; ++j) {
         const bool last = (j == A.max_digits - 1);
@@ -66,12 +80,17 @@ __device__ __forceinline__ void rlwe_ks_core(uint64_t (&out_a)[kE], uint64_t (&o
         const uint64_t dm = last ? (2 * A.least - 1) : mask;
         const uint64_t delta = last ? A.least : (1ULL << (A.base_log2 - 1));
         uint64_t d[kE];
+        if (shift >= 32 && A.base_log2 < 32) {
 #pragma unroll
-        for (int k = 0; k < kE; ++k) d[k] = (gl::add(a[k], A.corrector) >> shift) & dm;
+            for (int k = 0; k < kE; ++k)
+                d[k] = __builtin_amdgcn_ubfe(glv::hi32(xc[k]), (uint32_t)(shift - 32), (uint32_t)A.base_log2);
+        } else {
+#pragma unroll
+            for (int k = 0; k < kE; ++k) d[k] = (xc[k] >> shift) & dm;
+        }
         // the first transform level folds the offset in while the digits are small
         if (dm < (1ULL << 16)) {
-            uint64_t kp, km;
-            signed_first_consts(delta, kp, km);
+            const uint64_t kp = last ? kp_l : kp_n, km = last ? km_l : km_n;
 #pragma unroll
             for (int k = 0; k < kE / 2; ++k) ct_bfly_signed_first(d[k], d[k + kE / 2], kp, km);
         } else {
@@ -98,13 +117,21 @@ __device__ __forceinline__ void rlwe_ks_core(uint64_t (&out_a)[kE], uint64_t (&o
             constexpr int c0 = 2 * G;
             U64x2 na[2], nb[2];
             if constexpr (G < 3) load_group(na, nb, c0 + 2);
+            // coefficients 2 c0 .. 2 c0 + 3 against both key rows, one group of eight
+            uint64_t g8[8], d8[8], k8[8];
 #pragma unroll
-            for (int q = 0; q < 2; ++q) {
-                const int c = c0 + q;
-                out_a[2 * c] = gl::mac_lazy(d[2 * c], va[q].x, out_a[2 * c]);
-                out_a[2 * c + 1] = gl::mac_lazy(d[2 * c + 1], va[q].y, out_a[2 * c + 1]);
-                out_b[2 * c] = gl::mac_lazy(d[2 * c], vb[q].x, out_b[2 * c]);
-                out_b[2 * c + 1] = gl::mac_lazy(d[2 * c + 1], vb[q].y, out_b[2 * c + 1]);
+            for (int m = 0; m < 4; ++m) {
+                g8[m] = out_a[2 * c0 + m];
+                g8[m + 4] = out_b[2 * c0 + m];
+                d8[m] = d8[m + 4] = d[2 * c0 + m];
+            }
+            k8[0] = va[0].x; k8[1] = va[0].y; k8[2] = va[1].x; k8[3] = va[1].y;
+            k8[4] = vb[0].x; k8[5] = vb[0].y; k8[6] = vb[1].x; k8[7] = vb[1].y;
+            glv::mac_lazy_g<8>(g8, d8, k8);
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                out_a[2 * c0 + m] = g8[m];
+                out_b[2 * c0 + m] = g8[m + 4];
             }
             if constexpr (G < 3) {
 #pragma unroll
