@@ -253,7 +253,7 @@ br_kernel(BrArgs A) {
                             }
                             __builtin_amdgcn_sched_barrier(0);
                         };
-                        ntt_forward<LOGN, 1>(d, buf, A.tw, L, [&]() __attribute__((always_inline)) {
+                        ntt_fwd<LOGN, 1>(d, buf, A.tw, L, [&]() __attribute__((always_inline)) {
                             load_group(va, vb, 0);
                         });
                         static_for<4>([&](auto G) __attribute__((always_inline)) {
@@ -320,7 +320,7 @@ br_kernel(BrArgs A) {
                             __builtin_amdgcn_sched_barrier(0);
                         };
                         PairLoads cur;
-                        ntt_forward<LOGN, 1>(d, buf, A.tw, L, [&]() __attribute__((always_inline)) {
+                        ntt_fwd<LOGN, 1>(d, buf, A.tw, L, [&]() __attribute__((always_inline)) {
                             load_pair(cur, 0);
                         });
                         static_for<kE / 2>([&](auto C) __attribute__((always_inline)) {
@@ -468,7 +468,7 @@ br_kernel(BrArgs A) {
             // one half at a time through a reference, so acc is never indexed at run time
             auto emit = [&](uint64_t (&x)[kE], uint64_t* o) __attribute__((always_inline)) {
                 if (A.out_rlwe_ntt) {
-                    ntt_forward<LOGN>(x, buf, A.tw, L);
+                    ntt_fwd<LOGN>(x, buf, A.tw, L);
 #pragma unroll
                     for (int k = 0; k < kE; ++k) o[idx3(L, k)] = gl::canon(x[k]);
                 } else {

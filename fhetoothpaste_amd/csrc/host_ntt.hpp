@@ -34,6 +34,35 @@ struct HostNtt {
         }
     }
 
+    // Tables of the twisted forward transform at N = 1024 (2N words, placed after psi_rev):
+    // [0, N): rho_j^i, rho_j = psi^(2 brv4(j) + 1), for coefficient i of block j (64 consecutive
+    //   NTT indices) in key-row pair order: word 2 (64 c + L) + q holds lane L's k = 2c + q, the
+    //   pass-2 element i = 4 k + (L & 3) of block j = L >> 2;
+    // [N, 2N): at N + 256 + q, w^(2 brv4(q & 15)), and at N + 512 + q, w^(brv5(q & 31)),
+    //   w = psi^32: the cyclic 64-point transform's last two levels at the places where the
+    //   plain transform keeps its levels 8 and 9; the rest is zero.
+    std::vector<uint64_t> twisted64() const {
+        std::vector<uint64_t> t(2 * (size_t)N, 0);
+        if (N != 1024) return t;
+        auto brv = [](uint32_t x, int bits) {
+            uint32_t r = 0;
+            for (int b = 0; b < bits; ++b) { r = (r << 1) | (x & 1); x >>= 1; }
+            return r;
+        };
+        const uint64_t psi = psi_rev[N / 2];
+        const uint64_t w = gl::pow(psi, 32);
+        for (uint32_t c = 0; c < 8; ++c)
+            for (uint32_t L = 0; L < 64; ++L)
+                for (uint32_t q = 0; q < 2; ++q) {
+                    const uint32_t k = 2 * c + q, j = L >> 2, i = 4 * k + (L & 3);
+                    const uint64_t rho = gl::pow(psi, 2 * brv(j, 4) + 1);
+                    t[2 * (64 * c + L) + q] = gl::pow(rho, i);
+                }
+        for (uint32_t q = 0; q < 256; ++q) t[N + 256 + q] = gl::pow(w, 2 * brv(q & 15, 4));
+        for (uint32_t q = 0; q < 512; ++q) t[N + 512 + q] = gl::pow(w, brv(q & 31, 5));
+        return t;
+    }
+
     void forward(uint64_t* a) const {
         for (uint32_t m = 1, t = N / 2; m < N; m <<= 1, t >>= 1)
             for (uint32_t i = 0; i < m; ++i) {

@@ -25,7 +25,7 @@ __global__ void __launch_bounds__(256) ntt_batch_kernel(uint64_t* __restrict__ o
         if (!INVERSE) {
 #pragma unroll
             for (int k = 0; k < kE; ++k) x[k] = src[idx1<LOGN>(L, k)];
-            ntt_forward<LOGN>(x, lds[g], tw, L);
+            ntt_fwd<LOGN>(x, lds[g], tw, L);
             transpose<LOGN, Layout::kPass3, Layout::kPass1>(x, lds[g], L);  // coalesced writes
             if (live)
 #pragma unroll

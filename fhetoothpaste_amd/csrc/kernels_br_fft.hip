@@ -209,7 +209,7 @@ br_fft_kernel(BrArgs A, const cf::C* __restrict__ fkey, const cf::C* __restrict_
             uint64_t* o = A.out_rlwe + (size_t)slot * 2 * N + h * N;
             if (A.out_rlwe_ntt) {
                 poly_sync<1>();
-                ntt_forward<LOGN>(acc, src, A.tw, L);
+                ntt_fwd<LOGN>(acc, src, A.tw, L);
 #pragma unroll
                 for (int k = 0; k < kE; ++k) o[idx3(L, k)] = gl::canon(acc[k]);
             } else {

@@ -176,7 +176,7 @@ __global__ void __launch_bounds__(2 * LD * 64) br_latency_kernel(BrArgs A) {
                     }
                     __builtin_amdgcn_sched_barrier(0);
                 };
-                ntt_forward<LOGN, 1>(d, tb, A.tw, L, [&]() __attribute__((always_inline)) { load_group(va, vb, 0); });
+                ntt_fwd<LOGN, 1>(d, tb, A.tw, L, [&]() __attribute__((always_inline)) { load_group(va, vb, 0); });
                 static_for<4>([&](auto G) __attribute__((always_inline)) {
                     constexpr int c0 = 2 * G;
                     U64x2 na[2], nb[2];
@@ -228,7 +228,7 @@ __global__ void __launch_bounds__(2 * LD * 64) br_latency_kernel(BrArgs A) {
                     return e >= (uint32_t)N ? U64x2{gl::neg(v.x), gl::neg(v.y)} : v;
                 };
                 PairLoads cur;
-                ntt_forward<LOGN, 1>(d, tb, A.tw, L, [&]() __attribute__((always_inline)) { load_pair(cur, 0); });
+                ntt_fwd<LOGN, 1>(d, tb, A.tw, L, [&]() __attribute__((always_inline)) { load_pair(cur, 0); });
                 static_for<kE / 2>([&](auto C) __attribute__((always_inline)) {
                     constexpr int c = C;
                     PairLoads nxt;
@@ -327,7 +327,7 @@ __global__ void __launch_bounds__(2 * LD * 64) br_latency_kernel(BrArgs A) {
                     uint64_t* o = A.out_rlwe + (size_t)slot * 2 * N + (size_t)w * N;
                     if (A.out_rlwe_ntt) {
                         poly_sync<S::WAVES>();
-                        ntt_forward<LOGN>(x, accL[w], A.tw, L);
+                        ntt_fwd<LOGN>(x, accL[w], A.tw, L);
 #pragma unroll
                         for (int k = 0; k < kE; ++k) o[idx3(L, k)] = gl::canon(x[k]);
                     } else {

@@ -112,7 +112,7 @@ __device__ __forceinline__ void rlwe_ks_core(uint64_t (&out_a)[kE], uint64_t (&o
             }
             __builtin_amdgcn_sched_barrier(0);
         };
-        ntt_forward<LOGN, 1>(d, lds, A.tw, L, [&]() __attribute__((always_inline)) { load_group(va, vb, 0); });
+        ntt_fwd<LOGN, 1>(d, lds, A.tw, L, [&]() __attribute__((always_inline)) { load_group(va, vb, 0); });
         static_for<4>([&](auto G) __attribute__((always_inline)) {
             constexpr int c0 = 2 * G;
             U64x2 na[2], nb[2];
@@ -152,7 +152,7 @@ __device__ __forceinline__ void automorphism_eval(uint64_t (&a)[kE], uint64_t (&
                                                   const KeySwitchArgs& A, const uint64_t* ksk, int L) {
     automorphism_regs<LOGN>(a, lds, t, L);
     automorphism_regs<LOGN>(b, lds, t, L);
-    ntt_forward<LOGN>(b, lds, A.tw, L);
+    ntt_fwd<LOGN>(b, lds, A.tw, L);
     uint64_t oa[kE], ob[kE];
     rlwe_ks_core<LOGN>(oa, ob, a, lds, A, ksk, L);
 #pragma unroll
@@ -189,7 +189,7 @@ automorphism_kernel(KeySwitchArgs A, const uint64_t* in, uint64_t* out, int coun
 #pragma unroll
         for (int k = 0; k < kE; ++k) b[k] = src[N + idx1<LOGN>(L, k)];
         automorphism_regs<LOGN>(b, lds, t, L);
-        ntt_forward<LOGN>(b, lds, A.tw, L);
+        ntt_fwd<LOGN>(b, lds, A.tw, L);
 #pragma unroll
         for (int k = 0; k < kE; ++k) {
             oa[k] = gl::neg(gl::canon(oa[k]));
@@ -244,7 +244,7 @@ rlwe_keyswitch_kernel(KeySwitchArgs A, const uint64_t* in, uint64_t* out, int co
 #pragma unroll
             for (int k = 0; k < kE; ++k) b[k] = src[N + idx1<LOGN>(L, k)];
             automorphism_regs<LOGN>(b, lds, t, L);
-            ntt_forward<LOGN>(b, lds, A.tw, L);
+            ntt_fwd<LOGN>(b, lds, A.tw, L);
         } else {
 #pragma unroll
             for (int k = 0; k < kE; ++k) b[k] = 0;
@@ -441,7 +441,7 @@ __global__ void __launch_bounds__(Shape<LOGN>::P) trace_kernel(KeySwitchArgs A, 
             uint64_t y[kE];
 #pragma unroll
             for (int k = 0; k < kE; ++k) y[k] = h ? o[1][k] : o[0][k];
-            ntt_forward<LOGN>(y, lds, A.tw, L);
+            ntt_fwd<LOGN>(y, lds, A.tw, L);
             transpose<LOGN, Layout::kPass3, Layout::kPass1>(y, lds, L);  // coalesced writes
 #pragma unroll
             for (int k = 0; k < kE; ++k) dst[h * N + idx1<LOGN>(L, k)] = gl::canon(y[k]);

@@ -273,6 +273,14 @@ __device__ __forceinline__ void pow2_products(const uint64_t (&v)[8], uint64_t (
     pow2_kind_group<E, glv::P2::kLo>(v, t, qm1);
     pow2_kind_group<E, glv::P2::kMid>(v, t, qm1);
     pow2_kind_group<E, glv::P2::kNeg>(v, t, qm1);
+    // twiddle 1: the product is v itself, made canonical for the butterfly's sum and difference
+    constexpr KindList one = kind_list<E>(glv::P2::kOne);
+    if constexpr (one.n > 0) {
+        uint64_t xv[one.n];
+        static_for<one.n>([&](auto M_) __attribute__((always_inline)) { xv[M_] = v[one.b[M_]]; });
+        glv::canon_g<one.n>(xv, qm1);
+        static_for<one.n>([&](auto M_) __attribute__((always_inline)) { t[one.b[M_]] = xv[M_]; });
+    }
 }
 
 // Cooley-Tukey outputs of the 8 butterflies from their canonical products t (neg(b): the
@@ -387,6 +395,109 @@ __device__ __forceinline__ void ntt_forward(uint64_t (&x)[kE], uint64_t* lds, co
             for (int b = 0; b < 8; ++b) w[b] = wn[b];
         }
     });
+}
+
+// ---------------------------------------------------------------- forward, N = 1024, twisted
+//
+// After the four power-of-two levels, block j (64 consecutive NTT indices) holds a polynomial B_j
+// modulo X^64 - rho_j^64, rho_j = psi^(2 brv4(j) + 1), and its outputs are B_j(rho_j w^brv6(t)),
+// w = psi^32 = 7^((Q-1)/64), a 64th root of unity: a cyclic 64-point transform of the coefficients
+// times rho_j^i.  So the six levels with table twiddles become one product per coefficient (the
+// twist, table) and a cyclic transform whose twiddles w^(brv_t(m) (32 >> t)) are powers of two
+// (w = 2^39): its first four levels run with shift-based products on compile-time exponents, its
+// last two (where a lane's sub-blocks differ from lane to lane) with table twiddles.  The outputs
+// are the same words in the same places as ntt_forward's.
+
+// Exponents of 2 of the cyclic 64-point transform's twiddles, level t < 4, sub-block m.
+constexpr int kDft64Pow2[4][8] = {
+    {0, 0, 0, 0, 0, 0, 0, 0}, {0, 48, 0, 0, 0, 0, 0, 0}, {0, 48, 120, 168, 0, 0, 0, 0},
+    {0, 48, 120, 168, 156, 12, 84, 132}};
+
+// Level t of the cyclic transform in the pass-2 layout (lane L holds i = 4 k + (L & 3) of block
+// L >> 2): distance 8 >> t in k, sub-block k >> (4 - t).
+template <int T>
+struct DftP2 {
+    static constexpr int H = 8 >> T;
+    static constexpr int e(int b) { return kDft64Pow2[T][bf_u(H, b) >> (4 - T)]; }
+    static constexpr bool neg(int b) { return glv::pow2_kind(e(b)).neg; }
+};
+
+// t64: [0, N) twist factors rho_j^i in key-row order (lane L, pair c: words 2 (64 c + L) + {0, 1}
+// for k = 2c, 2c + 1), [N, 2N) the twiddles of the cyclic transform's last two levels at the
+// indices ntt_forward reads its levels 8 and 9 from.
+template <int FIRST_LV = 0, class AtLast = NoPrefetch>
+__device__ __forceinline__ void ntt_forward_t64(uint64_t (&x)[kE], uint64_t* lds, const uint64_t* t64, int L,
+                                                AtLast&& at_last = AtLast{}) {
+    constexpr int LOGN = 10;
+    constexpr int N = 1 << LOGN;
+    const uint64_t qm1 = glv::qm1_const();
+    t64 = opaque(t64);
+    fwd_pass1<LOGN, FIRST_LV>(x);
+    // twist factors requested before the transpose
+    const uint64_t* tf = opaque_after(t64, x[0]);
+    U64x2 rf[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) rf[c] = ld_pair(tf, c * 64 + L);
+    __builtin_amdgcn_sched_barrier(0);
+    transpose<LOGN, Layout::kPass1, Layout::kPass2>(x, lds, L);
+    {
+        uint64_t r[kE], rr[kE];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            rr[2 * c] = rf[c].x;
+            rr[2 * c + 1] = rf[c].y;
+        }
+        static_for<2>([&](auto G) __attribute__((always_inline)) {
+            uint64_t a8[8], b8[8], o8[8];
+            static_for<8>([&](auto M_) __attribute__((always_inline)) {
+                a8[M_] = x[8 * G + M_];
+                b8[M_] = rr[8 * G + M_];
+            });
+            glv::mul_canon_g<8>(a8, b8, o8, qm1);
+            static_for<8>([&](auto M_) __attribute__((always_inline)) { r[8 * G + M_] = o8[M_]; });
+        });
+#pragma unroll
+        for (int k = 0; k < kE; ++k) x[k] = r[k];
+    }
+    // level t = 0: twiddle 1 on canonical values
+    {
+        using E = DftP2<0>;
+        uint64_t t[8];
+        static_for<8>([&](auto B) __attribute__((always_inline)) { t[B] = x[bf_u(E::H, B) + E::H]; });
+        ct_outputs<E::H, E>(x, t);
+    }
+    uint64_t w[8];
+    ld_level<Level<LOGN, 8>::NB>(w, opaque_after(t64 + N, x[0]), Level<LOGN, 8>::first(L));
+    static_for<3>([&](auto I) __attribute__((always_inline)) {
+        using E = DftP2<1 + I>;
+        uint64_t v[8], t[8];
+        static_for<8>([&](auto B) __attribute__((always_inline)) { v[B] = x[bf_u(E::H, B) + E::H]; });
+        pow2_products<E>(v, t, qm1);
+        ct_outputs<E::H, E>(x, t);
+    });
+    transpose<LOGN, Layout::kPass2, Layout::kPass3>(x, lds, L);
+    {
+        using Lv = Level<LOGN, 8>;
+        uint64_t wn[8];
+        ld_level<Level<LOGN, 9>::NB>(wn, opaque_after(t64 + N, x[0]), Level<LOGN, 9>::first(L));
+        ct_level_tw<Lv::HALF, Lv::SLOT_SHIFT>(x, w);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) w[b] = wn[b];
+    }
+    {
+        using Lv = Level<LOGN, 9>;
+        at_last();
+        ct_level_tw<Lv::HALF, Lv::SLOT_SHIFT>(x, w);
+    }
+}
+
+// The forward transform the kernels run: the twisted form at N = 1024 (tw then holds its tables
+// after psi_rev, N words in), ntt_forward otherwise.
+template <int LOGN, int FIRST_LV = 0, class AtLast = NoPrefetch>
+__device__ __forceinline__ void ntt_fwd(uint64_t (&x)[kE], uint64_t* lds, const uint64_t* tw, int L,
+                                        AtLast&& at_last = AtLast{}) {
+    if constexpr (LOGN == 10) ntt_forward_t64<FIRST_LV>(x, lds, tw + (1 << LOGN), L, at_last);
+    else ntt_forward<LOGN, FIRST_LV>(x, lds, tw, L, at_last);
 }
 
 // ---------------------------------------------------------------- inverse
