@@ -420,7 +420,7 @@ br_kernel(BrArgs A) {
                     // inverse transform (canonical out) -> acc + result, canonical
                     const uint64_t qm1 = glv::qm1_const();
                     uint64_t t[kE];
-                    ntt_inverse<LOGN>(out[0], buf, A.itw, L);
+                    ntt_inv<LOGN>(out[0], buf, A.itw, L);
                     glv::add_cc_g<kE>(out[0], acc[0], t, qm1);
 #pragma unroll
                     for (int k = 0; k < kE; ++k) {
@@ -429,7 +429,7 @@ br_kernel(BrArgs A) {
                         out[0][k] = out[1][k];
                     }
                 } else {
-                    ntt_inverse<LOGN>(out[0], buf, A.itw, L);
+                    ntt_inv<LOGN>(out[0], buf, A.itw, L);
 #pragma unroll
                     for (int k = 0; k < kE; ++k) {
                         const uint64_t t = gl::canon(gl::add_lc(out[0][k], acc[0][k]));

@@ -265,16 +265,17 @@ int tp_context_create(tp_context** out, const tp_params* p, int device) {
         return fail(TP_E_DEVICE, "stream creation failed");
     }
     tp::HostNtt T(N);
-    // d_tw: psi_rev, then the twisted forward transform's tables (used at N = 1024)
-    const std::vector<uint64_t> t64 = T.twisted64();
-    if (hipMalloc(&c->d_tw, 3 * N * 8) != hipSuccess || hipMalloc(&c->d_itw, N * 8) != hipSuccess ||
+    // d_tw, d_itw: psi_rev, psi_inv_rev, then the twisted transforms' tables (used at N = 1024)
+    const std::vector<uint64_t> t64 = T.twisted64(), it64 = T.twisted64(true);
+    if (hipMalloc(&c->d_tw, 3 * N * 8) != hipSuccess || hipMalloc(&c->d_itw, 3 * N * 8) != hipSuccess ||
         hipMalloc(&c->d_io, (size_t)8 * N * 8) != hipSuccess) {
         tp_context_destroy(c);
         return fail(TP_E_NOMEM, "table allocation failed");
     }
     bool ok = hipMemcpy(c->d_tw, T.psi_rev.data(), N * 8, hipMemcpyHostToDevice) == hipSuccess &&
               hipMemcpy(c->d_tw + N, t64.data(), 2 * N * 8, hipMemcpyHostToDevice) == hipSuccess &&
-              hipMemcpy(c->d_itw, T.psi_inv_rev.data(), N * 8, hipMemcpyHostToDevice) == hipSuccess;
+              hipMemcpy(c->d_itw, T.psi_inv_rev.data(), N * 8, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(c->d_itw + N, it64.data(), 2 * N * 8, hipMemcpyHostToDevice) == hipSuccess;
     for (auto& ev : c->ev) ok = ok && hipEventCreate(&ev) == hipSuccess;
     if (!ok) {
         tp_context_destroy(c);

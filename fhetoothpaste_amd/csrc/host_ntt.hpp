@@ -41,7 +41,8 @@ struct HostNtt {
     // [N, 2N): at N + 256 + q, w^(2 brv4(q & 15)), and at N + 512 + q, w^(brv5(q & 31)),
     //   w = psi^32: the cyclic 64-point transform's last two levels at the places where the
     //   plain transform keeps its levels 8 and 9; the rest is zero.
-    std::vector<uint64_t> twisted64() const {
+    // The inverse transform's tables (inverse = true) hold the inverses at the same places.
+    std::vector<uint64_t> twisted64(bool inverse = false) const {
         std::vector<uint64_t> t(2 * (size_t)N, 0);
         if (N != 1024) return t;
         auto brv = [](uint32_t x, int bits) {
@@ -49,7 +50,7 @@ struct HostNtt {
             for (int b = 0; b < bits; ++b) { r = (r << 1) | (x & 1); x >>= 1; }
             return r;
         };
-        const uint64_t psi = psi_rev[N / 2];
+        const uint64_t psi = inverse ? gl::inv(psi_rev[N / 2]) : psi_rev[N / 2];
         const uint64_t w = gl::pow(psi, 32);
         for (uint32_t c = 0; c < 8; ++c)
             for (uint32_t L = 0; L < 64; ++L)

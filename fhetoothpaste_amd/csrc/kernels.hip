@@ -33,7 +33,7 @@ __global__ void __launch_bounds__(256) ntt_batch_kernel(uint64_t* __restrict__ o
         } else {
 #pragma unroll
             for (int k = 0; k < kE; ++k) x[k] = src[idx3(L, k)];
-            ntt_inverse<LOGN>(x, lds[g], tw, L);
+            ntt_inv<LOGN>(x, lds[g], tw, L);
             if (live)
 #pragma unroll
                 for (int k = 0; k < kE; ++k) out[p * S::N + idx1<LOGN>(L, k)] = gl::canon(x[k]);

@@ -284,7 +284,7 @@ __global__ void __launch_bounds__(2 * LD * 64) br_latency_kernel(BrArgs A) {
                 for (int k = 0; k < kE; ++k) s[k] = red[0][w * N + idx3(L, k)];
                 if constexpr (METHOD == 0) {
                     uint64_t t[kE];
-                    ntt_inverse<LOGN>(s, red[2 + w], A.itw, L);  // canonical out
+                    ntt_inv<LOGN>(s, red[2 + w], A.itw, L);  // canonical out
                     glv::add_cc_g<kE>(s, accR, t, glv::qm1_const());
 #pragma unroll
                     for (int k = 0; k < kE; ++k) accR[k] = t[k];
@@ -292,7 +292,7 @@ __global__ void __launch_bounds__(2 * LD * 64) br_latency_kernel(BrArgs A) {
                     uint64_t prev[kE];
 #pragma unroll
                     for (int k = 0; k < kE; ++k) prev[k] = accL[w][phys(idx1<LOGN>(L, k))];
-                    ntt_inverse<LOGN>(s, accL[w], A.itw, L);
+                    ntt_inv<LOGN>(s, accL[w], A.itw, L);
                     poly_sync<S::WAVES>();
 #pragma unroll
                     for (int k = 0; k < kE; ++k)

@@ -290,7 +290,7 @@ __device__ __forceinline__ void auto_ntt_core(uint64_t (&ya)[kE], uint64_t (&yb)
     uint64_t a[kE];
 #pragma unroll
     for (int k = 0; k < kE; ++k) a[k] = src[idx3(L, k)];
-    ntt_inverse<LOGN>(a, lds, A.itw, L);
+    ntt_inv<LOGN>(a, lds, A.itw, L);
 #pragma unroll
     for (int k = 0; k < kE; ++k) a[k] = gl::canon(a[k]);
     automorphism_regs<LOGN>(a, lds, t, L);
@@ -427,8 +427,8 @@ __global__ void __launch_bounds__(Shape<LOGN>::P) trace_kernel(KeySwitchArgs A, 
                 b[k] = o[1][k];
             }
             automorphism_eval<LOGN>(a, b, lds, (1u << j) + 1, A, keys + (size_t)(j - 1) * key_words, L);
-            ntt_inverse<LOGN>(a, lds, A.itw, L);
-            ntt_inverse<LOGN>(b, lds, A.itw, L);
+            ntt_inv<LOGN>(a, lds, A.itw, L);
+            ntt_inv<LOGN>(b, lds, A.itw, L);
 #pragma unroll
             for (int k = 0; k < kE; ++k) {
                 o[0][k] = gl::canon(gl::add_lc(a[k], o[0][k]));
@@ -515,8 +515,8 @@ __global__ void __launch_bounds__(Shape<LOGN>::P) pack_merge_kernel(KeySwitchArg
             }
         }
         automorphism_eval<LOGN>(m[0], m[1], lds, (1u << lvl) + 1, A, ksk, L);
-        ntt_inverse<LOGN>(m[0], lds, A.itw, L);
-        ntt_inverse<LOGN>(m[1], lds, A.itw, L);
+        ntt_inv<LOGN>(m[0], lds, A.itw, L);
+        ntt_inv<LOGN>(m[1], lds, A.itw, L);
         poly_sync<S::WAVES>();
 #pragma unroll
         for (int k = 0; k < kE; ++k) {
@@ -550,8 +550,8 @@ __global__ void __launch_bounds__(Shape<LOGN>::P) pack_tail_kernel(KeySwitchArgs
             b[k] = o[1][k];
         }
         automorphism_eval<LOGN>(a, b, lds, (1u << lvl) + 1, A, keys + (size_t)(lvl - 1) * key_words, L);
-        ntt_inverse<LOGN>(a, lds, A.itw, L);
-        ntt_inverse<LOGN>(b, lds, A.itw, L);
+        ntt_inv<LOGN>(a, lds, A.itw, L);
+        ntt_inv<LOGN>(b, lds, A.itw, L);
 #pragma unroll
         for (int k = 0; k < kE; ++k) {
             o[0][k] = gl::canon(gl::add_lc(a[k], o[0][k]));

@@ -615,4 +615,89 @@ __device__ __forceinline__ void ntt_inverse(uint64_t (&x)[kE], uint64_t* lds, co
     inv_pass1<LOGN>(x);
 }
 
+// ---------------------------------------------------------------- inverse, N = 1024, twisted
+//
+// The inverse of ntt_forward_t64: the cyclic transform's last two levels with table twiddles, its
+// first four Gentleman-Sande levels with inverse power-of-two twiddles w^-(brv_t(m) (32 >> t)),
+// then the untwist rho_j^-i (table).  Six unscaled Gentleman-Sande levels are 2^6 times the
+// inverse of the six forward levels, as are the cyclic transform's, so no factor moves; N^-1 stays
+// in the last power-of-two level as in ntt_inverse.
+
+template <int T>
+struct InvDftP2 {
+    static constexpr int H = 8 >> T;
+    static constexpr int e(int b) { return (192 - kDft64Pow2[T][bf_u(H, b) >> (4 - T)]) % 192; }
+    static constexpr bool neg(int b) { return glv::pow2_kind(e(b)).neg; }
+};
+
+// it64: [0, N) rho_j^-i in the forward's pair order, [N, 2N) the inverse twiddles of the cyclic
+// transform's last two levels at ntt_inverse's level-9 and level-8 indices.
+template <bool CANON_IN = false>
+__device__ __forceinline__ void ntt_inverse_t64(uint64_t (&x)[kE], uint64_t* lds, const uint64_t* it64, int L) {
+    constexpr int LOGN = 10;
+    constexpr int N = 1 << LOGN;
+    const uint64_t qm1 = glv::qm1_const();
+    if constexpr (!CANON_IN) glv::canon_g<kE>(x, qm1);
+    it64 = opaque(it64);
+    using L9 = Level<LOGN, 9>;
+    using L8 = Level<LOGN, 8>;
+    uint64_t w[8];
+    ld_level<L9::NB>(w, opaque_after(it64 + N, x[0]), L9::first(L));
+    {
+        uint64_t wn[8];
+        ld_level<L8::NB>(wn, opaque_after(it64 + N, x[0]), L8::first(L));
+        gs_level_tw<L9::HALF, L9::SLOT_SHIFT>(x, w);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) w[b] = wn[b];
+    }
+    gs_level_tw<L8::HALF, L8::SLOT_SHIFT>(x, w);
+    transpose<LOGN, Layout::kPass3, Layout::kPass2>(x, lds, L);
+    // untwist factors, in flight during the four levels below
+    const uint64_t* tf = opaque_after(it64, x[0]);
+    U64x2 rf[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) rf[c] = ld_pair(tf, c * 64 + L);
+    __builtin_amdgcn_sched_barrier(0);
+    static_for<3>([&](auto I) __attribute__((always_inline)) {
+        using E = InvDftP2<3 - I>;
+        uint64_t d[8], t[8];
+        gs_sums_diffs<E::H, E>(x, d, qm1);
+        pow2_products<E>(d, t, qm1);
+        static_for<8>([&](auto B) __attribute__((always_inline)) { x[bf_u(E::H, B) + E::H] = t[B]; });
+    });
+    {
+        // level t = 0: twiddle 1, the differences go to the untwist as they are
+        uint64_t d[8];
+        gs_sums_diffs<8, NoNeg>(x, d, qm1);
+        static_for<8>([&](auto B) __attribute__((always_inline)) { x[B + 8] = d[B]; });
+    }
+    {
+        uint64_t rr[kE];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            rr[2 * c] = rf[c].x;
+            rr[2 * c + 1] = rf[c].y;
+        }
+        static_for<2>([&](auto G) __attribute__((always_inline)) {
+            uint64_t a8[8], b8[8], o8[8];
+            static_for<8>([&](auto M_) __attribute__((always_inline)) {
+                a8[M_] = x[8 * G + M_];
+                b8[M_] = rr[8 * G + M_];
+            });
+            glv::mul_canon_g<8>(a8, b8, o8, qm1);
+            static_for<8>([&](auto M_) __attribute__((always_inline)) { x[8 * G + M_] = o8[M_]; });
+        });
+    }
+    transpose<LOGN, Layout::kPass2, Layout::kPass1>(x, lds, L);
+    inv_pass1<LOGN>(x);
+}
+
+// The inverse transform the kernels run: the twisted form at N = 1024 (itw then holds its tables
+// after psi_inv_rev, N words in), ntt_inverse otherwise.
+template <int LOGN, bool CANON_IN = false>
+__device__ __forceinline__ void ntt_inv(uint64_t (&x)[kE], uint64_t* lds, const uint64_t* itw, int L) {
+    if constexpr (LOGN == 10) ntt_inverse_t64<CANON_IN>(x, lds, itw + (1 << LOGN), L);
+    else ntt_inverse<LOGN, CANON_IN>(x, lds, itw, L);
+}
+
 }  // namespace tp
