@@ -85,6 +85,7 @@ struct tp_context {
     int br_engine = 0;           // 0: Goldilocks NTT (default), 1: floating-point product where ready (opt-in)
     int fft_waves = 1;           // waves per SIMD of the floating-point kernel (1 or 2)
     int br_grid = 0;             // work-groups of a br_kernel launch: 0 one per ciphertext, else at most this many
+    int split_halves = 1;        // digit-parallel launches: 1 = two waves per transform where built
     int cu_count = 256;          // compute units of the context's device
     char arch[32] = {0};         // gcnArchName of the context's device
 };
@@ -168,10 +169,17 @@ bool fft_built(const tp_context* c) {
     return c->logN == 10 && c->p.method == TP_METHOD_CGGI_BINARY && c->p.br_digits >= 2 && c->p.br_digits <= 4;
 }
 bool fft_active(const tp_context* c) { return c->br_engine == 1 && c->fft_ready; }
+// digit-parallel kernel with every transform split over two waves: binary CGGI, N = 1024, 2..3 digits
+bool halves_built(const tp_context* c) {
+    return c->logN == 10 && c->p.method == TP_METHOD_CGGI_BINARY && (c->p.br_digits == 2 || c->p.br_digits == 3);
+}
 
 hipError_t launch_br(tp_context* c, const tp::BrArgs& A, size_t count, int grid) {
-    if (split_built(c) && count <= c->split_max)
+    if (split_built(c) && count <= c->split_max) {
+        if (c->split_halves && halves_built(c))
+            return tp::launch_blind_rotate_half((int)c->p.br_digits, A, grid, c->stream);
         return tp::launch_blind_rotate_latency((int)c->p.method, (int)c->p.br_digits, A, grid, c->stream);
+    }
     if (fft_active(c))
         return tp::launch_blind_rotate_fft((int)c->p.br_digits, A, c->d_fkey, c->d_zeta, grid, c->fft_waves,
                                            c->stream);
@@ -265,9 +273,11 @@ int tp_context_create(tp_context** out, const tp_params* p, int device) {
         return fail(TP_E_DEVICE, "stream creation failed");
     }
     tp::HostNtt T(N);
-    // d_tw, d_itw: psi_rev, psi_inv_rev, then the twisted transforms' tables (used at N = 1024)
+    // d_tw, d_itw: psi_rev, psi_inv_rev, then the twisted transforms' tables and the two-wave
+    // transforms' tables (both used at N = 1024)
     const std::vector<uint64_t> t64 = T.twisted64(), it64 = T.twisted64(true);
-    if (hipMalloc(&c->d_tw, 3 * N * 8) != hipSuccess || hipMalloc(&c->d_itw, 3 * N * 8) != hipSuccess ||
+    const std::vector<uint64_t> h64 = T.half_tables(), ih64 = T.half_tables(true);
+    if (hipMalloc(&c->d_tw, 5 * N * 8) != hipSuccess || hipMalloc(&c->d_itw, 5 * N * 8) != hipSuccess ||
         hipMalloc(&c->d_io, (size_t)8 * N * 8) != hipSuccess) {
         tp_context_destroy(c);
         return fail(TP_E_NOMEM, "table allocation failed");
@@ -275,7 +285,9 @@ int tp_context_create(tp_context** out, const tp_params* p, int device) {
     bool ok = hipMemcpy(c->d_tw, T.psi_rev.data(), N * 8, hipMemcpyHostToDevice) == hipSuccess &&
               hipMemcpy(c->d_tw + N, t64.data(), 2 * N * 8, hipMemcpyHostToDevice) == hipSuccess &&
               hipMemcpy(c->d_itw, T.psi_inv_rev.data(), N * 8, hipMemcpyHostToDevice) == hipSuccess &&
-              hipMemcpy(c->d_itw + N, it64.data(), 2 * N * 8, hipMemcpyHostToDevice) == hipSuccess;
+              hipMemcpy(c->d_itw + N, it64.data(), 2 * N * 8, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(c->d_tw + 3 * N, h64.data(), 2 * N * 8, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(c->d_itw + 3 * N, ih64.data(), 2 * N * 8, hipMemcpyHostToDevice) == hipSuccess;
     for (auto& ev : c->ev) ok = ok && hipEventCreate(&ev) == hipSuccess;
     if (!ok) {
         tp_context_destroy(c);
@@ -818,6 +830,11 @@ int tp_set_option(tp_context* c, int option, int value) {
         c->br_grid = value;
         return TP_OK;
     }
+    if (option == TP_OPT_BR_SPLIT_HALVES) {
+        if (value != 0 && value != 1) return fail(TP_E_INVALID, "split halves must be 0 or 1");
+        c->split_halves = value;
+        return TP_OK;
+    }
     return fail(TP_E_INVALID, "unknown option");
 }
 
@@ -841,6 +858,10 @@ int tp_get_option(tp_context* c, int option, int* value) {
     }
     if (option == TP_OPT_BR_GRID) {
         *value = c->br_grid;
+        return TP_OK;
+    }
+    if (option == TP_OPT_BR_SPLIT_HALVES) {
+        *value = c->split_halves && halves_built(c) ? 1 : 0;
         return TP_OK;
     }
     return fail(TP_E_INVALID, "unknown option");

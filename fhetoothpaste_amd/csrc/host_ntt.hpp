@@ -64,6 +64,36 @@ struct HostNtt {
         return t;
     }
 
+    // Tables of the two-wave N = 1024 transforms (2N words): [0, N) the twist rho_j^i at
+    // (64 q + l) 8 + k for position 512 q + ((l & 7) | k << 3 | (l >> 3) << 6) (block j, index i
+    // inside it); [N, 2N) at (64 q + l) 8 + s the cyclic levels' twiddles of lane l, with m = l & 7:
+    // s = 0: w^(4 brv3(m)) (t3), s = 1, 2: w^(2 brv4(2m + s - 1)) (t4), s = 3..6: w^(brv5(4m + s - 3))
+    // (t5), s = 7 unused.  inverse = true: the inverses at the same places.
+    std::vector<uint64_t> half_tables(bool inverse = false) const {
+        std::vector<uint64_t> t(2 * (size_t)N, 0);
+        if (N != 1024) return t;
+        auto brv = [](uint32_t x, int bits) {
+            uint32_t r = 0;
+            for (int b = 0; b < bits; ++b) { r = (r << 1) | (x & 1); x >>= 1; }
+            return r;
+        };
+        const uint64_t psi = inverse ? gl::inv(psi_rev[N / 2]) : psi_rev[N / 2];
+        const uint64_t w = gl::pow(psi, 32);
+        for (uint32_t q = 0; q < 2; ++q)
+            for (uint32_t l = 0; l < 64; ++l) {
+                const size_t base = (size_t)(64 * q + l) * 8;
+                for (uint32_t k = 0; k < 8; ++k) {
+                    const uint32_t j = 8 * q + (l >> 3), i = (l & 7) | (k << 3);
+                    t[base + k] = gl::pow(gl::pow(psi, 2 * brv(j, 4) + 1), i);
+                }
+                const uint32_t m = l & 7;
+                t[N + base + 0] = gl::pow(w, 4 * brv(m, 3));
+                for (uint32_t s2 = 1; s2 < 3; ++s2) t[N + base + s2] = gl::pow(w, 2 * brv(2 * m + s2 - 1, 4));
+                for (uint32_t s2 = 3; s2 < 7; ++s2) t[N + base + s2] = gl::pow(w, brv(4 * m + s2 - 3, 5));
+            }
+        return t;
+    }
+
     void forward(uint64_t* a) const {
         for (uint32_t m = 1, t = N / 2; m < N; m <<= 1, t >>= 1)
             for (uint32_t i = 0; i < m; ++i) {

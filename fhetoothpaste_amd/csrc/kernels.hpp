@@ -96,6 +96,9 @@ hipError_t launch_blind_rotate_fft(int digits, const BrArgs& A, const void* fkey
 // Digit-parallel variant at N = 1024 (digits 2..4): the 2 l digit products of a step run on 2 l
 // waves of one work-group per ciphertext; `grid` work-groups stride over A.count ciphertexts.
 hipError_t launch_blind_rotate_latency(int method, int digits, const BrArgs& A, int grid, hipStream_t s);
+// binary CGGI at N = 1024, 2 or 3 digits: the digit-parallel kernel with every transform split over
+// two waves (4 * digits waves per work-group; A.tw and A.itw hold the two-wave tables 3N words in)
+hipError_t launch_blind_rotate_half(int digits, const BrArgs& A, int grid, hipStream_t s);
 hipError_t launch_lwe_keyswitch(const KsArgs& A, hipStream_t s);
 // SAP digit decomposition (kernels_sap.hip): per-input modulus and digit count; one digit step
 // of the bookkeeping; products of rows of N words with one NTT-domain polynomial; a HomTrunc leaf

@@ -233,6 +233,14 @@ class Context:
         check(lib().tp_get_option(self.h, T.TP_OPT_BR_SPLIT_MAX, ctypes.byref(v)))
         return v.value
 
+    def set_br_split_halves(self, on):
+        """1: digit-parallel launches split every transform over two waves (binary CGGI, N = 1024,
+        2 or 3 digits), 0: one wave per digit transform."""
+        check(lib().tp_set_option(self.h, T.TP_OPT_BR_SPLIT_HALVES, int(on)))
+
+    def br_split_halves(self):
+        return self._get(T.TP_OPT_BR_SPLIT_HALVES)
+
     def import_rlwe_ksk(self, key):
         key = np.ascontiguousarray(key, dtype=np.uint64)
         check(lib().tp_import_rlwe_ksk(self.h, ptr(key), key.size))

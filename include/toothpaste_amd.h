@@ -294,6 +294,11 @@ int tp_graph_destroy(tp_graph* graph);
  * kernels; 0 (default) launches one per ciphertext, a positive value at most that many, each
  * striding over the launch's ciphertexts (results are the same either way). */
 #define TP_OPT_BR_GRID 5
+/* TP_OPT_BR_SPLIT_HALVES: for binary CGGI at N = 1024 with 2 or 3 digits, 1 (default) runs the
+ * digit-parallel launches with every transform split over two waves (4 l waves per ciphertext,
+ * kernels_br_half.hip), 0 with one wave per digit transform (2 l waves); tp_get_option reports 1
+ * only where the split kernel is built.  Results are the same either way. */
+#define TP_OPT_BR_SPLIT_HALVES 6
 int tp_set_option(tp_context* ctx, int option, int value);
 int tp_get_option(tp_context* ctx, int option, int* value);
 int tp_autotune(tp_context* ctx, size_t count, float* ms);

@@ -376,3 +376,36 @@ def test_digit_parallel_batch_matches_oracle(tp, oracle, method, n, b, l):
         assert np.array_equal(outs[count, False][0][c], oracle.sample_extract(ref, N)), c
         ref_ntt = oracle.blind_rotate(N, lwes[c], tvs[c], brk, n, b, l, method, False)
         assert np.array_equal(outs[count, True][1][c], ref_ntt), c
+
+
+@pytest.mark.parametrize("n,b,l", [(24, 12, 3), (18, 16, 2), (630, 12, 3)])
+def test_split_halves_matches_one_wave_per_digit(tp, oracle, n, b, l):
+    """Binary CGGI at N=1024: the digit-parallel launch with every transform split over two waves
+    (4 l waves per ciphertext) and the one with one wave per digit transform (2 l waves) give the
+    same extracted LWEs and accumulators in both domains, equal to the oracle (first and last
+    ciphertext at n=630, every ciphertext otherwise)."""
+    N, count = 1024, 5
+    say("split halves", n, b, l)
+    lk, rk, brk = _keys(oracle, N, n, b, l, oracle.BINARY, 0, 3.2, 120 + n + l)
+    ctx = tp.Context(N, n, b, l, 0)
+    ctx.import_brk(brk)
+    ctx.set_br_split_max(count)
+    assert ctx.br_split_halves() == 1
+    rng = np.random.default_rng(121 + n)
+    lwes = rng.integers(0, 2 * N, size=(count, n + 1), dtype=np.uint64)
+    lwes[1, ::2] = 0
+    lwes[3, :n] = N  # every rotation by X^N = -1
+    tvs = rng.integers(0, Q, size=(count, 2 * N), dtype=np.uint64)
+    outs = {}
+    for halves in (1, 0):
+        ctx.set_br_split_halves(halves)
+        for rlwe_ntt in (False, True):
+            outs[halves, rlwe_ntt] = ctx.eval_func_batch(lwes, tvs, want_rlwe=True, rlwe_ntt=rlwe_ntt)
+    assert ctx.br_split_halves() == 0
+    for rlwe_ntt in (False, True):
+        assert np.array_equal(outs[0, rlwe_ntt][0], outs[1, rlwe_ntt][0])
+        assert np.array_equal(outs[0, rlwe_ntt][1], outs[1, rlwe_ntt][1])
+    for c in ([0, count - 1] if n > 100 else range(count)):
+        ref = oracle.blind_rotate(N, lwes[c], tvs[c], brk, n, b, l, oracle.BINARY, True)
+        assert np.array_equal(outs[1, False][1][c], ref), c
+        assert np.array_equal(outs[1, False][0][c], oracle.sample_extract(ref, N)), c
