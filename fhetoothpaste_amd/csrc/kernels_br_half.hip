@@ -131,6 +131,7 @@ __global__ void __launch_bounds__(4 * LD * 64) br_half_kernel(BrArgs A) {
             __syncthreads();
 
             // digits: digit j of half h, index half q; the first level needs positions p and p + 512
+            __builtin_amdgcn_s_setprio(3);  // lowered as the wave progresses (half::forward)
             {
                 const int shift = A.first_shift - j * A.base_log2;
                 const uint32_t* src32 = (const uint32_t*)par[h];
@@ -181,6 +182,7 @@ __global__ void __launch_bounds__(4 * LD * 64) br_half_kernel(BrArgs A) {
                 if (q == 0) half::forward<0>(x, buf, A.tw + 3 * N, l, first_keys);
                 else half::forward<1>(x, buf, A.tw + 3 * N, l, first_keys);
                 half::wave_sync();  // the transpose buffer's last reads are done
+                __builtin_amdgcn_s_setprio(0);
                 static_for<4>([&](auto C) __attribute__((always_inline)) {
                     constexpr int c = C;
                     U64x2 na, nb;
@@ -200,6 +202,7 @@ __global__ void __launch_bounds__(4 * LD * 64) br_half_kernel(BrArgs A) {
                     }
                 });
             }
+            __builtin_amdgcn_s_setprio(3);
             __syncthreads();
 
             // reduce: sums[o N + g] = sum over the 2 l digit polynomials (waves 2 d + (g >> 9))

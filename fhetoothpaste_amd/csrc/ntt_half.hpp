@@ -213,6 +213,9 @@ __device__ __forceinline__ void scale8(uint64_t (&x)[kH], const uint64_t (&f)[kH
 //
 // x: H1 values after level 0 (lazy); out: H3 values, lazy, position 512 Q + pos<k3>(l, k).
 // tab = the forward tables; buf = the wave's transpose buffer; at_last runs before level t5.
+// The wave's priority drops from 3 (set by the caller) to 2 and then 1 as it passes the two
+// transposes, so among the waves of a SIMD the one that is furthest behind issues first and
+// they finish the phase together.
 template <int Q, class AtLast>
 __device__ __forceinline__ void forward(uint64_t (&x)[kH], uint64_t* buf, const uint64_t* tab, int l,
                                         AtLast&& at_last) {
@@ -225,6 +228,7 @@ __device__ __forceinline__ void forward(uint64_t (&x)[kH], uint64_t* buf, const 
     uint64_t f[kH];
     ld_lane8(f, opaque_after(tab, x[0]), ql);
     __builtin_amdgcn_sched_barrier(0);
+    __builtin_amdgcn_s_setprio(2);  // see forward's comment
     transpose<HL::k1, HL::k2>(x, buf, l);
     scale8(x, f, qm1);
     {
@@ -238,6 +242,7 @@ __device__ __forceinline__ void forward(uint64_t (&x)[kH], uint64_t* buf, const 
     uint64_t w[kH];
     ld_lane8(w, opaque_after(tab + 1024, x[0]), ql);
     __builtin_amdgcn_sched_barrier(0);
+    __builtin_amdgcn_s_setprio(1);
     transpose<HL::k2, HL::k3>(x, buf, l);
     {
         const uint64_t w3[4] = {w[0], w[0], w[0], w[0]};
