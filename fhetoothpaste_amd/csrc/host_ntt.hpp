@@ -3,6 +3,7 @@
 // (lib/src/backend/solinas.cpp:168-261): psi = 7^((Q-1)/2N), psi_rev[i] = psi^brv(i),
 // natural order in, bit-reversed order out.
 #pragma once
+#include <stddef.h>
 #include <stdint.h>
 #include <vector>
 
