@@ -378,12 +378,13 @@ def test_digit_parallel_batch_matches_oracle(tp, oracle, method, n, b, l):
         assert np.array_equal(outs[count, True][1][c], ref_ntt), c
 
 
-@pytest.mark.parametrize("n,b,l", [(24, 12, 3), (18, 16, 2), (630, 12, 3)])
+@pytest.mark.parametrize("n,b,l", [(24, 12, 3), (18, 16, 2), (20, 20, 3), (630, 12, 3)])
 def test_split_halves_matches_one_wave_per_digit(tp, oracle, n, b, l):
     """Binary CGGI at N=1024: the digit-parallel launch with every transform split over two waves
     (4 l waves per ciphertext) and the one with one wave per digit transform (2 l waves) give the
     same extracted LWEs and accumulators in both domains, equal to the oracle (first and last
-    ciphertext at n=630, every ciphertext otherwise)."""
+    ciphertext at n=630, every ciphertext otherwise).  b = 20 takes digits above 16 bits (the
+    generic first level) and a digit straddling the two 32-bit words."""
     N, count = 1024, 5
     say("split halves", n, b, l)
     lk, rk, brk = _keys(oracle, N, n, b, l, oracle.BINARY, 0, 3.2, 120 + n + l)
