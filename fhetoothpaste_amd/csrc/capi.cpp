@@ -902,6 +902,7 @@ int tp_autotune(tp_context* c, size_t count, float* ms) {
     const bool fft_candidate = c->br_engine == 1 && c->fft_ready;
     const int saved_engine = c->br_engine, saved_fw = c->fft_waves;
     const int saved_grid = c->br_grid;
+    const int saved_halves = c->split_halves;
     float t_var[6] = {-1.0f, -1.0f, -1.0f, -1.0f, -1.0f, -1.0f};
     // a warm-up launch, then the faster of two timed launches (one timing flips close variants)
     auto time_launch = [&](float& t) {
@@ -927,7 +928,23 @@ int tp_autotune(tp_context* c, size_t count, float* ms) {
         c->br_engine = v >= 3 ? 1 : 0;
         c->fft_waves = v == 4 ? 2 : 1;
         float t = -1.0f;
-        time_launch(t);
+        if (v == 2 && halves_built(c)) {
+            // the digit-parallel kernel in both forms: two waves per transform, then one; the
+            // faster is kept (two waves win at few ciphertexts, where SIMDs would idle)
+            float t_one_wave = -1.0f;
+            c->split_halves = 1;
+            time_launch(t);
+            if (!rc) {
+                c->split_halves = 0;
+                time_launch(t_one_wave);
+            }
+            if (!rc) {
+                c->split_halves = (t_one_wave >= 0 && t_one_wave < t) ? 0 : 1;
+                if (c->split_halves == 0) t = t_one_wave;
+            }
+        } else {
+            time_launch(t);
+        }
         t_var[v] = t;
     }
     if (rc) {
@@ -936,6 +953,7 @@ int tp_autotune(tp_context* c, size_t count, float* ms) {
         c->br_engine = saved_engine;
         c->fft_waves = saved_fw;
         c->br_grid = saved_grid;
+        c->split_halves = saved_halves;
     } else {
         c->br_waves = (t_var[1] >= 0 && t_var[1] < t_var[0]) ? 1 : 2;
         float t_one = c->br_waves == 1 ? t_var[1] : t_var[0];
@@ -963,6 +981,7 @@ int tp_autotune(tp_context* c, size_t count, float* ms) {
             c->br_engine = saved_engine;
             c->fft_waves = saved_fw;
             c->br_grid = saved_grid;
+            c->split_halves = saved_halves;
         } else {
             if (t_var[5] >= 0 && t_var[5] < t_one) {
                 c->br_grid = slots;
@@ -1004,13 +1023,14 @@ int tp_autotune_cached(tp_context* c, size_t count, const char* path, int* hit, 
         // stored settings, checked against what this build and context allow
         if ((e.waves != 1 && e.waves != 2) || (e.waves == 1 && c->p.br_digits != 3 && c->p.br_digits != 4) ||
             (e.split_max > 0 && !split_built(c)) || (e.engine == 1 && !c->fft_ready) || e.grid < 0 ||
-            (e.fft_waves != 1 && e.fft_waves != 2))
+            (e.fft_waves != 1 && e.fft_waves != 2) || (e.split_halves != 0 && e.split_halves != 1))
             return fail(TP_E_INVALID, "tuning cache entry does not fit this context");
         c->br_waves = e.waves;
         c->split_max = (size_t)e.split_max;
         c->br_engine = e.engine;
         c->fft_waves = e.fft_waves;
         c->br_grid = e.grid;
+        c->split_halves = e.split_halves;
         if (ms)
             for (int v = 0; v < 6; ++v) ms[v] = v < (int)e.ms.size() ? e.ms[v] : -1.0f;
         if (hit) *hit = 1;
@@ -1024,6 +1044,7 @@ int tp_autotune_cached(tp_context* c, size_t count, const char* path, int* hit, 
     key.engine = c->br_engine;
     key.fft_waves = c->fft_waves;
     key.grid = c->br_grid;
+    key.split_halves = c->split_halves;
     key.ms.assign(t, t + 6);
     if (ms)
         for (int v = 0; v < 6; ++v) ms[v] = t[v];

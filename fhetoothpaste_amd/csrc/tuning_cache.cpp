@@ -85,6 +85,7 @@ bool parse_entry(Cursor& c, TuneEntry& e) {
             else if (key == "engine") e.engine = (int)x;
             else if (key == "fft_waves") e.fft_waves = (int)x;
             else if (key == "grid") e.grid = (int)x;
+            else if (key == "split_halves") e.split_halves = (int)x;
             // other keys are ignored
         }
     } while (c.eat(','));
@@ -147,9 +148,9 @@ int write_tuning_cache(const char* path, const std::vector<TuneEntry>& entries, 
         std::fprintf(f,
                      "{\"arch\": \"%s\", \"cus\": %d, \"N\": %u, \"n\": %u, \"b\": %u, \"l\": %u, \"method\": %u, "
                      "\"count\": %llu, \"fft_opt_in\": %d, \"waves\": %d, \"split_max\": %llu, \"engine\": %d, "
-                     "\"fft_waves\": %d, \"grid\": %d, \"ms\": [",
+                     "\"fft_waves\": %d, \"grid\": %d, \"split_halves\": %d, \"ms\": [",
                      e.arch.c_str(), e.cus, e.N, e.n, e.b, e.l, e.method, (unsigned long long)e.count, e.fft_opt_in,
-                     e.waves, (unsigned long long)e.split_max, e.engine, e.fft_waves, e.grid);
+                     e.waves, (unsigned long long)e.split_max, e.engine, e.fft_waves, e.grid, e.split_halves);
         for (size_t v = 0; v < e.ms.size(); ++v) std::fprintf(f, "%s%.4f", v ? ", " : "", (double)e.ms[v]);
         std::fprintf(f, "]}%s\n", k + 1 < entries.size() ? "," : "");
     }

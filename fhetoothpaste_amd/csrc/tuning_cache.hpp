@@ -17,6 +17,7 @@ struct TuneEntry {
     int waves = 2;
     uint64_t split_max = 0;
     int engine = 0, fft_waves = 1, grid = 0;
+    int split_halves = 1;  // digit-parallel form: 1 = two waves per transform, 0 = one
     std::vector<float> ms;
     bool same_key(const TuneEntry& o) const;
 };

@@ -181,11 +181,12 @@ class Context:
 
     def autotune(self, count, cache=None):
         """Time the blind-rotation variants on `count` synthetic ciphertexts (tp_autotune): keeps the
-        fastest one-wave-per-ciphertext variant (engine and waves) and grid, and moves the
+        fastest one-wave-per-ciphertext variant (engine and waves) and grid, the faster form of the
+        digit-parallel kernel (one or two waves per transform, where both are built), and moves the
         digit-parallel threshold to the side of `count` that won; with `cache` (a JSON file path,
         tp_autotune_cached) a stored result for this device, parameter set and count is applied
         instead of timing.  Returns {"waves", "split_max", "engine", "fft_waves", "grid",
-        "cache_hit", "ms_two_waves", "ms_one_wave", "ms_digit_parallel", "ms_fft_one_wave",
+        "split_halves", "cache_hit", "ms_two_waves", "ms_one_wave", "ms_digit_parallel", "ms_fft_one_wave",
         "ms_fft_two_waves", "ms_persistent_grid"} (-1 for a variant that is not built, not usable
         or not timed)."""
         ms = (ctypes.c_float * 6)()
@@ -196,7 +197,7 @@ class Context:
             check(lib().tp_autotune_cached(self.h, count, os.fsencode(cache), ctypes.byref(hit), ms))
         return {"waves": self.br_waves(), "split_max": self.br_split_max(), "engine": self.br_engine(),
                 "fft_waves": self._get(T.TP_OPT_BR_FFT_WAVES), "grid": self._get(T.TP_OPT_BR_GRID),
-                "cache_hit": bool(hit.value), "ms_two_waves": ms[0], "ms_one_wave": ms[1],
+                "split_halves": self.br_split_halves(), "cache_hit": bool(hit.value), "ms_two_waves": ms[0], "ms_one_wave": ms[1],
                 "ms_digit_parallel": ms[2], "ms_fft_one_wave": ms[3], "ms_fft_two_waves": ms[4],
                 "ms_persistent_grid": ms[5]}
 

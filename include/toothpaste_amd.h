@@ -275,8 +275,10 @@ int tp_graph_destroy(tp_graph* graph);
  * the imported key (the engine-1 variants only when the context opted in to engine 1), keeps
  * the fastest one-wave-per-ciphertext variant (engine and waves), times that variant on a
  * persistent grid (TP_OPT_BR_GRID = the ciphertexts resident at once) when `count` exceeds it and
- * keeps the faster grid, moves the digit-parallel threshold to the side of `count` that won, and
- * reports ms[0] (NTT engine, two waves per SIMD), ms[1] (one wave), ms[2] (digit-parallel), ms[3]
+ * keeps the faster grid, times the digit-parallel kernel in both forms where the two-wave form
+ * is built (TP_OPT_BR_SPLIT_HALVES 1 and 0) and keeps the faster, moves the digit-parallel
+ * threshold to the side of `count` that won, and reports ms[0] (NTT engine, two waves per SIMD),
+ * ms[1] (one wave), ms[2] (digit-parallel, the faster form), ms[3]
  * (floating-point engine, one wave), ms[4] (two waves) and ms[5] (the winner on the persistent
  * grid), -1 where a variant is not built, not usable or not timed; ms must hold 6 floats.  The
  * context's later launches, including the server's chunks, follow these settings.

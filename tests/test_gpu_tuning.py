@@ -1,6 +1,6 @@
 """GPU: the tuning cache (tp_autotune_cached).  A miss times the variants and writes the JSON file;
 a second context of the same parameter set and count hits it (no timing) and takes the same
-settings; another count misses; a malformed file is rejected; launches on the tuned settings still
+settings, the digit-parallel form (one or two waves per transform) included; another count misses; a malformed file is rejected; launches on the tuned settings still
 equal the oracle."""
 import json
 import sys
@@ -31,15 +31,18 @@ def test_tuning_cache_roundtrip(oracle, tmp_path):
     for attempt in range(2):
         ctx = tp.Context(N, n, b, l, 0)
         ctx.import_brk(brk)
+        if attempt == 1:
+            ctx.set_br_split_halves(1 - settings[0][4])   # a hit restores the stored form
         r = ctx.autotune(1500, cache=path)
         assert r["cache_hit"] == (attempt == 1)
-        settings.append((r["waves"], r["split_max"], r["engine"], r["grid"]))
-        assert r["ms_two_waves"] > 0
+        settings.append((r["waves"], r["split_max"], r["engine"], r["grid"], r["split_halves"]))
+        assert r["ms_two_waves"] > 0 and r["ms_digit_parallel"] > 0 and r["split_halves"] in (0, 1)
         if attempt == 0:
             doc = json.load(open(path))
             (e,) = doc["entries"]
             assert (e["N"], e["n"], e["b"], e["l"], e["method"], e["count"]) == (N, n, b, l, 0, 1500)
             assert e["arch"].startswith("gfx950") and len(e["ms"]) == 6
+            assert e["split_halves"] == r["split_halves"]
         lwes = oracle.random_mod_pow2(3 * (n + 1), 2 * N, 42, attempt).reshape(3, n + 1)
         tv = np.random.default_rng(attempt).integers(0, Q, size=2 * N, dtype=np.uint64)
         _, rl = ctx.eval_func_batch(lwes, tv, want_rlwe=True)
