@@ -42,7 +42,8 @@ struct HostNtt {
     // [N, 2N): at N + 256 + q, w^(2 brv4(q & 15)), and at N + 512 + q, w^(brv5(q & 31)),
     //   w = psi^32: the cyclic 64-point transform's last two levels at the places where the
     //   plain transform keeps its levels 8 and 9; the rest is zero.
-    // The inverse transform's tables (inverse = true) hold the inverses at the same places.
+    // The inverse transform's tables (inverse = true) hold the inverses at the same places, the
+    // untwist rho_j^-i times N^-1 (the inverse transform's only scaling).
     std::vector<uint64_t> twisted64(bool inverse = false) const {
         std::vector<uint64_t> t(2 * (size_t)N, 0);
         if (N != 1024) return t;
@@ -53,12 +54,13 @@ struct HostNtt {
         };
         const uint64_t psi = inverse ? gl::inv(psi_rev[N / 2]) : psi_rev[N / 2];
         const uint64_t w = gl::pow(psi, 32);
+        const uint64_t scale = inverse ? gl::inv(N) : 1;
         for (uint32_t c = 0; c < 8; ++c)
             for (uint32_t L = 0; L < 64; ++L)
                 for (uint32_t q = 0; q < 2; ++q) {
                     const uint32_t k = 2 * c + q, j = L >> 2, i = 4 * k + (L & 3);
                     const uint64_t rho = gl::pow(psi, 2 * brv(j, 4) + 1);
-                    t[2 * (64 * c + L) + q] = gl::pow(rho, i);
+                    t[2 * (64 * c + L) + q] = gl::mul(gl::pow(rho, i), scale);
                 }
         for (uint32_t q = 0; q < 256; ++q) t[N + 256 + q] = gl::pow(w, 2 * brv(q & 15, 4));
         for (uint32_t q = 0; q < 512; ++q) t[N + 512 + q] = gl::pow(w, brv(q & 31, 5));
@@ -69,7 +71,7 @@ struct HostNtt {
     // (64 q + l) 8 + k for position 512 q + ((l & 7) | k << 3 | (l >> 3) << 6) (block j, index i
     // inside it); [N, 2N) at (64 q + l) 8 + s the cyclic levels' twiddles of lane l, with m = l & 7:
     // s = 0: w^(4 brv3(m)) (t3), s = 1, 2: w^(2 brv4(2m + s - 1)) (t4), s = 3..6: w^(brv5(4m + s - 3))
-    // (t5), s = 7 unused.  inverse = true: the inverses at the same places.
+    // (t5), s = 7 unused.  inverse = true: the inverses at the same places, the untwist times N^-1.
     std::vector<uint64_t> half_tables(bool inverse = false) const {
         std::vector<uint64_t> t(2 * (size_t)N, 0);
         if (N != 1024) return t;
@@ -80,12 +82,13 @@ struct HostNtt {
         };
         const uint64_t psi = inverse ? gl::inv(psi_rev[N / 2]) : psi_rev[N / 2];
         const uint64_t w = gl::pow(psi, 32);
+        const uint64_t scale = inverse ? gl::inv(N) : 1;
         for (uint32_t q = 0; q < 2; ++q)
             for (uint32_t l = 0; l < 64; ++l) {
                 const size_t base = (size_t)(64 * q + l) * 8;
                 for (uint32_t k = 0; k < 8; ++k) {
                     const uint32_t j = 8 * q + (l >> 3), i = (l & 7) | (k << 3);
-                    t[base + k] = gl::pow(gl::pow(psi, 2 * brv(j, 4) + 1), i);
+                    t[base + k] = gl::mul(gl::pow(gl::pow(psi, 2 * brv(j, 4) + 1), i), scale);
                 }
                 const uint32_t m = l & 7;
                 t[N + base + 0] = gl::pow(w, 4 * brv(m, 3));

@@ -260,7 +260,7 @@ __device__ __forceinline__ void forward(uint64_t (&x)[kH], uint64_t* buf, const 
 // Part 1 on wave half Q: x = H3 values (position 512 Q + pos<k3>(l, k), any 64-bit), out: the
 // values before the last level (bit 9), canonical, H1.  Part 2 (after the caller has exchanged
 // halves): lo = position p, hi = position p + 512 (both canonical) -> wave Q's output coefficient
-// at 512 Q + p, canonical, with N^-1 applied.
+// at 512 Q + p, canonical, with N^-1 applied (the untwist table carries it).
 template <int Q>
 __device__ __forceinline__ void inverse_part1(uint64_t (&x)[kH], uint64_t* buf, const uint64_t* tab, int l) {
     const uint64_t qm1 = glv::qm1_const();
@@ -297,10 +297,10 @@ __device__ __forceinline__ void inverse_part1(uint64_t (&x)[kH], uint64_t* buf, 
     gs_pow2_4<InvH<1, Q>>(x, qm1);
 }
 
-// The last level with N^-1 = 2^-10 merged (as ntt_inverse's): half 0 takes (lo + hi) 2^-10,
-// half 1 (lo - hi) 2^(e1 - 10), e1 the level's exponent.
+// The last level: half 0 takes lo + hi, half 1 (lo - hi) 2^e1, e1 the level's exponent (N^-1 is
+// already in, from the untwist).
 struct InvLast4 {
-    static constexpr int e(int) { return (gl::kInvPow2[1] + 192 - 10) % 192; }
+    static constexpr int e(int) { return gl::kInvPow2[1]; }
     static constexpr bool neg(int) { return glv::pow2_kind(e(0)).neg; }
 };
 template <int Q>
@@ -313,10 +313,7 @@ __device__ __forceinline__ void inverse_part2(uint64_t (&x)[kH], const uint64_t 
             b[M_] = hi[4 * G + M_];
         });
         if constexpr (Q == 0) {
-            uint64_t s_[4];
-            int sh[4] = {10, 10, 10, 10};
-            glv::add_cc_g<4>(a, b, s_, qm1);
-            glv::mul_2e_neg_g<4>(s_, sh, r);
+            glv::add_cc_g<4>(a, b, r, qm1);
         } else {
             uint64_t d[4];
             if constexpr (InvLast4::neg(0)) glv::sub_lc_g<4>(b, a, d);

@@ -532,17 +532,17 @@ struct InvP1 {
     static constexpr bool neg(int b) { return glv::pow2_kind(e(b)).neg; }
 };
 
-// The last inverse level with N^-1 = 2^-LOGN merged in: x[k] = (a + b) 2^-LOGN and
-// x[k + 8] = (a - b) 2^(e1 - LOGN), e1 the level's exponent.
-template <int LOGN>
+// The last inverse level with N^-1 = 2^-S merged in (S = LOGN, or 0 where a table carries N^-1):
+// x[k] = (a + b) 2^-S and x[k + 8] = (a - b) 2^(e1 - S), e1 the level's exponent.
+template <int S>
 struct InvLast {
-    static constexpr int e(int) { return (gl::kInvPow2[1] + 192 - LOGN) % 192; }
+    static constexpr int e(int) { return (gl::kInvPow2[1] + 192 - S) % 192; }
     static constexpr bool neg(int b) { return glv::pow2_kind(e(b)).neg; }
 };
 
-// Levels m = 8..1 with power-of-two inverse twiddles on canonical values; N^-1 = 2^(192 - LOGN)
-// is merged into the last level (both of its factors are powers of two).
-template <int LOGN>
+// Levels m = 8..1 with power-of-two inverse twiddles on canonical values; with SCALE,
+// N^-1 = 2^(192 - LOGN) is merged into the last level (both of its factors are powers of two).
+template <int LOGN, bool SCALE = true>
 __device__ __forceinline__ void inv_pass1(uint64_t (&x)[kE]) {
     const uint64_t qm1 = glv::qm1_const();
     static_for<3>([&](auto I) __attribute__((always_inline)) {
@@ -553,19 +553,21 @@ __device__ __forceinline__ void inv_pass1(uint64_t (&x)[kE]) {
         pow2_products<E>(d, t, qm1);
         static_for<8>([&](auto B) __attribute__((always_inline)) { x[bf_u(E::H, B) + E::H] = t[B]; });
     });
-    using E = InvLast<LOGN>;
+    using E = InvLast<SCALE ? LOGN : 0>;
     uint64_t d[8], t[8];
     gs_sums_diffs<8, E>(x, d, qm1);
-    static_for<2>([&](auto G) __attribute__((always_inline)) {
-        uint64_t s_[4], r[4];
-        int sh[4];
-        static_for<4>([&](auto M_) __attribute__((always_inline)) {
-            s_[M_] = x[4 * G + M_];
-            sh[M_] = LOGN;
+    if constexpr (SCALE) {
+        static_for<2>([&](auto G) __attribute__((always_inline)) {
+            uint64_t s_[4], r[4];
+            int sh[4];
+            static_for<4>([&](auto M_) __attribute__((always_inline)) {
+                s_[M_] = x[4 * G + M_];
+                sh[M_] = LOGN;
+            });
+            glv::mul_2e_neg_g<4>(s_, sh, r);
+            static_for<4>([&](auto M_) __attribute__((always_inline)) { x[4 * G + M_] = r[M_]; });
         });
-        glv::mul_2e_neg_g<4>(s_, sh, r);
-        static_for<4>([&](auto M_) __attribute__((always_inline)) { x[4 * G + M_] = r[M_]; });
-    });
+    }
     pow2_products<E>(d, t, qm1);
     static_for<8>([&](auto B) __attribute__((always_inline)) { x[B + 8] = t[B]; });
 }
@@ -619,9 +621,9 @@ __device__ __forceinline__ void ntt_inverse(uint64_t (&x)[kE], uint64_t* lds, co
 //
 // The inverse of ntt_forward_t64: the cyclic transform's last two levels with table twiddles, its
 // first four Gentleman-Sande levels with inverse power-of-two twiddles w^-(brv_t(m) (32 >> t)),
-// then the untwist rho_j^-i (table).  Six unscaled Gentleman-Sande levels are 2^6 times the
-// inverse of the six forward levels, as are the cyclic transform's, so no factor moves; N^-1 stays
-// in the last power-of-two level as in ntt_inverse.
+// then the untwist rho_j^-i N^-1 (table: N^-1 rides on the untwist, so the last power-of-two level
+// scales nothing).  Six unscaled Gentleman-Sande levels are 2^6 times the inverse of the six
+// forward levels, as are the cyclic transform's, so no factor moves.
 
 template <int T>
 struct InvDftP2 {
@@ -630,7 +632,7 @@ struct InvDftP2 {
     static constexpr bool neg(int b) { return glv::pow2_kind(e(b)).neg; }
 };
 
-// it64: [0, N) rho_j^-i in the forward's pair order, [N, 2N) the inverse twiddles of the cyclic
+// it64: [0, N) rho_j^-i N^-1 in the forward's pair order, [N, 2N) the inverse twiddles of the cyclic
 // transform's last two levels at ntt_inverse's level-9 and level-8 indices.
 template <bool CANON_IN = false>
 __device__ __forceinline__ void ntt_inverse_t64(uint64_t (&x)[kE], uint64_t* lds, const uint64_t* it64, int L) {
@@ -689,7 +691,7 @@ __device__ __forceinline__ void ntt_inverse_t64(uint64_t (&x)[kE], uint64_t* lds
         });
     }
     transpose<LOGN, Layout::kPass2, Layout::kPass1>(x, lds, L);
-    inv_pass1<LOGN>(x);
+    inv_pass1<LOGN, false>(x);
 }
 
 // The inverse transform the kernels run: the twisted form at N = 1024 (itw then holds its tables
