@@ -23,6 +23,7 @@ TP_OPT_BR_ENGINE = 3
 TP_OPT_BR_FFT_WAVES = 4
 TP_OPT_BR_GRID = 5
 TP_OPT_BR_SPLIT_HALVES = 6
+TP_OPT_KS_MFMA = 7
 TP_ELT_ADD, TP_ELT_SUB, TP_ELT_MUL, TP_ELT_FMA, TP_ELT_MULS, TP_ELT_REDUCE = 0, 1, 2, 3, 4, 5
 
 u64p = ctypes.POINTER(ctypes.c_uint64)

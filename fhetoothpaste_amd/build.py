@@ -25,6 +25,7 @@ SOURCES = [
     ("kernels_br_fft.hip", DEVICE_FLAGS),
     ("kernels_br_lat.hip", DEVICE_FLAGS),
     ("kernels_br_half.hip", DEVICE_FLAGS),
+    ("kernels_ks_mfma.hip", DEVICE_FLAGS),
     ("kernels_pack.hip", DEVICE_FLAGS),
     ("kernels_eltwise.hip", DEVICE_FLAGS),
     ("kernels_sap.hip", DEVICE_FLAGS),

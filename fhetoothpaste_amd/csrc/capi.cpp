@@ -14,6 +14,7 @@
 #include "fft_key.hpp"
 #include "host_ntt.hpp"
 #include "kernels.hpp"
+#include "ks_limbs.hpp"
 #include "tuning_cache.hpp"
 
 namespace {
@@ -55,6 +56,8 @@ struct tp_context {
     uint64_t* d_corr = nullptr;
     size_t ksk_words = 0;
     uint64_t ks_corrector = 0;
+    int8_t* d_kl = nullptr;      // the key-switching key as matrix-core operands (ks_limbs.hpp), where it fits
+    int ks_mfma = 1;             // key switch past the row-split sizes on the matrix cores where d_kl is built
     uint64_t* d_scratch = nullptr;  // extracted LWEs of tp_bootstrap_batch_device
     size_t scratch_words = 0;
     uint64_t* d_io = nullptr;  // staging of the host-pointer entry points
@@ -341,6 +344,7 @@ int tp_context_destroy(tp_context* c) {
         if (p) (void)hipFree(p);
     for (double* p : {c->d_fkey, c->d_zeta})
         if (p) (void)hipFree(p);
+    if (c->d_kl) (void)hipFree(c->d_kl);
     for (auto& ev : c->ev)
         if (ev) (void)hipEventDestroy(ev);
     if (c->d_perm) (void)hipFree(c->d_perm);
@@ -441,6 +445,17 @@ int tp_import_lwe_ksk(tp_context* c, const uint64_t* ksk, size_t words) {
     TP_HIP(hipMemset(c->d_ksk + words, 0, tp::kKsPadWords * 8));
     TP_HIP(hipMemcpy(c->d_corr, corr.data(), W * 8, hipMemcpyHostToDevice));
     c->ksk_words = words;
+    if (c->d_kl) { TP_HIP(hipFree(c->d_kl)); c->d_kl = nullptr; }
+    if (tp::ks_mfma_fits(n_in, b, d)) {
+        const size_t bytes = tp::ks_limb_bytes(n_in, n_out);
+        std::vector<int8_t> kl(bytes);
+        tp::build_ks_limbs(ksk, n_in, n_out, b, d, kl.data());
+        if (hipMalloc(&c->d_kl, bytes) != hipSuccess) {
+            c->d_kl = nullptr;
+            return fail(TP_E_NOMEM, "ksk operand allocation failed");
+        }
+        TP_HIP(hipMemcpy(c->d_kl, kl.data(), bytes, hipMemcpyHostToDevice));
+    }
     return TP_OK;
 }
 
@@ -680,6 +695,7 @@ int launch_ks_rows(tp_context* c, size_t count, const uint64_t* in, size_t in_st
     A.base_log2 = (int)c->p.ks_base_log2;
     A.digits = (int)c->p.ks_digits;
     A.count = (int)count;
+    A.kl = c->ks_mfma ? c->d_kl : nullptr;
     hipError_t e = tp::launch_lwe_keyswitch(A, c->stream);
     if (e != hipSuccess) return fail(TP_E_DEVICE, std::string("key switch launch: ") + hipGetErrorString(e));
     return TP_OK;
@@ -835,6 +851,11 @@ int tp_set_option(tp_context* c, int option, int value) {
         c->split_halves = value;
         return TP_OK;
     }
+    if (option == TP_OPT_KS_MFMA) {
+        if (value != 0 && value != 1) return fail(TP_E_INVALID, "matrix-core key switch must be 0 or 1");
+        c->ks_mfma = value;
+        return TP_OK;
+    }
     return fail(TP_E_INVALID, "unknown option");
 }
 
@@ -862,6 +883,10 @@ int tp_get_option(tp_context* c, int option, int* value) {
     }
     if (option == TP_OPT_BR_SPLIT_HALVES) {
         *value = c->split_halves && halves_built(c) ? 1 : 0;
+        return TP_OK;
+    }
+    if (option == TP_OPT_KS_MFMA) {
+        *value = c->ks_mfma && c->d_kl ? 1 : 0;
         return TP_OK;
     }
     return fail(TP_E_INVALID, "unknown option");

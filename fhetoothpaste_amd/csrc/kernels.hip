@@ -262,6 +262,8 @@ hipError_t launch_blind_rotate(int logN, int method, int digits, const BrArgs& A
 }
 
 hipError_t launch_lwe_keyswitch(const KsArgs& A, hipStream_t s) {
+    // past the row-split sizes the matrix-core kernel, where the key is held in its operand form
+    if (A.kl && A.count > kKsRowSplitMax) return launch_lwe_keyswitch_mfma(A, s);
     if (A.count <= kKsRowSplitMax) {
         hipLaunchKernelGGL(lwe_keyswitch_one_kernel, dim3((A.n_out + 1 + kKsSplitCols - 1) / kKsSplitCols, A.count),
                            dim3(kKsBlock), 0, s, A);

@@ -43,6 +43,7 @@ struct KsArgs {
     uint64_t corrector;    // signed-to-unsigned corrector C mod Q
     int n_in, n_out, base_log2, digits, count;
     const uint32_t* rows;  // [count] input row of ciphertext c (null: row c); output row is c
+    const int8_t* kl;      // the key as matrix-core operands (ks_limbs.hpp), or null
 };
 
 // RLWE key switching with signed digits (automorphism, trace and packing keys).
@@ -100,6 +101,8 @@ hipError_t launch_blind_rotate_latency(int method, int digits, const BrArgs& A, 
 // two waves (4 * digits waves per work-group; A.tw and A.itw hold the two-wave tables 3N words in)
 hipError_t launch_blind_rotate_half(int digits, const BrArgs& A, int grid, hipStream_t s);
 hipError_t launch_lwe_keyswitch(const KsArgs& A, hipStream_t s);
+// the matrix-core form (kernels_ks_mfma.hip); A.kl set, ks_mfma_fits(n_in, base_log2, digits)
+hipError_t launch_lwe_keyswitch_mfma(const KsArgs& A, hipStream_t s);
 // SAP digit decomposition (kernels_sap.hip): per-input modulus and digit count; one digit step
 // of the bookkeeping; products of rows of N words with one NTT-domain polynomial; a HomTrunc leaf
 // whose tree is a single leaf; the sum of the tree's leaf contributions; sample extraction into

@@ -242,6 +242,14 @@ class Context:
     def br_split_halves(self):
         return self._get(T.TP_OPT_BR_SPLIT_HALVES)
 
+    def set_ks_mfma(self, on):
+        """1: LWE key switches of more than 64 ciphertexts run on the matrix cores where the key fits
+        that form, 0: on the vector units (same results)."""
+        check(lib().tp_set_option(self.h, T.TP_OPT_KS_MFMA, int(on)))
+
+    def ks_mfma(self):
+        return self._get(T.TP_OPT_KS_MFMA)
+
     def import_rlwe_ksk(self, key):
         key = np.ascontiguousarray(key, dtype=np.uint64)
         check(lib().tp_import_rlwe_ksk(self.h, ptr(key), key.size))

@@ -118,7 +118,7 @@ def test_eval_func_batch_vs_oracle(tp, oracle, N, n, b, l, count, split):
 
 @pytest.mark.parametrize("count", [37, 70])
 def test_keyswitch_batch_vs_oracle(tp, oracle, count):
-    """Up to 64 ciphertexts run the row-split kernel, more the one-lane-per-ciphertext kernel."""
+    """Up to 64 ciphertexts run the row-split kernel, more the matrix-core kernel."""
     N, n_out, b, d = 1024, 630, 4, 14
     say("lwe keyswitch", N, n_out, b, d, count)
     src = oracle.secret(N, 0, 3, 2)
@@ -157,9 +157,9 @@ def test_keyswitch_chunk_shapes(tp, oracle, N, n_out, b, d, count):
 
 
 def test_keyswitch_tile_widths_agree(tp, oracle):
-    """A launch large enough for the 8-column tiles (4000 ciphertexts, 1024 -> 630, 16 blocks of
-    256 lanes) equals two launches of 2000 on the 4-column tiles (8 blocks, the last one partial),
-    word for word, and sampled rows equal the oracle."""
+    """Vector-unit kernel (matrix cores off): a launch large enough for the 8-column tiles (4000
+    ciphertexts, 1024 -> 630, 16 blocks of 256 lanes) equals two launches of 2000 on the 4-column
+    tiles (8 blocks, the last one partial), word for word, and sampled rows equal the oracle."""
     N, n_out, b, d = 1024, 630, 4, 14
     say("lwe keyswitch tile widths", N, n_out, b, d)
     src = tp.keygen_secret(N, 0, 12, 2)
@@ -167,6 +167,8 @@ def test_keyswitch_tile_widths_agree(tp, oracle):
     ksk = tp.keygen_lwe_ksk(src, dst, N, n_out, b, d, 3.2, 12)
     ctx = tp.Context(N, 4, 12, 3, 0, n_out, b, d)
     ctx.import_lwe_ksk(ksk)
+    ctx.set_ks_mfma(0)
+    assert ctx.ks_mfma() == 0
     count = 4000
     lwes = np.random.default_rng(13).integers(0, Q, size=(count, N + 1), dtype=np.uint64)
     lwes[7] = Q - 1
@@ -175,6 +177,35 @@ def test_keyswitch_tile_widths_agree(tp, oracle):
     assert np.array_equal(whole, halves)
     for c in (0, 7, 1999, 2000, count - 1):
         assert np.array_equal(whole[c], oracle.lwe_keyswitch(lwes[c], ksk, N, n_out, b, d)), c
+
+
+@pytest.mark.parametrize("N,n_out,b,d", [(1024, 630, 4, 14), (2048, 800, 5, 12), (1024, 40, 6, 10), (1024, 33, 7, 9),
+                                         (1024, 70, 3, 16), (1024, 31, 4, 16), (1024, 64, 2, 16), (1024, 20, 1, 16),
+                                         (1024, 95, 4, 5)])
+@pytest.mark.parametrize("count", [65, 300])
+def test_keyswitch_matrix_cores_match(tp, oracle, N, n_out, b, d, count):
+    """More than 64 ciphertexts run the matrix-core key switch (int8 digit and key-byte operands,
+    32-bit sums recombined mod Q): word for word equal to the vector-unit kernel, sampled rows and
+    the edge inputs (all Q - 1, all 0) equal to the oracle.  Counts 65 and 300 end in partial
+    128-row tiles; n_out + 1 = 41, 34, 71, 32, 65, 21, 96 give partial and whole 32-column groups;
+    b = 4 takes the nibble path, the other bases the bit-field one."""
+    say("lwe keyswitch matrix cores", N, n_out, b, d, count)
+    src = tp.keygen_secret(N, 0, 14 + b, 2)
+    dst = tp.keygen_secret(n_out, 0, 14 + b, 1)
+    ksk = tp.keygen_lwe_ksk(src, dst, N, n_out, b, d, 3.2, 14 + d)
+    ctx = tp.Context(N, 4, 12, 3, 0, n_out, b, d)
+    ctx.import_lwe_ksk(ksk)
+    assert ctx.ks_mfma() == 1
+    lwes = np.random.default_rng(count + 7 * b + d).integers(0, Q, size=(count, N + 1), dtype=np.uint64)
+    lwes[1] = Q - 1
+    lwes[2] = 0
+    got = ctx.lwe_keyswitch_batch(lwes)
+    ctx.set_ks_mfma(0)
+    assert ctx.ks_mfma() == 0
+    vec = ctx.lwe_keyswitch_batch(lwes)
+    assert np.array_equal(got, vec)
+    for c in (0, 1, 2, count // 2, count - 1):
+        assert np.array_equal(got[c], oracle.lwe_keyswitch(lwes[c], ksk, N, n_out, b, d)), c
 
 
 @pytest.mark.parametrize("N,n_out,b,d", [(1024, 630, 4, 14), (2048, 800, 5, 12), (1024, 17, 16, 4)])
