@@ -504,6 +504,22 @@ def main():
     log("batch sweep: " + ", ".join("%d: %.1f/%.1f ms" % (r["batch"], r["one_wave_per_ct_ms"], r["digit_parallel_ms"])
                                     for r in sweep))
 
+    # the batch's key switch on the vector units (matrix cores off; the same words), for comparison
+    ks_vec_ms = None
+    if ctx.ks_mfma():
+        ctx.set_ks_mfma(0)
+        chk(lib.tp_lwe_keyswitch_batch_device(ctx.h, B, d_ext.p, N + 1, d_out.p))
+        chk(lib.tp_timer_mark(ctx.h, 1))
+        for _ in range(3):
+            chk(lib.tp_lwe_keyswitch_batch_device(ctx.h, B, d_ext.p, N + 1, d_out.p))
+        chk(lib.tp_timer_mark(ctx.h, 2))
+        t_ms = ctypes.c_float()
+        chk(lib.tp_timer_elapsed(ctx.h, 1, 2, ctypes.byref(t_ms)))
+        ks_vec_ms = t_ms.value / 3
+        ctx.set_ks_mfma(1)
+        log("key switch of the batch: %.3f ms on the matrix cores, %.3f ms on the vector units"
+            % (sum(ks_ms) / len(ks_ms), ks_vec_ms))
+
     # batch latency percentiles: every bootstrap of a batch completes with its batch
     lat_ms = sorted(br + ks for br, ks in zip(br_ms, ks_ms))
     p50 = float(np.percentile(lat_ms, 50))
@@ -538,7 +554,9 @@ def main():
         "latency_ms": {"batch_p50": round(p50, 3), "batch_p99": round(p99, 3),
                        "plain_mean": round(1000 * sum(plain_lat) / max(len(plain_lat), 1), 3)},
         "kernel_ms": {"blind_rotate_avg": round(1000 * br_avg_s, 3),
-                      "keyswitch_avg": round(sum(ks_ms) / len(ks_ms), 3)},
+                      "keyswitch_avg": round(sum(ks_ms) / len(ks_ms), 3),
+                      "keyswitch_engine": "matrix cores (int8, exact)" if ks_vec_ms is not None else "vector units",
+                      "keyswitch_vector_units_avg": None if ks_vec_ms is None else round(ks_vec_ms, 3)},
         "roofline": {"bound": "valu", "achieved": round(achieved, 4), "peak": round(peak, 4), "unit": "Tmulmod/s",
                      "frac": round(achieved / peak, 4), "traffic": traffic,
                      "kernel": "br_kernel<10,3,binary>", "work_per_launch_mulmod": w_br * B,
