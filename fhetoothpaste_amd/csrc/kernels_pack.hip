@@ -368,29 +368,51 @@ sap_tree_kernel(KeySwitchArgs A, const uint64_t* X, uint64_t* out, const uint64_
         // x and the polynomials and its writes are coalesced
         transpose<LOGN, Layout::kPass3, Layout::kPass1>(ya, lds, L);
         transpose<LOGN, Layout::kPass3, Layout::kPass1>(yb, lds, L);
+        // node arithmetic on canonical values with the VALU-layer primitives, eight coefficients of
+        // one half (a or b) per group
+        const uint64_t qm1 = glv::qm1_const();
         if (mode == 0) {
             uint64_t* ev = out + ((size_t)(2 * e) * count + i) * 2 * N;
             uint64_t* od = out + ((size_t)(2 * e + 1) * count + i) * 2 * N;
-#pragma unroll
-            for (int k = 0; k < kE; ++k) {
-                const int q = idx1<LOGN>(L, k);
-                const uint64_t xa = x[q], xb = x[N + q], sh = polys0[q];
-                ev[q] = gl::add(xa, ya[k]);
-                ev[N + q] = gl::add(xb, yb[k]);
-                od[q] = gl::mul(gl::sub(xa, ya[k]), sh);
-                od[N + q] = gl::mul(gl::sub(xb, yb[k]), sh);
-            }
+            static_for<4>([&](auto G) __attribute__((always_inline)) {
+                constexpr int h = G >> 1, k0 = 8 * (G & 1);
+                uint64_t xv[8], yv[8], sv[8], dv[8], mv[8], pv[8];
+                static_for<8>([&](auto M_) __attribute__((always_inline)) {
+                    const int q = idx1<LOGN>(L, k0 + M_);
+                    xv[M_] = x[h * N + q];
+                    yv[M_] = h ? yb[k0 + M_] : ya[k0 + M_];
+                    mv[M_] = polys0[q];
+                });
+                glv::add_cc_g<8>(xv, yv, sv, qm1);
+                glv::sub_lc_g<8>(xv, yv, dv);
+                glv::mul_canon_g<8>(dv, mv, pv, qm1);
+                static_for<8>([&](auto M_) __attribute__((always_inline)) {
+                    const int q = idx1<LOGN>(L, k0 + M_);
+                    ev[h * N + q] = sv[M_];
+                    od[h * N + q] = pv[M_];
+                });
+            });
         } else {
             uint64_t* o = out + (size_t)r * 2 * N;
             const uint64_t* me = polys0 + (size_t)(2 * e) * N;
             const uint64_t* mo = polys1 + (size_t)e * N;
-#pragma unroll
-            for (int k = 0; k < kE; ++k) {
-                const int q = idx1<LOGN>(L, k);
-                const uint64_t xa = x[q], xb = x[N + q], m0 = me[q], m1 = mo[q];
-                o[q] = gl::add(gl::mul(gl::add(xa, ya[k]), m0), gl::mul(gl::sub(xa, ya[k]), m1));
-                o[N + q] = gl::add(gl::mul(gl::add(xb, yb[k]), m0), gl::mul(gl::sub(xb, yb[k]), m1));
-            }
+            static_for<4>([&](auto G) __attribute__((always_inline)) {
+                constexpr int h = G >> 1, k0 = 8 * (G & 1);
+                uint64_t xv[8], yv[8], sv[8], dv[8], m0[8], m1[8], p0[8], p1[8], ov[8];
+                static_for<8>([&](auto M_) __attribute__((always_inline)) {
+                    const int q = idx1<LOGN>(L, k0 + M_);
+                    xv[M_] = x[h * N + q];
+                    yv[M_] = h ? yb[k0 + M_] : ya[k0 + M_];
+                    m0[M_] = me[q];
+                    m1[M_] = mo[q];
+                });
+                glv::add_cc_g<8>(xv, yv, sv, qm1);
+                glv::sub_lc_g<8>(xv, yv, dv);
+                glv::mul_canon_g<8>(sv, m0, p0, qm1);
+                glv::mul_canon_g<8>(dv, m1, p1, qm1);
+                glv::add_cc_g<8>(p0, p1, ov, qm1);
+                static_for<8>([&](auto M_) __attribute__((always_inline)) { o[h * N + idx1<LOGN>(L, k0 + M_)] = ov[M_]; });
+            });
         }
         poly_sync<S::WAVES>();
     }
