@@ -65,6 +65,7 @@ constexpr int kKsBlock = 256;
 // at N = 1024, d = 14, on 8 columns), while the row-split kernel reads the key once per ciphertext
 // (about 60 us; 0.62 ms for 64, 1.51 ms for 128 against 0.80 ms on 4 columns).
 constexpr int kKsRowSplitMax = 64;
+constexpr int kKsMfmaMin = 16;
 
 template <int COLS>
 __global__ void __launch_bounds__(kKsBlock) lwe_keyswitch_kernel(KsArgs A) {
@@ -262,8 +263,10 @@ hipError_t launch_blind_rotate(int logN, int method, int digits, const BrArgs& A
 }
 
 hipError_t launch_lwe_keyswitch(const KsArgs& A, hipStream_t s) {
-    // past the row-split sizes the matrix-core kernel, where the key is held in its operand form
-    if (A.kl && A.count > kKsRowSplitMax) return launch_lwe_keyswitch_mfma(A, s);
+    // the matrix-core kernel where the key is held in its operand form: its time is about 0.18 ms
+    // up to a few thousand ciphertexts at 1024 -> 630, d = 14 (the k loop), the row-split kernel's
+    // 0.06 ms + 9 us per ciphertext, so it takes launches of more than kKsMfmaMin
+    if (A.kl && A.count > kKsMfmaMin) return launch_lwe_keyswitch_mfma(A, s);
     if (A.count <= kKsRowSplitMax) {
         hipLaunchKernelGGL(lwe_keyswitch_one_kernel, dim3((A.n_out + 1 + kKsSplitCols - 1) / kKsSplitCols, A.count),
                            dim3(kKsBlock), 0, s, A);

@@ -243,7 +243,7 @@ class Context:
         return self._get(T.TP_OPT_BR_SPLIT_HALVES)
 
     def set_ks_mfma(self, on):
-        """1: LWE key switches of more than 64 ciphertexts run on the matrix cores where the key fits
+        """1: LWE key switches of more than 16 ciphertexts run on the matrix cores where the key fits
         that form, 0: on the vector units (same results)."""
         check(lib().tp_set_option(self.h, T.TP_OPT_KS_MFMA, int(on)))
 

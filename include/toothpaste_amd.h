@@ -301,7 +301,7 @@ int tp_graph_destroy(tp_graph* graph);
  * kernels_br_half.hip), 0 with one wave per digit transform (2 l waves); tp_get_option reports 1
  * only where the split kernel is built.  Results are the same either way. */
 #define TP_OPT_BR_SPLIT_HALVES 6
-/* TP_OPT_KS_MFMA: LWE key switches of more than 64 ciphertexts run on the matrix cores (1, default)
+/* TP_OPT_KS_MFMA: LWE key switches of more than 16 ciphertexts run on the matrix cores (1, default)
  * where the key fits that form (base 2^b with b <= 7, at most 16 digits, n_in d (2^b - 1) 255 < 2^31:
  * the key is then also held as int8 operands, about 1.2x its size), else (0) on the vector units;
  * tp_get_option reports 1 only where the operands are built.  Results are the same either way. */
