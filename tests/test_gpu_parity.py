@@ -226,10 +226,10 @@ def test_keyswitch_single_ciphertext(tp, oracle, N, n_out, b, d):
         assert np.array_equal(got[0], oracle.lwe_keyswitch(lwe[0], ksk, N, n_out, b, d)), trial
 
 
-def test_bootstrap_batch_cfg2_shape(tp, oracle):
+def test_bootstrap_batch_n630_N1024_shape(tp, oracle):
     """n=630, N=1024, b=12 l=3, KS 1024->630 b=4 d=14: bit-exact and decrypts."""
     N, n, b, l, n_out, kb, kd = 1024, 630, 12, 3, 630, 4, 14
-    say("bootstrap batch cfg2 shape")
+    say("bootstrap batch n=630 N=1024 shape")
     lk, rk, brk = _keys(oracle, N, n, b, l, oracle.BINARY, 0, 3.2, 11)
     ksk = oracle.lwe_ksk(rk, lk, N, n_out, kb, kd, 3.2, 11)
     ctx = tp.Context(N, n, b, l, 0, n_out, kb, kd)
