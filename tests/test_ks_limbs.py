@@ -102,7 +102,9 @@ def model(kl, lwes, n_in, n_out, b, dg, corr):
     return out
 
 
-@pytest.mark.parametrize("n_in,n_out,b,dg", [(16, 40, 4, 14), (16, 33, 5, 12), (8, 31, 4, 16), (8, 20, 7, 9)])
+@pytest.mark.parametrize("n_in,n_out,b,dg", [(16, 40, 4, 14), (16, 33, 5, 12), (8, 31, 4, 16), (8, 20, 7, 9),
+                                            (8, 20, 1, 16), (8, 63, 2, 16), (8, 32, 3, 16), (8, 64, 6, 10),
+                                            (16, 7, 4, 5)])
 def test_operands_reproduce_the_key_switch(dump, oracle, n_in, n_out, b, dg):
     rng = np.random.default_rng(n_in + 7 * b + dg)
     ksk = rng.integers(0, Q, size=n_in * dg * (n_out + 1), dtype=np.uint64)
@@ -115,7 +117,8 @@ def test_operands_reproduce_the_key_switch(dump, oracle, n_in, n_out, b, dg):
     got = model(kl, lwes, n_in, n_out, b, dg, corr)
     for c in range(lwes.shape[0]):
         assert np.array_equal(got[c], oracle.lwe_keyswitch(lwes[c], ksk, n_in, n_out, b, dg)), c
-    # a planted error (one key byte of one operand) changes an output
+    # a planted error (one key byte of one operand, in a slot that holds a digit) changes an output
     bad = kl.copy()
-    bad[3 * 16 + 5] ^= 1
+    used = [p for p in range(16) if slots[p] >= 0]
+    bad[3 * 16 + used[len(used) // 2]] ^= 1
     assert not np.array_equal(model(bad, lwes, n_in, n_out, b, dg, corr), got)
