@@ -29,7 +29,10 @@ def dump(tmp_path_factory):
         pytest.skip("needs g++")
     d = tmp_path_factory.mktemp("ks_limbs")
     exe = str(d / "dump")
-    subprocess.check_call(["g++", "-O2", "-std=c++17", "-Wall", "-Werror", "-I", CSRC, "-o", exe, SRC])
+    # host code under AddressSanitizer + UndefinedBehaviorSanitizer (the operand builder walks a large
+    # index space; any out-of-bounds access or overflow aborts the dump)
+    subprocess.check_call(["g++", "-O1", "-std=c++17", "-Wall", "-Werror", "-fsanitize=address,undefined",
+                           "-fno-sanitize-recover=all", "-I", CSRC, "-o", exe, SRC])
 
     def run(ksk, n_in, n_out, b, dg):
         kin, kout = str(d / "ksk.bin"), str(d / "kl.bin")

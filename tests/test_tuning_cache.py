@@ -15,7 +15,10 @@ def test_tuning_cache_file(tmp_path):
     if shutil.which("g++") is None:
         pytest.skip("needs g++")
     exe = str(tmp_path / "check")
-    subprocess.check_call(["g++", "-O1", "-std=c++17", "-Wall", "-Werror", "-I", CSRC, "-o", exe,
+    # the parser under AddressSanitizer + UndefinedBehaviorSanitizer (malformed input must not read past
+    # the text)
+    subprocess.check_call(["g++", "-O1", "-std=c++17", "-Wall", "-Werror", "-fsanitize=address,undefined",
+                           "-fno-sanitize-recover=all", "-I", CSRC, "-o", exe,
                            os.path.join(HERE, "cpp", "tuning_cache_check.cpp"), os.path.join(CSRC, "tuning_cache.cpp")])
     r = subprocess.run([exe, str(tmp_path)], capture_output=True, text=True)
     assert r.returncode == 0 and r.stdout.strip() == "ok", r.stdout + r.stderr
