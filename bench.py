@@ -301,6 +301,24 @@ def extra_workloads(tp, reps=2):
         b.free()
     ctx.close()
 
+    # batched negacyclic NTT (forward and inverse, in place) of 16384 polynomials at N = 1024 and 2048
+    out["ntt_batch"] = {}
+    for NN in (1024, 2048):
+        P = 16384
+        ctx = tp.Context(NN, 4, 12, 3, 0)
+        d = DeviceBuffer.from_host(ctx, rng.integers(0, Q, size=P * NN, dtype=np.uint64))
+        ms_f = timed(ctx, lib, chk, lambda: chk(lib.tp_ntt_forward_device(ctx.h, d.p, d.p, P)), reps)
+        ms_i = timed(ctx, lib, chk, lambda: chk(lib.tp_ntt_inverse_device(ctx.h, d.p, d.p, P)), reps)
+        butterflies = P * (NN // 2) * (NN.bit_length() - 1)
+        out["ntt_batch"][str(NN)] = {
+            "polynomials": P, "forward_ms": round(ms_f, 3), "inverse_ms": round(ms_i, 3),
+            "forward_per_s": round(P / ms_f * 1000, 1), "inverse_per_s": round(P / ms_i * 1000, 1),
+            "forward_GBps": round(16 * NN * P / ms_f / 1e6, 1),
+            "forward_butterflies_per_s": round(butterflies / ms_f * 1000, 1)}
+        log("NTT N=%d: %d polynomials, forward %.3f ms, inverse %.3f ms" % (NN, P, ms_f, ms_i))
+        d.free()
+        ctx.close()
+
     # wire format (host code): encode and decode of a blind-rotation key of the headline shape
     # (binary n=630, N=1024, l=3: 7,741,440 words), header checks and FNV-1a checksum included
     from fhetoothpaste_amd import wire
