@@ -394,7 +394,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=BATCH)
-    ap.add_argument("--plain-steps", type=int, default=3)
+    ap.add_argument("--plain-steps", type=int, default=20)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=60.0,
@@ -570,7 +570,9 @@ def main():
         "plain_bootstraps_per_s": round(plain_bs, 2),
         "batched_vs_plain": round(value / plain_bs, 2),
         "latency_ms": {"batch_p50": round(p50, 3), "batch_p99": round(p99, 3),
-                       "plain_mean": round(1000 * sum(plain_lat) / max(len(plain_lat), 1), 3)},
+                       "plain_mean": round(1000 * sum(plain_lat) / max(len(plain_lat), 1), 3),
+                       "plain_p50": round(1000 * float(np.percentile(plain_lat, 50)), 3) if plain_lat else None,
+                       "plain_p99": round(1000 * float(np.percentile(plain_lat, 99)), 3) if plain_lat else None},
         "kernel_ms": {"blind_rotate_avg": round(1000 * br_avg_s, 3),
                       "keyswitch_avg": round(sum(ks_ms) / len(ks_ms), 3),
                       "keyswitch_engine": "matrix cores (int8, exact)" if ks_vec_ms is not None else "vector units",
